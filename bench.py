@@ -1,0 +1,148 @@
+"""Benchmark of the HIP raycasting path on one MI355X.
+
+Workload: 1024^3 uint16 fBm + 64 hollow shells, bricked into 8^3 bricks of
+128^3 core + 1 overlap voxel (130^3 stored), lit 2D-TF raycast (4 Gaussian
+lobes over value x gradient magnitude), 1920x1080, 16 views on an orbit,
+0.5-voxel step (the reference's fRayStepsize), FP32 compositing.
+One step = the 16 views rendered in one launch with inputs resident in HBM.
+
+Metric: Gsamples/s = N_ref / t, where N_ref is the number of composited
+samples of the fixed-step reference march (per-brick restart, local
+a >= 0.99 exit, no cross-brick termination) counted on the device in an
+untimed pass; frames/s = 16 / t.  The timed pass uses early ray termination
+at a total transmittance of 2^-11 (error <= 2^-11 per channel).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+ERT_EPS = 2.0 ** -11
+
+
+def cpu_baseline(ren, vol, views, threads):
+    """The oracle (CPU restatement of the reference raycaster) on a band of rows
+    of the first view, timed on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    from helpers import oracle_camera_from, oracle_params_from
+    from tuvok_amd.transfer import tf2d_gaussian_lobes
+    cfg_brick = ren.GetDatasetDesc().brick_size
+    ds = O.Dataset(vol, tuple(cfg_brick), 1, lod0_only=True)
+    ren.SetRotation(views[0])
+    cam = oracle_camera_from(ren, views[0])
+    p = oracle_params_from(ren, tf2d=tf2d_gaussian_lobes().GetByteArray())
+    W, H = cam["width"], cam["height"]
+    rows = 24
+    roi = (0, H // 2 - rows // 2, W, H // 2 + rows // 2)
+    t0 = time.perf_counter()
+    _, ns = O.render(ds, cam, p, roi=roi, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=ns / dt / 1e9, unit="Gsamples/s", cores=threads, kind="port",
+                sample=f"oracle (GL-faithful FP32 restatement) on rows {roi[1]}..{roi[3] - 1} of view 0 "
+                       f"({W}x{rows} rays, {ns} samples, {dt:.1f} s, {threads} threads)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--size", type=int, default=1024, help="volume edge (default: the 1024^3 workload)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
+              f"MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from tuvok_amd import workloads
+    cfg = dict(workloads.FBM_SHELLS_1024)
+    n = args.size
+    dims = (n, n, n)
+    t0 = time.perf_counter()
+    ren = workloads.setup(cfg, dims=dims, keep_linear=True)
+    print(f"[bench] dataset ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    views = workloads.orbit(cfg["views"])
+    nviews = len(views)
+
+    # untimed counting pass: N_ref with the reference semantics (no cross-brick termination)
+    ren.SetEarlyRayTermination(0.0)
+    ren.SetCountSamples(True)
+    ren.PaintViews(views, download=False)
+    n_ref = int(ren.last_stats.samples)
+    b_bricks, n_touched = ren.TouchedBytes()
+    exact_ms = ren.last_stats.ms
+    # executed samples with early ray termination
+    ren.SetEarlyRayTermination(ERT_EPS)
+    ren.PaintViews(views, download=False)
+    n_exec = int(ren.last_stats.samples)
+    ren.SetCountSamples(False)
+    print(f"[bench] N_ref={n_ref} executed={n_exec} touched bricks={n_touched} ({b_bricks / 1e9:.2f} GB)",
+          file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        ren.PaintViews(views, download=False)
+    ren.Synchronize()
+    kernel_ms = []
+    t_wall = time.perf_counter()
+    ren.EventRecord(0)
+    for _ in range(args.steps):
+        ren.PaintViews(views, download=False)
+        kernel_ms.append(ren.EventElapsed(6, 7))
+    ren.EventRecord(1)
+    ren.Synchronize()
+    wall = time.perf_counter() - t_wall
+    total_ms = ren.EventElapsed(0, 1)
+    ms_step = total_ms / args.steps
+    k_ms = float(np.mean(kernel_ms))
+    pixels = cfg["width"] * cfg["height"] * nviews
+    alg_bytes = b_bricks + pixels * 16 + pixels * 4
+    achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    value = n_ref / (ms_step / 1e3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        vol = ren.DownloadVolume()
+        thr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        cpu = cpu_baseline(ren, vol, views, thr)
+
+    out = {
+        "metric": "Gsamples/s (fixed-step-equivalent) + frames/s @1080p",
+        "value": round(value, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16 voxels, f32 compositing",
+        "data": "synthetic (seeded fBm + shells generated on the device)",
+        "config": {"workload": f"fbm_shells{n}_u16_2dtf_lit_1920x1080_x{nviews}views",
+                   "volume": list(dims), "brick_stored": list(cfg["brick"]), "overlap": cfg["overlap"],
+                   "mode": "2D TF + lighting", "step_voxels": "0.5 (fRayStepsize)", "views": nviews,
+                   "ert_eps": ERT_EPS},
+        "frames_per_s": round(nviews / (ms_step / 1e3), 2),
+        "samples_ref": n_ref,
+        "samples_executed": n_exec,
+        "exact_mode_ms_per_step": round(exact_ms, 3),
+        "wall_s": round(wall, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(alg_bytes), "kernel_ms": round(k_ms, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

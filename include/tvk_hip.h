@@ -1,0 +1,191 @@
+/* tvk_hip.h -- C ABI of the MI355X-native Tuvok raycasting path.
+ *
+ * One context = one GPU + one HIP stream.  The library owns all device memory;
+ * the caller owns every host buffer it passes.  Every call returns TVK_OK (0)
+ * or a negative error code; tvk_last_error() gives the message.  No C++
+ * exception crosses this boundary.  A context is single-threaded (like the
+ * GL context that owns a Tuvok renderer).
+ *
+ * The entry points replace the following reference interfaces:
+ *   tvk_load_volume / tvk_generate_volume
+ *       RAWConverter::ConvertRAWDataset -> ExtendedOctreeConverter::Convert
+ *       (IO/RAWConverter.h:79-100, IO/UVF/ExtendedOctree/ExtendedOctreeConverter.cpp:128-280)
+ *       and the upload half of GPUMemMan::GetVolume (Renderer/GPUMemMan/GPUMemMan.cpp:846-996)
+ *   tvk_lod_info        ExtendedOctree::GetLoDSize/GetBrickCount/GetBrickAspect
+ *                       (IO/UVF/ExtendedOctree/ExtendedOctree.cpp:255-294)
+ *   tvk_brick_md        BrickedDataset::GetBrickMetadata (IO/BrickedDataset.cpp:52-67),
+ *                       filled as UVFDataset::ComputeMetadataTOC (IO/uvfDataset.cpp:242-288)
+ *   tvk_brick_minmax    UVFDataset::MaxMinForKey (IO/uvfDataset.cpp:1183-1199)
+ *   tvk_get_brick       Dataset::GetBrick (IO/Dataset.h:93-100)
+ *   tvk_set_tf1d        AbstrRenderer::Set1DTrans (Renderer/AbstrRenderer.h:189)
+ *   tvk_set_tf2d        AbstrRenderer::Changed2DTrans (Renderer/AbstrRenderer.h:195)
+ *   tvk_set_camera      AbstrRenderer::SetViewParameters + GLRenderer::ComputeViewAndProjection
+ *                       (Renderer/AbstrRenderer.h:250-261, Renderer/GL/GLRenderer.cpp:889-920)
+ *   tvk_set_view_matrices  AbstrRenderer::SetUserMatrices + RenderRegion rotation/translation
+ *                       (Renderer/AbstrRenderer.h:236-238, Renderer/RenderRegion.h:50-196)
+ *   tvk_set_params      SetRendermode / SetUseLighting / SetSampleRateModifier / SetIsoValue
+ *                       (Renderer/AbstrRenderer.h:323, :414-418)
+ *   tvk_plan            AbstrRenderer::PlanFrame + BuildSubFrameBrickList
+ *                       (Renderer/AbstrRenderer.cpp:999-1100, :1125-1212)
+ *   tvk_render          GLRenderer::Paint -> Render3DRegion -> Render3DView with
+ *                       GLRaycaster::Render3DInLoop (Renderer/GL/GLRenderer.cpp:571-667,
+ *                       :2663-2748; Renderer/GL/GLRaycaster.cpp:383-478); the output is the
+ *                       premultiplied RGBA32F image of GLRenderer::GetLastFBO (GLRenderer.h:105)
+ */
+#ifndef TVK_HIP_H
+#define TVK_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tvk_ctx tvk_ctx;
+
+enum {
+  TVK_OK = 0,
+  TVK_ERR_ARG = -1,      /* invalid argument                                   */
+  TVK_ERR_HIP = -2,      /* HIP runtime error                                  */
+  TVK_ERR_STATE = -3,    /* call out of order (no dataset, no TF, ...)         */
+  TVK_ERR_RANGE = -4,    /* index/key out of range (BrickedDataset domain_error)*/
+  TVK_ERR_NOMEM = -5,    /* device allocation failed                           */
+  TVK_ERR_UNSUPPORTED = -6
+};
+
+enum { TVK_U8 = 1, TVK_U16 = 2 };                          /* bytes per voxel */
+enum { TVK_RM_1DTRANS = 0, TVK_RM_2DTRANS = 1, TVK_RM_ISOSURFACE = 2 }; /* ERenderMode */
+
+/* synthetic volume generators (tvk_generate_volume) */
+enum {
+  TVK_GEN_BLOBS = 1,      /* sum of isotropic Gaussians + integer noise         */
+  TVK_GEN_FBM_SHELLS = 2, /* 3-octave value-noise fBm + hollow spherical shells  */
+  TVK_GEN_ELLIPSOID_SDF = 3, /* clamp(128 + 20*d_voxels) of random ellipsoids    */
+  TVK_GEN_PHANTOM = 4     /* nested ellipsoids, piecewise constant + noise       */
+};
+
+typedef struct {
+  int mode;              /* TVK_RM_*                                               */
+  int lighting;          /* m_bUseLighting                                          */
+  float srm;             /* m_fSampleRateModifier                                   */
+  float isovalue;        /* m_fIsovalue, data units                                 */
+  int lod;               /* LOD to render (<0: ComputeMinLODForCurrentView)          */
+  float tf_scale;        /* fTransScale; <=0 -> computed as CalculateScaling()      */
+  float grad_scale;      /* fGradientScale = 1/MaxGradientMagnitude                  */
+  float domain_scale[3]; /* vDomainScale = 1/GetScale()                              */
+  float light_ambient[3], light_diffuse[3], light_specular[3], light_dir[3];
+  float iso_color[3];
+  int refine_steps;      /* RefineIsosurface iterations (reference: 5)               */
+  int clamp_to_edge;     /* 0: GL_CLAMP (border 0, bDisableBorder=false), 1: CLAMP_TO_EDGE */
+  float ert_eps;         /* 0: exact reference semantics.  >0: a ray stops once its
+                            total transmittance is below ert_eps (error <= ert_eps
+                            per channel); decided per wavefront by ballot.         */
+  float screen_space_error; /* CullingLOD m_fScreenSpaceError (for lod < 0)          */
+} tvk_params;
+
+typedef struct {
+  uint64_t key_index;    /* BrickKey 1D index within the LOD                    */
+  uint64_t toc_index;    /* index into the brick pool (TOC order)               */
+  float center[3], extension[3];
+  uint32_t voxel_count[3];
+  float tex_min[3], tex_max[3];
+  float distance;        /* Brick::fDistance                                    */
+  int is_empty;          /* Brick::bIsEmpty                                     */
+} tvk_brick_rec;
+
+typedef struct {
+  int lod_count;
+  uint64_t brick_count;  /* all LODs                                            */
+  uint64_t pool_bytes;   /* HBM bytes of the brick pool                         */
+  double range_min, range_max;   /* UVFDataset::GetRange                        */
+  int bit_width;
+  uint64_t dims[3];
+  uint32_t brick_size[3];
+  uint32_t overlap;
+} tvk_dataset_desc;
+
+typedef struct {
+  uint64_t samples;      /* composited samples executed (when counting is on)   */
+  uint32_t bricks_in_list, bricks_nonempty;
+  float ms;              /* device time of the last tvk_render call             */
+} tvk_stats;
+
+/* ---- context ------------------------------------------------------------ */
+int tvk_create(tvk_ctx** out, int device);
+void tvk_destroy(tvk_ctx* ctx);
+const char* tvk_last_error(const tvk_ctx* ctx);
+int tvk_default_params(tvk_params* p);   /* AbstrRenderer constructor defaults */
+
+/* ---- dataset ------------------------------------------------------------ */
+/* Brick a linear volume (x fastest) into the ExtendedOctree layout in HBM:
+ * brick = stored brick size incl. 2*overlap, LOD pyramid by 2x box filter
+ * (mean, truncating), per-brick min/max over all stored voxels.  max_lods = 0
+ * builds every level down to 1x1x1 like the reference; k > 0 keeps the first
+ * k levels.  keep_linear retains the linear LOD0 volume for tvk_download_volume. */
+int tvk_load_volume(tvk_ctx* ctx, const void* host_voxels, int dtype, const uint64_t dims[3],
+                    const uint32_t brick[3], uint32_t overlap, int clamp_to_edge, int max_lods,
+                    int keep_linear);
+/* Same, with the linear volume generated on the device (TVK_GEN_*). */
+int tvk_generate_volume(tvk_ctx* ctx, int kind, uint64_t seed, int dtype, const uint64_t dims[3],
+                        const uint32_t brick[3], uint32_t overlap, int clamp_to_edge,
+                        int max_lods, int keep_linear);
+/* Copy the retained linear LOD0 volume back to the host. */
+int tvk_download_volume(tvk_ctx* ctx, void* host_out, size_t bytes);
+int tvk_dataset_info(tvk_ctx* ctx, tvk_dataset_desc* out);
+int tvk_lod_info(tvk_ctx* ctx, int lod, uint64_t size[3], uint64_t bricks[3], uint64_t* offset,
+                 double aspect[3]);
+int tvk_brick_info(tvk_ctx* ctx, uint64_t toc_index, uint64_t coords[4], uint32_t size[3],
+                   uint64_t* pool_offset);
+int tvk_brick_md(tvk_ctx* ctx, int lod, uint64_t index, float center[3], float extents[3],
+                 uint32_t n_voxels[3]);
+int tvk_brick_minmax(tvk_ctx* ctx, uint64_t toc_index, double minmax[2]);
+int tvk_get_brick(tvk_ctx* ctx, uint64_t toc_index, void* host_out, size_t bytes);
+int tvk_get_pool(tvk_ctx* ctx, void* host_out, size_t bytes);
+/* MaxGradientMagnitude over LOD0 core voxels (Histogram2DDataBlock.cpp:349-390) */
+int tvk_max_gradient(tvk_ctx* ctx, float* out);
+
+/* ---- transfer functions --------------------------------------------------- */
+int tvk_set_tf1d(tvk_ctx* ctx, const uint8_t* rgba8, int n);
+int tvk_set_tf2d(tvk_ctx* ctx, const uint8_t* rgba8, int w, int h);
+
+/* ---- view ------------------------------------------------------------------ */
+/* Row-vector matrices in Basics/Vectors.h layout (array[0]=m11, array[12..14] = translation);
+ * modelView = rot * trans * view (GLRenderer.cpp:627). */
+int tvk_set_view_matrices(tvk_ctx* ctx, const float view[16], const float rot[16],
+                          const float trans[16], const float proj[16], uint32_t width,
+                          uint32_t height, float fov, float znear, float zfar);
+/* BuildLookAt + Perspective (Vectors.h:1250-1277); rotation/translation = identity */
+int tvk_set_camera(tvk_ctx* ctx, const float eye[3], const float at[3], const float up[3], float fov,
+                   float znear, float zfar, uint32_t width, uint32_t height);
+/* RenderRegion rotation (e.g. an orbit); translation stays as set */
+int tvk_set_rotation(tvk_ctx* ctx, const float rot[16]);
+int tvk_get_view_matrices(tvk_ctx* ctx, float view[16], float proj[16]);
+
+/* ---- parameters, planning, rendering -------------------------------------- */
+int tvk_set_params(tvk_ctx* ctx, const tvk_params* p);
+int tvk_get_params(tvk_ctx* ctx, tvk_params* p);
+int tvk_current_lod(tvk_ctx* ctx, int* lod);
+int tvk_plan(tvk_ctx* ctx, tvk_brick_rec* out, int cap, int* n);
+/* Render one frame.  host_out (may be NULL) receives W*H RGBA32F, premultiplied,
+ * row 0 = bottom (GL window order). */
+int tvk_render(tvk_ctx* ctx, float* host_out, tvk_stats* stats);
+/* Render n views (rotation matrices rots[n*16]) back to back in one launch per view
+ * on the context stream; host_out (may be NULL) receives n frames. */
+int tvk_render_views(tvk_ctx* ctx, int n, const float* rots, float* host_out, tvk_stats* stats);
+/* Enable/disable the per-frame sample counter and touched-brick record (off by
+ * default: costs an atomic per wave). */
+int tvk_set_count_samples(tvk_ctx* ctx, int on);
+/* Stored bytes (and count) of the bricks in which the last counted render took at
+ * least one sample, summed over its views. */
+int tvk_touched_bytes(tvk_ctx* ctx, uint64_t* bytes, uint64_t* bricks);
+/* Device pointer of the last rendered frame(s) (for zero-copy consumers). */
+int tvk_frame_device_ptr(tvk_ctx* ctx, void** dptr);
+
+/* ---- timing on the context stream ------------------------------------------- */
+int tvk_event_record(tvk_ctx* ctx, int slot);          /* slot in [0, 8) */
+int tvk_event_elapsed(tvk_ctx* ctx, int a, int b, float* ms);
+int tvk_synchronize(tvk_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TVK_HIP_H */

@@ -1,0 +1,255 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Bit-exact: brick payloads, overlap voxels, per-brick min/max, brick table,
+visible-brick list.  Rendered RGBA: max |diff| <= 1/255 per channel and
+mean < 1e-3 (the HIP march repeats the oracle's float operations, so the
+observed difference is ~1e-7; the tolerance is the acceptance gate).
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from helpers import blobs_volume, err_stats, oracle_camera_from, oracle_params_from
+from tuvok_amd import (GEN_BLOBS, GEN_FBM_SHELLS, RM_1DTRANS, RM_2DTRANS, RM_ISOSURFACE, HIPRaycaster,
+                       rotation_y, tf1d_blue_white_red, tf2d_gaussian_lobes)
+
+pytestmark = pytest.mark.gpu
+
+TOL_MAX = 1.0 / 255.0
+TOL_MEAN = 1e-3
+
+
+@pytest.fixture(scope="module")
+def ren():
+    r = HIPRaycaster(0)
+    yield r
+    r.close()
+
+
+def lod0_bricks(ds):
+    return [i for i in range(ds.brick_count) if ds.brick_info(i)["coords"][3] == 0]
+
+
+BRICK_CASES = [
+    ((64, 64, 64), (18, 18, 18), 1, False, np.uint8),
+    ((37, 21, 50), (12, 10, 16), 1, False, np.uint16),
+    ((40, 36, 44), (12, 10, 10), 2, True, np.uint16),
+    ((33, 5, 1), (12, 6, 6), 1, False, np.uint8),
+]
+
+
+@pytest.mark.parametrize("dims,brick,ov,clamp,dt", BRICK_CASES)
+def test_device_bricking_bit_exact(ren, dims, brick, ov, clamp, dt):
+    print(f"bricking dims={dims} brick={brick} ov={ov} clamp={clamp} dtype={np.dtype(dt).name}", flush=True)
+    rng = np.random.default_rng(11)
+    full = 255 if dt == np.uint8 else 65535
+    vol = rng.integers(0, full + 1, size=dims[::-1]).astype(dt)
+    ds = O.Dataset(vol, brick, ov, clamp=clamp)
+    ren.LoadVolume(vol, brick, ov, clamp_to_edge=clamp, max_lods=1)
+    d = ren.GetDatasetDesc()
+    assert d.brick_count == len(lod0_bricks(ds))
+    pool = ren.GetPool()
+    ref = ds.all_data()
+    n0 = sum(int(np.prod(ds.brick_info(i)["size"])) for i in lod0_bricks(ds)) * vol.itemsize
+    assert pool.nbytes == n0
+    assert np.array_equal(pool, ref[:n0])
+    for i in lod0_bricks(ds):
+        assert ren.MaxMinForKey(i) == ds.brick_info(i)["minmax"]
+        assert ren.GetBrickInfo(i)["size"] == ds.brick_info(i)["size"]
+        c = ds.brick_info(i)["coords"]
+        bc = ren.GetLODInfo(0)["bricks"]
+        lin = c[0] + c[1] * bc[0] + c[2] * bc[0] * bc[1]
+        gc, ge, gn = ren.GetBrickMetadata(0, lin)
+        oc, oe, on = ds.brick_md(0, lin)
+        assert np.array_equal(gc, oc) and np.array_equal(ge, oe) and np.array_equal(gn, on)
+    assert d.range_max == ds.range_max()
+
+
+@pytest.mark.parametrize("dims,brick,dt", [((44, 36, 40), (10, 10, 10), np.uint8),
+                                           ((70, 50, 90), (12, 12, 16), np.uint16)])
+def test_device_lod_pyramid_bit_exact(ren, dims, brick, dt):
+    """All levels, overlap voxels included (FillOverlap's copy order reproduced)."""
+    print(f"lod pyramid dims={dims} brick={brick}", flush=True)
+    full = 255 if dt == np.uint8 else 65535
+    vol = np.random.default_rng(5).integers(0, full + 1, size=dims[::-1]).astype(dt)
+    ds = O.Dataset(vol, brick, 1)
+    ren.LoadVolume(vol, brick, 1, max_lods=0)
+    pool = ren.GetPool()
+    ref = ds.all_data()
+    assert pool.nbytes == ref.nbytes
+    assert np.array_equal(pool, ref)
+    for i in range(ds.brick_count):
+        assert ren.MaxMinForKey(i) == ds.brick_info(i)["minmax"]
+
+
+def test_device_lod_pyramid_clamped_cores(ren):
+    """With clamp-to-edge the coarse levels' cores match; a few overlap corners of the
+    reference hold stale buffer contents (not reproduced)."""
+    vol = np.random.default_rng(6).integers(0, 256, size=(44, 36, 40)).astype(np.uint8)
+    ds = O.Dataset(vol, (10, 10, 10), 1, clamp=True)
+    ren.LoadVolume(vol, (10, 10, 10), 1, clamp_to_edge=True, max_lods=0)
+    pool = ren.GetPool()
+    ref = ds.all_data()
+    for i in range(ds.brick_count):
+        info = ds.brick_info(i)
+        nx, ny, nz = info["size"]
+        g = pool[info["offset"]:info["offset"] + nx * ny * nz].reshape(nz, ny, nx)
+        r = ref[info["offset"]:info["offset"] + nx * ny * nz].reshape(nz, ny, nx)
+        assert np.array_equal(g[1:-1, 1:-1, 1:-1], r[1:-1, 1:-1, 1:-1]), info
+        if info["coords"][3] == 0:
+            assert np.array_equal(g, r), info
+
+
+def setup_blobs(ren, n=64, brick=18, w=96, h=80, eye=(0.2, -0.1, 2.0), dtype=np.uint8):
+    ren.GenerateVolume(GEN_BLOBS, 0, dtype, (n, n, n), (brick,) * 3, 1, max_lods=1, keep_linear=True)
+    vol = ren.DownloadVolume()
+    tf = tf1d_blue_white_red().GetByteArray()
+    ren.Set1DTrans(tf)
+    ren.SetViewParameters(45.0, 0.01, 1000.0)
+    ren.SetViewPosition(eye)
+    ren.Resize(w, h)
+    return vol, tf
+
+
+def compare(ren, ds, tf1d=None, tf2d=None, rot=None):
+    cam = oracle_camera_from(ren, rot)
+    p = oracle_params_from(ren, tf1d=tf1d, tf2d=tf2d)
+    ref, ns = O.render(ds, cam, p)
+    ren.SetCountSamples(True)
+    if rot is not None:
+        ren.SetRotation(rot)
+    img = ren.Paint()
+    mx, mean = err_stats(img, ref)
+    return img, ref, mx, mean, ns, ren.last_stats.samples
+
+
+def test_plan_matches_oracle(ren):
+    vol, tf = setup_blobs(ren)
+    ren.SetRendermode(RM_1DTRANS)
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    for rot in (None, rotation_y(0.7)):
+        if rot is not None:
+            ren.SetRotation(rot)
+        got = ren.PlanFrame()
+        cam = oracle_camera_from(ren, rot)
+        want = O.plan(ds, cam, oracle_params_from(ren, tf1d=tf))
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            assert (g.key_index, g.toc_index, g.is_empty) == (w.key_index, w.toc_index, w.is_empty)
+            assert g.distance == w.distance
+            assert list(g.center) == list(w.center) and list(g.extension) == list(w.extension)
+            assert list(g.tex_min) == list(w.tex_min) and list(g.tex_max) == list(w.tex_max)
+    ren.SetRotation(np.eye(4, dtype=np.float32))
+
+
+@pytest.mark.parametrize("mode,lit", [(RM_1DTRANS, False), (RM_1DTRANS, True), (RM_2DTRANS, False),
+                                      (RM_2DTRANS, True), (RM_ISOSURFACE, False)])
+def test_render_modes_match_oracle(ren, mode, lit):
+    print(f"render mode={mode} lit={lit}", flush=True)
+    vol, tf = setup_blobs(ren)
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    tf2 = tf2d_gaussian_lobes().GetByteArray()
+    ren.Set2DTrans(tf2)
+    ren.SetRendermode(mode)
+    ren.SetUseLighting(lit)
+    ren.SetGradientScale(1.0 / max(ren.MaxGradientMagnitude(), 1e-6))
+    ren.SetIsoValue(128.0)
+    ren.SetEarlyRayTermination(0.0)
+    img, ref, mx, mean, ns_ref, ns_gpu = compare(ren, ds, tf1d=tf, tf2d=tf2)
+    print(f"  max {mx:.3g} mean {mean:.3g} samples oracle {ns_ref} gpu {ns_gpu}", flush=True)
+    assert ref[..., 3].max() > 0.05
+    assert mx <= TOL_MAX and mean < TOL_MEAN
+    assert ns_gpu == ns_ref
+
+
+def test_blobs_256_full_frame(ren):
+    """The 256^3 uint8 1D-TF workload at 512x512, whole frame against the oracle."""
+    print("blobs 256^3, 512x512", flush=True)
+    vol, tf = setup_blobs(ren, n=256, brick=66, w=512, h=512, eye=(0.0, 0.0, 2.0))
+    ren.SetRendermode(RM_1DTRANS)
+    ren.SetUseLighting(False)
+    ren.SetEarlyRayTermination(0.0)
+    ds = O.Dataset(vol, (66, 66, 66), 1)
+    img, ref, mx, mean, ns_ref, ns_gpu = compare(ren, ds, tf1d=tf)
+    print(f"  max {mx:.3g} mean {mean:.3g} samples oracle {ns_ref} gpu {ns_gpu}", flush=True)
+    assert mx <= TOL_MAX and mean < TOL_MEAN
+    assert ns_gpu == ns_ref
+    fp64 = O.render_fp64(ds, oracle_camera_from(ren), oracle_params_from(ren, tf1d=tf), 64 / 66 / 256 * 0.5)
+    m64, a64 = err_stats(img, fp64)
+    print(f"  vs FP64 integral: max {m64:.3g} mean {a64:.3g}", flush=True)
+
+
+def test_early_termination_bound(ren):
+    vol, tf = setup_blobs(ren, eye=(0.0, 0.0, 1.7))
+    ren.SetRendermode(RM_1DTRANS)
+    ren.SetUseLighting(False)
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    ren.SetEarlyRayTermination(0.0)
+    exact = ren.Paint()
+    for eps in (1 / 2048, 1 / 512):
+        ren.SetEarlyRayTermination(eps)
+        cut = ren.Paint()
+        assert np.abs(cut - exact).max() <= eps + 1e-6
+        cam = oracle_camera_from(ren)
+        ref, _ = O.render(ds, cam, oracle_params_from(ren, tf1d=tf))
+        assert np.abs(cut - ref).max() <= 1e-5
+    ren.SetEarlyRayTermination(0.0)
+
+
+def test_views_batch_equals_single(ren):
+    setup_blobs(ren)
+    ren.SetRendermode(RM_1DTRANS)
+    ren.SetUseLighting(True)
+    rots = [rotation_y(a) for a in (0.0, 0.9, 2.1, 4.0)]
+    batch = ren.PaintViews(rots)
+    for k, r in enumerate(rots):
+        ren.SetRotation(r)
+        assert np.array_equal(ren.Paint(), batch[k])
+    ren.SetRotation(np.eye(4, dtype=np.float32))
+
+
+def test_fbm_shells_u16_lit_2d_crop(ren):
+    """uint16 fBm+shells, lit 2D TF, 2 views; a crop of each frame against the oracle."""
+    n = 128
+    ren.GenerateVolume(GEN_FBM_SHELLS, 1, np.uint16, (n, n, n), (34, 34, 34), 1, max_lods=1, keep_linear=True)
+    vol = ren.DownloadVolume()
+    tf2 = tf2d_gaussian_lobes().GetByteArray()
+    ren.Set2DTrans(tf2)
+    ren.SetRendermode(RM_2DTRANS)
+    ren.SetUseLighting(True)
+    g = ren.MaxGradientMagnitude()
+    ren.SetGradientScale(1.0 / g)
+    ren.SetEarlyRayTermination(0.0)
+    ren.SetViewParameters(45.0, 0.01, 1000.0)
+    ren.SetViewPosition((0.0, 0.0, 2.0))
+    ren.Resize(192, 108)
+    ds = O.Dataset(vol, (34, 34, 34), 1)
+    for rot in (rotation_y(0.0), rotation_y(1.3)):
+        img, ref, mx, mean, ns_ref, ns_gpu = compare(ren, ds, tf2d=tf2, rot=rot)
+        print(f"  fbm rot: max {mx:.3g} mean {mean:.3g} samples {ns_ref}/{ns_gpu}", flush=True)
+        assert ref[..., 3].max() > 0.05
+        assert mx <= TOL_MAX and mean < TOL_MEAN
+    ren.SetRotation(np.eye(4, dtype=np.float32))
+
+
+def test_full_size_bricking_properties(ren):
+    """1024^3 uint16 with 128^3 cores: every brick's core equals the linear volume,
+    overlaps equal the neighbours (zero outside), min/max match the payload."""
+    n, b = 1024, 130
+    print("full-size bricking 1024^3 u16", flush=True)
+    ren.GenerateVolume(GEN_FBM_SHELLS, 1, np.uint16, (n, n, n), (b, b, b), 1, max_lods=1, keep_linear=True)
+    vol = ren.DownloadVolume()
+    pool = ren.GetPool().view(np.uint16)
+    pad = np.zeros((n + 2, n + 2, n + 2), np.uint16)
+    pad[1:-1, 1:-1, 1:-1] = vol
+    off = 0
+    for z in range(8):
+        for y in range(8):
+            for x in range(8):
+                br = pool[off:off + b ** 3].reshape(b, b, b)
+                want = pad[z * 128:z * 128 + b, y * 128:y * 128 + b, x * 128:x * 128 + b]
+                assert np.array_equal(br, want), (x, y, z)
+                toc = x + 8 * y + 64 * z
+                assert ren.MaxMinForKey(toc) == (float(want.min()), float(want.max()))
+                off += b ** 3
+    assert off == pool.size

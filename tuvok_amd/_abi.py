@@ -1,0 +1,112 @@
+"""ctypes view of the C ABI in include/tvk_hip.h (libtvkhip.so)."""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtvkhip.so")
+
+TVK_OK = 0
+TVK_U8, TVK_U16 = 1, 2
+RM_1DTRANS, RM_2DTRANS, RM_ISOSURFACE = 0, 1, 2
+GEN_BLOBS, GEN_FBM_SHELLS, GEN_ELLIPSOID_SDF, GEN_PHANTOM = 1, 2, 3, 4
+
+# every symbol include/tvk_hip.h declares
+EXPORTS = [
+    "tvk_create", "tvk_destroy", "tvk_last_error", "tvk_default_params", "tvk_load_volume",
+    "tvk_generate_volume", "tvk_download_volume", "tvk_dataset_info", "tvk_lod_info", "tvk_brick_info",
+    "tvk_brick_md", "tvk_brick_minmax", "tvk_get_brick", "tvk_get_pool", "tvk_max_gradient",
+    "tvk_set_tf1d", "tvk_set_tf2d", "tvk_set_view_matrices", "tvk_set_camera", "tvk_set_rotation",
+    "tvk_get_view_matrices", "tvk_set_params", "tvk_get_params", "tvk_current_lod", "tvk_plan",
+    "tvk_render", "tvk_render_views", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_frame_device_ptr",
+    "tvk_event_record", "tvk_event_elapsed", "tvk_synchronize",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [("mode", C.c_int), ("lighting", C.c_int), ("srm", C.c_float), ("isovalue", C.c_float),
+                ("lod", C.c_int), ("tf_scale", C.c_float), ("grad_scale", C.c_float),
+                ("domain_scale", C.c_float * 3), ("light_ambient", C.c_float * 3),
+                ("light_diffuse", C.c_float * 3), ("light_specular", C.c_float * 3),
+                ("light_dir", C.c_float * 3), ("iso_color", C.c_float * 3), ("refine_steps", C.c_int),
+                ("clamp_to_edge", C.c_int), ("ert_eps", C.c_float), ("screen_space_error", C.c_float)]
+
+
+class BrickRec(C.Structure):
+    _fields_ = [("key_index", C.c_uint64), ("toc_index", C.c_uint64), ("center", C.c_float * 3),
+                ("extension", C.c_float * 3), ("voxel_count", C.c_uint32 * 3), ("tex_min", C.c_float * 3),
+                ("tex_max", C.c_float * 3), ("distance", C.c_float), ("is_empty", C.c_int)]
+
+
+class DatasetDesc(C.Structure):
+    _fields_ = [("lod_count", C.c_int), ("brick_count", C.c_uint64), ("pool_bytes", C.c_uint64),
+                ("range_min", C.c_double), ("range_max", C.c_double), ("bit_width", C.c_int),
+                ("dims", C.c_uint64 * 3), ("brick_size", C.c_uint32 * 3), ("overlap", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("bricks_in_list", C.c_uint32), ("bricks_nonempty", C.c_uint32),
+                ("ms", C.c_float)]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libtvkhip.so.  Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing; build it with `python -m tuvok_amd.build`")
+    L = C.CDLL(path)
+    vp, u64p, u32p = C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)
+    fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+    sig = {
+        "tvk_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "tvk_destroy": (None, [vp]),
+        "tvk_last_error": (C.c_char_p, [vp]),
+        "tvk_default_params": (C.c_int, [C.POINTER(Params)]),
+        "tvk_load_volume": (C.c_int, [vp, vp, C.c_int, u64p, u32p, C.c_uint32, C.c_int, C.c_int, C.c_int]),
+        "tvk_generate_volume": (C.c_int, [vp, C.c_int, C.c_uint64, C.c_int, u64p, u32p, C.c_uint32, C.c_int,
+                                          C.c_int, C.c_int]),
+        "tvk_download_volume": (C.c_int, [vp, vp, C.c_size_t]),
+        "tvk_dataset_info": (C.c_int, [vp, C.POINTER(DatasetDesc)]),
+        "tvk_lod_info": (C.c_int, [vp, C.c_int, u64p, u64p, u64p, dp]),
+        "tvk_brick_info": (C.c_int, [vp, C.c_uint64, u64p, u32p, u64p]),
+        "tvk_brick_md": (C.c_int, [vp, C.c_int, C.c_uint64, fp, fp, u32p]),
+        "tvk_brick_minmax": (C.c_int, [vp, C.c_uint64, dp]),
+        "tvk_get_brick": (C.c_int, [vp, C.c_uint64, vp, C.c_size_t]),
+        "tvk_get_pool": (C.c_int, [vp, vp, C.c_size_t]),
+        "tvk_max_gradient": (C.c_int, [vp, fp]),
+        "tvk_set_tf1d": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_int]),
+        "tvk_set_tf2d": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_int, C.c_int]),
+        "tvk_set_view_matrices": (C.c_int, [vp, fp, fp, fp, fp, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                            C.c_float]),
+        "tvk_set_camera": (C.c_int, [vp, fp, fp, fp, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32]),
+        "tvk_set_rotation": (C.c_int, [vp, fp]),
+        "tvk_get_view_matrices": (C.c_int, [vp, fp, fp]),
+        "tvk_set_params": (C.c_int, [vp, C.POINTER(Params)]),
+        "tvk_get_params": (C.c_int, [vp, C.POINTER(Params)]),
+        "tvk_current_lod": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "tvk_plan": (C.c_int, [vp, C.POINTER(BrickRec), C.c_int, C.POINTER(C.c_int)]),
+        "tvk_render": (C.c_int, [vp, fp, C.POINTER(Stats)]),
+        "tvk_render_views": (C.c_int, [vp, C.c_int, fp, fp, C.POINTER(Stats)]),
+        "tvk_set_count_samples": (C.c_int, [vp, C.c_int]),
+        "tvk_touched_bytes": (C.c_int, [vp, u64p, u64p]),
+        "tvk_frame_device_ptr": (C.c_int, [vp, C.POINTER(vp)]),
+        "tvk_event_record": (C.c_int, [vp, C.c_int]),
+        "tvk_event_elapsed": (C.c_int, [vp, C.c_int, C.c_int, fp]),
+        "tvk_synchronize": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class TvkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tvk error {code}: {msg}")
+        self.code = code

@@ -1,0 +1,889 @@
+// C ABI of the raycasting path (include/tvk_hip.h).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+#include "tvk_hip.h"
+#include "host/host.hpp"
+#include "kernels/generate.hpp"
+#include "tvk_types.hpp"
+
+namespace tvk {
+hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st);
+hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
+                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, hipStream_t st);
+hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
+                             hipStream_t st);
+hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t* out,
+                         hipStream_t st);
+hipError_t launch_max_gradient(const void* lin, int bpv, const uint64_t d[3], int clamp, double* out, uint32_t blocks,
+                               hipStream_t st);
+hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bricks, const uint32_t bricks[3],
+                               uint32_t ov, const uint32_t* d_wave, uint32_t wave_n, uint32_t max_shell,
+                               hipStream_t st);
+}  // namespace tvk
+
+using namespace tvk;
+
+struct tvk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // dataset
+  bool has_ds = false;
+  Layout L;
+  int clamp = 0;
+  void* d_pool = nullptr;
+  size_t pool_bytes = 0;
+  void* d_lin = nullptr;
+  size_t lin_bytes = 0;
+  std::vector<double> minmax;
+  double range_min = 0, range_max = 0;
+  // transfer functions
+  std::vector<uint8_t> tf1d, tf2d;
+  int tf1d_n = 0, tf2d_w = 0, tf2d_h = 0;
+  uint8_t* d_tf1d = nullptr;
+  uint8_t* d_tf2d = nullptr;
+  size_t tf1d_cap = 0, tf2d_cap = 0;
+  // view + parameters
+  ViewState V;
+  bool has_view = false;
+  tvk_params P{};
+  // frame resources
+  FrameBrick* d_list = nullptr;
+  size_t list_cap = 0;
+  ViewConst* d_views = nullptr;
+  size_t views_cap = 0;
+  float* d_out = nullptr;
+  size_t out_cap = 0;
+  unsigned long long* d_counter = nullptr;
+  bool count = false;
+  uint32_t* d_touched = nullptr;
+  size_t touched_cap = 0, touched_n = 0;
+  hipEvent_t ev[8] = {};
+  std::vector<FrameBrick> h_list;
+  std::vector<ViewConst> h_views;
+  FrameBrick* h_list_pinned = nullptr;
+  size_t h_list_pinned_cap = 0;
+  ViewConst* h_views_pinned = nullptr;
+  size_t h_views_pinned_cap = 0;
+  int last_views = 0;
+  uint32_t last_w = 0, last_h = 0;
+  int last_lod = 0;
+};
+
+namespace {
+
+int fail(tvk_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+int hipfail(tvk_ctx* c, const char* what, hipError_t e) {
+  return fail(c, e == hipErrorOutOfMemory ? TVK_ERR_NOMEM : TVK_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(c, x)                                  \
+  do {                                                \
+    hipError_t e_ = (x);                              \
+    if (e_ != hipSuccess) return hipfail(c, #x, e_);  \
+  } while (0)
+
+template <typename T>
+int ensure(tvk_ctx* c, T** p, size_t* cap, size_t n) {
+  if (*cap >= n && *p) return TVK_OK;
+  if (*p) { (void)hipFree(*p); *p = nullptr; *cap = 0; }
+  size_t want = std::max<size_t>(n, 1);
+  HIPCHK(c, hipMalloc((void**)p, want * sizeof(T)));
+  *cap = want;
+  return TVK_OK;
+}
+
+template <typename T>
+int ensure_pinned(tvk_ctx* c, T** p, size_t* cap, size_t n) {
+  if (*cap >= n && *p) return TVK_OK;
+  if (*p) { (void)hipHostFree(*p); *p = nullptr; *cap = 0; }
+  size_t want = std::max<size_t>(n * 2, 64);
+  HIPCHK(c, hipHostMalloc((void**)p, want * sizeof(T), hipHostMallocDefault));
+  *cap = want;
+  return TVK_OK;
+}
+
+void free_dataset(tvk_ctx* c) {
+  if (c->d_pool) (void)hipFree(c->d_pool);
+  if (c->d_lin) (void)hipFree(c->d_lin);
+  c->d_pool = c->d_lin = nullptr;
+  c->pool_bytes = c->lin_bytes = 0;
+  c->has_ds = false;
+  c->minmax.clear();
+}
+
+int bit_width(const tvk_ctx* c) { return c->L.bpv == 1 ? 8 : 16; }
+
+// Brick the linear volume at d_lin (dims of c->L) into the pool.
+int build_pool(tvk_ctx* c, const void* d_lin) {
+  Layout& L = c->L;
+  const int bpv = L.bpv;
+  const size_t keep = L.toc.empty() ? 0 : (size_t)(L.toc.back().coords[3] + 1);
+  const uint32_t core[3] = {L.brick[0] - 2 * L.overlap, L.brick[1] - 2 * L.overlap, L.brick[2] - 2 * L.overlap};
+  if (keep > 1 && (core[0] % 2 || core[1] % 2 || core[2] % 2))
+    return fail(c, TVK_ERR_UNSUPPORTED, "coarser LODs need an even brick core (brick size - 2*overlap)");
+  c->pool_bytes = (size_t)L.pool_elems * bpv;
+  HIPCHK(c, hipMalloc(&c->d_pool, c->pool_bytes));
+  // descriptors of every kept brick
+  std::vector<GatherBrick> gb(L.toc.size());
+  for (size_t i = 0; i < L.toc.size(); ++i) {
+    for (int k = 0; k < 3; ++k) { gb[i].c[k] = (uint32_t)L.toc[i].coords[k]; gb[i].n[k] = L.toc[i].size[k]; }
+    gb[i].offset = L.toc[i].offset;
+  }
+  GatherBrick* d_gb = nullptr;
+  HIPCHK(c, hipMalloc(&d_gb, gb.size() * sizeof(GatherBrick)));
+  HIPCHK(c, hipMemcpyAsync(d_gb, gb.data(), gb.size() * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream));
+  void* tmp[2] = {nullptr, nullptr};
+  const void* src = d_lin;
+  int rc = TVK_OK;
+  for (size_t lod = 0; lod < keep && rc == TVK_OK; ++lod) {
+    const LodInfo& li = L.lods[lod];
+    if (lod > 0) {
+      const uint64_t* sd = L.lods[lod - 1].size;
+      void*& dst = tmp[(lod - 1) % 2];
+      if (!dst) {
+        size_t bytes = (size_t)(li.size[0] * li.size[1] * li.size[2]) * bpv;
+        hipError_t e = hipMalloc(&dst, bytes);
+        if (e != hipSuccess) { rc = hipfail(c, "hipMalloc(lod temp)", e); break; }
+      }
+      hipError_t e = launch_downsample(src, sd, dst, li.size, bpv, c->stream);
+      if (e != hipSuccess) { rc = hipfail(c, "downsample", e); break; }
+      src = dst;
+    }
+    const uint32_t nb = (uint32_t)(li.bricks[0] * li.bricks[1] * li.bricks[2]);
+    uint32_t maxv = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+      const TocBrick& t = L.toc[li.offset + b];
+      maxv = std::max<uint32_t>(maxv, t.size[0] * t.size[1] * t.size[2]);
+    }
+    hipError_t e = launch_gather(src, bpv, li.size, d_gb + li.offset, nb, maxv, c->d_pool, core, L.overlap, c->clamp,
+                                 c->stream);
+    if (e != hipSuccess) { rc = hipfail(c, "gather", e); break; }
+    if (lod > 0 && !c->clamp && nb > 1) {
+      // reproduce FillOverlap's copy order on the coarse level (see bricking.hip)
+      const uint32_t bxyz[3] = {(uint32_t)li.bricks[0], (uint32_t)li.bricks[1], (uint32_t)li.bricks[2]};
+      std::vector<uint32_t> order;
+      std::vector<uint32_t> start;
+      for (uint32_t w = 0; w + 2 < bxyz[0] + bxyz[1] + bxyz[2]; ++w) {
+        start.push_back((uint32_t)order.size());
+        for (uint32_t z = 0; z < bxyz[2]; ++z)
+          for (uint32_t y = 0; y < bxyz[1]; ++y) {
+            if (z + y > w || w - z - y >= bxyz[0]) continue;
+            order.push_back((w - z - y) + bxyz[0] * (y + bxyz[1] * z));
+          }
+      }
+      start.push_back((uint32_t)order.size());
+      uint32_t* d_order = nullptr;
+      e = hipMalloc(&d_order, order.size() * sizeof(uint32_t));
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+      uint32_t shell = 0;
+      for (uint32_t b = 0; b < nb; ++b) {
+        const TocBrick& t = L.toc[li.offset + b];
+        const uint32_t sh = 2 * L.overlap * (t.size[1] * t.size[2] + t.size[0] * t.size[2] + t.size[0] * t.size[1]);
+        shell = std::max(shell, sh);
+      }
+      for (size_t w = 0; w + 1 < start.size() && e == hipSuccess; ++w)
+        if (start[w + 1] > start[w])
+          e = launch_fill_overlap(c->d_pool, bpv, d_gb + li.offset, bxyz, L.overlap, d_order + start[w],
+                                  start[w + 1] - start[w], shell, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (d_order) (void)hipFree(d_order);
+      if (e != hipSuccess) { rc = hipfail(c, "fill_overlap", e); break; }
+    }
+  }
+  uint32_t* d_mm = nullptr;
+  if (rc == TVK_OK) {
+    hipError_t e = hipMalloc(&d_mm, 2 * gb.size() * sizeof(uint32_t));
+    if (e != hipSuccess) rc = hipfail(c, "hipMalloc(minmax)", e);
+  }
+  if (rc == TVK_OK) {
+    hipError_t e = launch_minmax(c->d_pool, bpv, d_gb, (uint32_t)gb.size(), d_mm, c->stream);
+    if (e != hipSuccess) rc = hipfail(c, "minmax", e);
+  }
+  std::vector<uint32_t> mm(2 * gb.size());
+  if (rc == TVK_OK) {
+    hipError_t e = hipMemcpyAsync(mm.data(), d_mm, mm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hipfail(c, "minmax download", e);
+  }
+  (void)hipStreamSynchronize(c->stream);
+  if (d_mm) (void)hipFree(d_mm);
+  (void)hipFree(d_gb);
+  for (void* t : tmp) if (t) (void)hipFree(t);
+  if (rc != TVK_OK) return rc;
+  c->minmax.resize(mm.size());
+  for (size_t i = 0; i < mm.size(); ++i) c->minmax[i] = (double)mm[i];
+  // UVFDataset::ComputeRange over the LOD0 bricks (uvfDataset.cpp:1120-1165)
+  const LodInfo& l0 = L.lods[0];
+  const uint64_t n0 = l0.bricks[0] * l0.bricks[1] * l0.bricks[2];
+  c->range_min = c->minmax[0];
+  c->range_max = c->minmax[1];
+  for (uint64_t i = 1; i < n0; ++i) {
+    c->range_min = std::min(c->range_min, c->minmax[2 * i]);
+    c->range_max = std::max(c->range_max, c->minmax[2 * i + 1]);
+  }
+  c->has_ds = true;
+  return TVK_OK;
+}
+
+int begin_dataset(tvk_ctx* c, int dtype, const uint64_t dims[3], const uint32_t brick[3], uint32_t overlap,
+                  int clamp, int max_lods) {
+  if (dtype != TVK_U8 && dtype != TVK_U16) return fail(c, TVK_ERR_ARG, "dtype must be TVK_U8 or TVK_U16");
+  if (!dims || !brick) return fail(c, TVK_ERR_ARG, "null dims/brick");
+  free_dataset(c);
+  std::string e;
+  if (!c->L.build(dims, brick, overlap, dtype, max_lods, &e)) return fail(c, TVK_ERR_ARG, e);
+  c->clamp = clamp ? 1 : 0;
+  return TVK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tvk_default_params(tvk_params* p) {
+  if (!p) return TVK_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  // AbstrRenderer constructor defaults (AbstrRenderer.cpp:71-214)
+  p->mode = TVK_RM_1DTRANS;
+  p->lighting = 1;
+  p->srm = 1.0f;
+  p->isovalue = 0.5f;
+  p->lod = 0;
+  p->tf_scale = 0.0f;
+  p->grad_scale = 1.0f;
+  for (int i = 0; i < 3; ++i) {
+    p->domain_scale[i] = 1.0f;
+    p->light_ambient[i] = 1.0f * 0.1f;   // m_cAmbient.xyz * w
+    p->light_diffuse[i] = 1.0f;
+    p->light_specular[i] = 1.0f;
+    p->iso_color[i] = 0.5f;
+  }
+  p->light_dir[0] = 0.0f; p->light_dir[1] = 0.0f; p->light_dir[2] = -1.0f;
+  p->refine_steps = 5;
+  p->clamp_to_edge = 0;
+  p->ert_eps = 0.0f;
+  p->screen_space_error = 1.0f;
+  return TVK_OK;
+}
+
+int tvk_create(tvk_ctx** out, int device) {
+  if (!out) return TVK_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return TVK_ERR_HIP;
+  if (device < 0 || device >= n) return TVK_ERR_ARG;
+  tvk_ctx* c = new (std::nothrow) tvk_ctx;
+  if (!c) return TVK_ERR_NOMEM;
+  c->device = device;
+  tvk_default_params(&c->P);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return TVK_ERR_HIP;
+  }
+  for (auto& ev : c->ev) (void)hipEventCreate(&ev);
+  if (hipMalloc(&c->d_counter, sizeof(unsigned long long)) != hipSuccess) {
+    delete c;
+    return TVK_ERR_NOMEM;
+  }
+  *out = c;
+  return TVK_OK;
+}
+
+void tvk_destroy(tvk_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_dataset(c);
+  if (c->d_tf1d) (void)hipFree(c->d_tf1d);
+  if (c->d_tf2d) (void)hipFree(c->d_tf2d);
+  if (c->d_list) (void)hipFree(c->d_list);
+  if (c->d_views) (void)hipFree(c->d_views);
+  if (c->d_out) (void)hipFree(c->d_out);
+  if (c->d_counter) (void)hipFree(c->d_counter);
+  if (c->d_touched) (void)hipFree(c->d_touched);
+  if (c->h_list_pinned) (void)hipHostFree(c->h_list_pinned);
+  if (c->h_views_pinned) (void)hipHostFree(c->h_views_pinned);
+  for (auto& ev : c->ev) if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* tvk_last_error(const tvk_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tvk_load_volume(tvk_ctx* c, const void* host, int dtype, const uint64_t dims[3], const uint32_t brick[3],
+                    uint32_t overlap, int clamp, int max_lods, int keep_linear) {
+  if (!c) return TVK_ERR_ARG;
+  if (!host) return fail(c, TVK_ERR_ARG, "null volume");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = begin_dataset(c, dtype, dims, brick, overlap, clamp, max_lods);
+  if (rc) return rc;
+  const size_t bytes = (size_t)(dims[0] * dims[1] * dims[2]) * dtype;
+  HIPCHK(c, hipMalloc(&c->d_lin, bytes));
+  c->lin_bytes = bytes;
+  HIPCHK(c, hipMemcpyAsync(c->d_lin, host, bytes, hipMemcpyHostToDevice, c->stream));
+  rc = build_pool(c, c->d_lin);
+  if (!keep_linear || rc) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_lin);
+    c->d_lin = nullptr;
+    c->lin_bytes = 0;
+  }
+  if (rc) free_dataset(c);
+  return rc;
+}
+
+int tvk_generate_volume(tvk_ctx* c, int kind, uint64_t seed, int dtype, const uint64_t dims[3],
+                        const uint32_t brick[3], uint32_t overlap, int clamp, int max_lods, int keep_linear) {
+  if (!c) return TVK_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = begin_dataset(c, dtype, dims, brick, overlap, clamp, max_lods);
+  if (rc) return rc;
+  const size_t bytes = (size_t)(dims[0] * dims[1] * dims[2]) * dtype;
+  HIPCHK(c, hipMalloc(&c->d_lin, bytes));
+  c->lin_bytes = bytes;
+  SplitMix rng(seed);
+  hipError_t e = hipSuccess;
+  GenEllipsoid* d_e = nullptr;
+  unsigned* d_tmp = nullptr;
+  if (kind == TVK_GEN_BLOBS) {
+    // 12 isotropic Gaussians: centres U[0.2,0.8]^3, sigma U[0.03,0.12], amplitude U[0.3,1.0]
+    GenBlobs g{};
+    g.n = 12;
+    for (int b = 0; b < g.n; ++b) {
+      for (int k = 0; k < 3; ++k) g.c[b][k] = (float)rng.uniform(0.2, 0.8);
+      double sigma = rng.uniform(0.03, 0.12);
+      g.amp[b] = (float)rng.uniform(0.3, 1.0);
+      g.inv2s2[b] = (float)(1.0 / (2.0 * sigma * sigma));
+    }
+    e = hipMalloc(&d_tmp, sizeof(unsigned));
+    if (e == hipSuccess) e = launch_blobs(g, dims, dtype, c->d_lin, d_tmp, seed, c->stream);
+  } else if (kind == TVK_GEN_FBM_SHELLS) {
+    // 3-octave value-noise fBm + 64 hollow shells (radius 0.02-0.1, 2 voxels thick, value 60000)
+    GenShells g{};
+    g.n = 64;
+    g.octaves = 3;
+    g.base_freq = 8.0f;
+    g.seed = rng.next();
+    for (int s = 0; s < g.n; ++s) {
+      for (int k = 0; k < 3; ++k) g.c[s][k] = (float)rng.uniform(0.15, 0.85);
+      g.r[s] = (float)rng.uniform(0.02, 0.1);
+    }
+    g.half_thickness_vox = 1.0f;
+    g.shell_value = dtype == TVK_U16 ? 60000.f : 233.f;
+    e = launch_fbm_shells(g, dims, dtype, c->d_lin, c->stream);
+  } else if (kind == TVK_GEN_ELLIPSOID_SDF || kind == TVK_GEN_PHANTOM) {
+    const int ne = kind == TVK_GEN_ELLIPSOID_SDF ? 500 : 40;
+    std::vector<GenEllipsoid> el((size_t)ne);
+    for (int k = 0; k < ne; ++k) {
+      GenEllipsoid& q = el[(size_t)k];
+      if (kind == TVK_GEN_ELLIPSOID_SDF) {
+        for (int a = 0; a < 3; ++a) { q.c[a] = (float)rng.uniform(0.1, 0.9); q.a[a] = (float)rng.uniform(0.01, 0.08); }
+        q.value = 0.f;
+      } else {
+        // nested: each ellipsoid lies inside the previous one
+        double s = 0.45 * std::pow(0.93, k);
+        for (int a = 0; a < 3; ++a) {
+          q.c[a] = (float)(0.5 + rng.uniform(-0.15, 0.15) * s);
+          q.a[a] = (float)(s * rng.uniform(0.7, 1.0));
+        }
+        q.value = (float)std::floor(rng.uniform(0.0, 4000.0));
+      }
+      q.amin = std::min(q.a[0], std::min(q.a[1], q.a[2]));
+    }
+    e = hipMalloc(&d_e, el.size() * sizeof(GenEllipsoid));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_e, el.data(), el.size() * sizeof(GenEllipsoid), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+      if (kind == TVK_GEN_ELLIPSOID_SDF) {
+        if (dtype != TVK_U8) { (void)hipFree(d_e); return fail(c, TVK_ERR_ARG, "SDF volumes are uint8"); }
+        e = launch_ellipsoid_sdf(d_e, ne, dims, c->d_lin, c->stream);
+      } else {
+        if (dtype != TVK_U16) { (void)hipFree(d_e); return fail(c, TVK_ERR_ARG, "phantom volumes are uint16"); }
+        e = launch_phantom(d_e, ne, dims, c->d_lin, 20.f, rng.next(), c->stream);
+      }
+    }
+  } else {
+    return fail(c, TVK_ERR_ARG, "unknown generator");
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_e) (void)hipFree(d_e);
+  if (d_tmp) (void)hipFree(d_tmp);
+  if (e != hipSuccess) { free_dataset(c); return hipfail(c, "generate", e); }
+  rc = build_pool(c, c->d_lin);
+  if (!keep_linear || rc) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_lin);
+    c->d_lin = nullptr;
+    c->lin_bytes = 0;
+  }
+  if (rc) free_dataset(c);
+  return rc;
+}
+
+int tvk_download_volume(tvk_ctx* c, void* host, size_t bytes) {
+  if (!c || !host) return TVK_ERR_ARG;
+  if (!c->d_lin) return fail(c, TVK_ERR_STATE, "linear volume was not kept");
+  if (bytes != c->lin_bytes) return fail(c, TVK_ERR_ARG, "size mismatch");
+  HIPCHK(c, hipMemcpyAsync(host, c->d_lin, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TVK_OK;
+}
+
+int tvk_dataset_info(tvk_ctx* c, tvk_dataset_desc* o) {
+  if (!c || !o) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  std::memset(o, 0, sizeof(*o));
+  o->lod_count = (int)(c->L.toc.back().coords[3] + 1);
+  o->brick_count = c->L.toc.size();
+  o->pool_bytes = c->pool_bytes;
+  o->range_min = c->range_min;
+  o->range_max = c->range_max;
+  o->bit_width = bit_width(c);
+  for (int i = 0; i < 3; ++i) { o->dims[i] = c->L.dims[i]; o->brick_size[i] = c->L.brick[i]; }
+  o->overlap = c->L.overlap;
+  return TVK_OK;
+}
+
+int tvk_lod_info(tvk_ctx* c, int lod, uint64_t size[3], uint64_t bricks[3], uint64_t* offset, double aspect[3]) {
+  if (!c) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (lod < 0 || (size_t)lod >= c->L.lods.size()) return fail(c, TVK_ERR_RANGE, "lod out of range");
+  const LodInfo& l = c->L.lods[(size_t)lod];
+  for (int i = 0; i < 3; ++i) {
+    if (size) size[i] = l.size[i];
+    if (bricks) bricks[i] = l.bricks[i];
+    if (aspect) aspect[i] = l.aspect[i];
+  }
+  if (offset) *offset = l.offset;
+  return TVK_OK;
+}
+
+int tvk_brick_info(tvk_ctx* c, uint64_t i, uint64_t coords[4], uint32_t size[3], uint64_t* off) {
+  if (!c) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (i >= c->L.toc.size()) return fail(c, TVK_ERR_RANGE, "unknown brick");
+  const TocBrick& t = c->L.toc[(size_t)i];
+  for (int k = 0; k < 4; ++k) if (coords) coords[k] = t.coords[k];
+  for (int k = 0; k < 3; ++k) if (size) size[k] = t.size[k];
+  if (off) *off = t.offset * (uint64_t)c->L.bpv;
+  return TVK_OK;
+}
+
+int tvk_brick_md(tvk_ctx* c, int lod, uint64_t index, float center[3], float extents[3], uint32_t nvox[3]) {
+  if (!c) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  auto it = c->L.table.find(BrickKey(0, (size_t)lod, (size_t)index));
+  if (it == c->L.table.end()) return fail(c, TVK_ERR_RANGE, "unknown brick");  // BrickedDataset.cpp:85
+  for (int k = 0; k < 3; ++k) {
+    if (center) center[k] = it->second.center[k];
+    if (extents) extents[k] = it->second.extents[k];
+    if (nvox) nvox[k] = it->second.n_voxels[k];
+  }
+  return TVK_OK;
+}
+
+int tvk_brick_minmax(tvk_ctx* c, uint64_t i, double mm[2]) {
+  if (!c || !mm) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (i >= c->L.toc.size()) return fail(c, TVK_ERR_RANGE, "unknown brick");
+  mm[0] = c->minmax[2 * (size_t)i];
+  mm[1] = c->minmax[2 * (size_t)i + 1];
+  return TVK_OK;
+}
+
+int tvk_get_brick(tvk_ctx* c, uint64_t i, void* host, size_t bytes) {
+  if (!c || !host) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (i >= c->L.toc.size()) return fail(c, TVK_ERR_RANGE, "unknown brick");
+  const TocBrick& t = c->L.toc[(size_t)i];
+  if (bytes != t.count * (size_t)c->L.bpv) return fail(c, TVK_ERR_ARG, "size mismatch");
+  HIPCHK(c, hipMemcpyAsync(host, (const uint8_t*)c->d_pool + t.offset * c->L.bpv, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TVK_OK;
+}
+
+int tvk_get_pool(tvk_ctx* c, void* host, size_t bytes) {
+  if (!c || !host) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (bytes != c->pool_bytes) return fail(c, TVK_ERR_ARG, "size mismatch");
+  HIPCHK(c, hipMemcpyAsync(host, c->d_pool, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TVK_OK;
+}
+
+int tvk_max_gradient(tvk_ctx* c, float* out) {
+  if (!c || !out) return TVK_ERR_ARG;
+  if (!c->d_lin) return fail(c, TVK_ERR_STATE, "linear volume was not kept");
+  const uint32_t blocks = 4096;
+  double* d = nullptr;
+  HIPCHK(c, hipMalloc(&d, blocks * sizeof(double)));
+  std::vector<double> h(blocks);
+  hipError_t e = launch_max_gradient(c->d_lin, c->L.bpv, c->L.dims, c->clamp, d, blocks, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, blocks * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hipfail(c, "max_gradient", e);
+  double m = 0;
+  for (double v : h) m = std::max(m, v);
+  *out = (float)m;
+  return TVK_OK;
+}
+
+int tvk_set_tf1d(tvk_ctx* c, const uint8_t* rgba8, int n) {
+  if (!c) return TVK_ERR_ARG;
+  if (!rgba8 || n < 2 || n > 4096) return fail(c, TVK_ERR_ARG, "1D TF needs 2..4096 RGBA8 entries");
+  c->tf1d.assign(rgba8, rgba8 + 4 * (size_t)n);
+  c->tf1d_n = n;
+  int rc = ensure(c, &c->d_tf1d, &c->tf1d_cap, 4 * (size_t)n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_tf1d, c->tf1d.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TVK_OK;
+}
+
+int tvk_set_tf2d(tvk_ctx* c, const uint8_t* rgba8, int w, int h) {
+  if (!c) return TVK_ERR_ARG;
+  if (!rgba8 || w < 2 || h < 2 || w > 4096 || h > 4096) return fail(c, TVK_ERR_ARG, "bad 2D TF size");
+  c->tf2d.assign(rgba8, rgba8 + 4 * (size_t)w * h);
+  c->tf2d_w = w;
+  c->tf2d_h = h;
+  int rc = ensure(c, &c->d_tf2d, &c->tf2d_cap, 4 * (size_t)w * h);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_tf2d, c->tf2d.data(), 4 * (size_t)w * h, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TVK_OK;
+}
+
+int tvk_set_view_matrices(tvk_ctx* c, const float view[16], const float rot[16], const float trans[16],
+                          const float proj[16], uint32_t w, uint32_t h, float fov, float znear, float zfar) {
+  if (!c || !view || !proj) return TVK_ERR_ARG;
+  if (w == 0 || h == 0 || w > 32767 || h > 32767) return fail(c, TVK_ERR_ARG, "bad viewport size");
+  c->V.view = Mat4::from(view);
+  c->V.proj = Mat4::from(proj);
+  c->V.rot = rot ? Mat4::from(rot) : Mat4();
+  c->V.trans = trans ? Mat4::from(trans) : Mat4();
+  c->V.W = w;
+  c->V.H = h;
+  c->V.fov = fov;
+  c->V.znear = znear;
+  c->V.zfar = zfar;
+  c->has_view = true;
+  return TVK_OK;
+}
+
+int tvk_set_camera(tvk_ctx* c, const float eye[3], const float at[3], const float up[3], float fov, float znear,
+                   float zfar, uint32_t w, uint32_t h) {
+  if (!c || !eye || !at || !up) return TVK_ERR_ARG;
+  if (w == 0 || h == 0 || w > 32767 || h > 32767) return fail(c, TVK_ERR_ARG, "bad viewport size");
+  // GLRenderer::ComputeViewAndProjection (GLRenderer.cpp:889-920)
+  Mat4 v = look_at(F3(eye[0], eye[1], eye[2]), F3(at[0], at[1], at[2]), F3(up[0], up[1], up[2]));
+  Mat4 p = perspective(fov, (float)w / (float)h, znear, zfar);
+  c->V.view = v;
+  c->V.proj = p;
+  c->V.rot = Mat4();
+  c->V.trans = Mat4();
+  c->V.W = w;
+  c->V.H = h;
+  c->V.fov = fov;
+  c->V.znear = znear;
+  c->V.zfar = zfar;
+  c->has_view = true;
+  return TVK_OK;
+}
+
+int tvk_set_rotation(tvk_ctx* c, const float rot[16]) {
+  if (!c || !rot) return TVK_ERR_ARG;
+  c->V.rot = Mat4::from(rot);
+  return TVK_OK;
+}
+
+int tvk_get_view_matrices(tvk_ctx* c, float view[16], float proj[16]) {
+  if (!c) return TVK_ERR_ARG;
+  for (int i = 0; i < 16; ++i) {
+    if (view) view[i] = c->V.view.a[i];
+    if (proj) proj[i] = c->V.proj.a[i];
+  }
+  return TVK_OK;
+}
+
+int tvk_set_params(tvk_ctx* c, const tvk_params* p) {
+  if (!c || !p) return TVK_ERR_ARG;
+  if (p->mode < 0 || p->mode > 2) return fail(c, TVK_ERR_ARG, "invalid render mode");
+  if (!(p->srm > 0.0f)) return fail(c, TVK_ERR_ARG, "sample rate modifier must be > 0");
+  if (p->refine_steps < 0 || p->refine_steps > 64) return fail(c, TVK_ERR_ARG, "refine_steps out of range");
+  if (p->ert_eps < 0.0f || p->ert_eps >= 1.0f) return fail(c, TVK_ERR_ARG, "ert_eps must be in [0,1)");
+  c->P = *p;
+  return TVK_OK;
+}
+
+int tvk_get_params(tvk_ctx* c, tvk_params* p) {
+  if (!c || !p) return TVK_ERR_ARG;
+  *p = c->P;
+  return TVK_OK;
+}
+
+int tvk_set_count_samples(tvk_ctx* c, int on) {
+  if (!c) return TVK_ERR_ARG;
+  c->count = on != 0;
+  return TVK_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int frame_lod(tvk_ctx* c, const ViewState& V, int* lod) {
+  int l = c->P.lod;
+  if (l < 0) l = min_lod_for_view(c->L, V, c->P.screen_space_error);
+  const int kept = (int)(c->L.toc.back().coords[3] + 1);
+  if (l >= kept) return fail(c, TVK_ERR_RANGE, "LOD " + std::to_string(l) + " is not in the brick pool");
+  *lod = l;
+  return TVK_OK;
+}
+
+int plan_params(tvk_ctx* c, int lod, PlanParams& pp) {
+  pp.mode = c->P.mode;
+  pp.lod = lod;
+  pp.srm = c->P.srm;
+  pp.isovalue = c->P.isovalue;
+  if (c->P.mode == TVK_RM_1DTRANS) {
+    if (c->tf1d_n == 0) return fail(c, TVK_ERR_STATE, "no 1D transfer function");
+    tf1d_limits(c->tf1d.data(), c->tf1d_n, pp.lim);
+    pp.tf_n = c->tf1d_n;
+  } else if (c->P.mode == TVK_RM_2DTRANS) {
+    if (c->tf2d_w == 0) return fail(c, TVK_ERR_STATE, "no 2D transfer function");
+    tf2d_limits(c->tf2d.data(), c->tf2d_w, c->tf2d_h, pp.lim);
+    pp.tf_n = c->tf2d_w;
+  } else {
+    pp.tf_n = c->tf1d_n > 0 ? c->tf1d_n : 256;
+  }
+  return TVK_OK;
+}
+
+// GLRenderer::CalculateScaling (GLRenderer.cpp:1813-1819)
+float tf_scale_of(const tvk_ctx* c) {
+  if (c->P.tf_scale > 0.0f) return c->P.tf_scale;
+  const double fMaxValue = c->range_max > 0 ? c->range_max : (double)std::max(c->tf1d_n, 1);
+  const uint32_t iMaxValue = (uint32_t)(1u << bit_width(c)) - 1u;
+  return float(iMaxValue / fMaxValue);
+}
+
+int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st) {
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (!c->has_view) return fail(c, TVK_ERR_STATE, "no view");
+  if (nviews < 1) return fail(c, TVK_ERR_ARG, "nviews must be >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->h_list.clear();
+  c->h_views.assign((size_t)nviews, ViewConst{});
+  std::vector<PlannedBrick> planned;
+  uint32_t groups = 0, in_list = 0;
+  uint64_t pixels = 0;
+  for (int v = 0; v < nviews; ++v) {
+    ViewState V = c->V;
+    if (rots) V.rot = Mat4::from(rots + 16 * v);
+    int lod = 0;
+    int rc = frame_lod(c, V, &lod);
+    if (rc) return rc;
+    PlanParams pp;
+    rc = plan_params(c, lod, pp);
+    if (rc) return rc;
+    plan_frame(c->L, c->minmax, c->range_max, V, pp, planned);
+    in_list += (uint32_t)planned.size();
+    ViewConst& vc = c->h_views[(size_t)v];
+    make_view_const(V, vc);
+    vc.list_offset = (uint32_t)c->h_list.size();
+    make_frame_bricks(c->L, V, pp, planned, c->h_list);
+    vc.nlist = (uint32_t)c->h_list.size() - vc.list_offset;
+    vc.out_offset = (uint32_t)pixels;
+    vc.group_offset = groups;
+    groups += vc.groups;
+    pixels += (uint64_t)V.W * V.H;
+    c->last_lod = lod;
+  }
+  if (pixels >= (1ull << 32)) return fail(c, TVK_ERR_ARG, "too many pixels in one call");
+  int rc = ensure(c, &c->d_list, &c->list_cap, std::max<size_t>(c->h_list.size(), 1));
+  if (!rc) rc = ensure(c, &c->d_views, &c->views_cap, (size_t)nviews);
+  if (!rc) rc = ensure(c, &c->d_out, &c->out_cap, (size_t)pixels * 4);
+  if (!rc) rc = ensure_pinned(c, &c->h_list_pinned, &c->h_list_pinned_cap, std::max<size_t>(c->h_list.size(), 1));
+  if (!rc) rc = ensure_pinned(c, &c->h_views_pinned, &c->h_views_pinned_cap, (size_t)nviews);
+  if (rc) return rc;
+  // staging through pinned memory keeps the uploads asynchronous; wait for the
+  // previous frame's uploads before overwriting the staging buffers
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::copy(c->h_list.begin(), c->h_list.end(), c->h_list_pinned);
+  std::copy(c->h_views.begin(), c->h_views.end(), c->h_views_pinned);
+  if (!c->h_list.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_list, c->h_list_pinned, c->h_list.size() * sizeof(FrameBrick), hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_views, c->h_views_pinned, (size_t)nviews * sizeof(ViewConst), hipMemcpyHostToDevice,
+                           c->stream));
+  MarchArgs a{};
+  a.pool = (const uint8_t*)c->d_pool;
+  a.list = c->d_list;
+  a.views = c->d_views;
+  a.mode = c->P.mode;
+  a.tf = c->P.mode == TVK_RM_2DTRANS ? c->d_tf2d : c->d_tf1d;
+  a.tf_n = c->tf1d_n;
+  a.tf_w = c->tf2d_w;
+  a.tf_h = c->tf2d_h;
+  a.out = c->d_out;
+  a.counter = c->count ? c->d_counter : nullptr;
+  a.ntoc = (uint32_t)c->L.toc.size();
+  a.touched = nullptr;
+  if (c->count) {
+    const size_t nt = (size_t)nviews * c->L.toc.size();
+    rc = ensure(c, &c->d_touched, &c->touched_cap, nt);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_touched, 0, nt * sizeof(uint32_t), c->stream));
+    c->touched_n = nt;
+    a.touched = c->d_touched;
+  }
+  a.nviews = nviews;
+  a.tf_scale = tf_scale_of(c);
+  a.grad_scale = c->P.grad_scale;
+  a.iso_n = (float)((double)c->P.isovalue / (double)(1 << bit_width(c)));  // GetNormalizedIsovalue
+  a.ert_eps = c->P.ert_eps;
+  for (int i = 0; i < 3; ++i) {
+    a.dscale[i] = c->P.domain_scale[i];
+    a.amb[i] = c->P.light_ambient[i];
+    a.dif[i] = c->P.light_diffuse[i];
+    a.spc[i] = c->P.light_specular[i];
+    a.ldir[i] = c->P.light_dir[i];
+    a.iso_col[i] = c->P.iso_color[i];
+  }
+  a.refine_steps = c->P.refine_steps;
+  a.clamp_edge = c->P.clamp_to_edge;
+  a.lighting = c->P.lighting;
+  a.groups_total = groups;
+  a.groups_padded = (groups + 7u) & ~7u;
+  if (c->count) HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+  if (a.groups_total > 0) HIPCHK(c, launch_march(a, c->L.bpv, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
+  c->last_views = nviews;
+  c->last_w = c->V.W;
+  c->last_h = c->V.H;
+  if (host_out)
+    HIPCHK(c, hipMemcpyAsync(host_out, c->d_out, (size_t)pixels * 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    unsigned long long ns = 0;
+    if (c->count)
+      HIPCHK(c, hipMemcpyAsync(&ns, c->d_counter, sizeof(ns), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    st->samples = ns;
+    st->bricks_in_list = in_list;
+    st->bricks_nonempty = (uint32_t)c->h_list.size();
+    st->ms = ms;
+  } else if (host_out) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return TVK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tvk_current_lod(tvk_ctx* c, int* lod) {
+  if (!c || !lod) return TVK_ERR_ARG;
+  if (!c->has_ds || !c->has_view) return fail(c, TVK_ERR_STATE, "no dataset or view");
+  return frame_lod(c, c->V, lod);
+}
+
+int tvk_plan(tvk_ctx* c, tvk_brick_rec* out, int cap, int* n) {
+  if (!c || !n) return TVK_ERR_ARG;
+  if (!c->has_ds || !c->has_view) return fail(c, TVK_ERR_STATE, "no dataset or view");
+  int lod = 0;
+  int rc = frame_lod(c, c->V, &lod);
+  if (rc) return rc;
+  PlanParams pp;
+  rc = plan_params(c, lod, pp);
+  if (rc) return rc;
+  std::vector<PlannedBrick> list;
+  plan_frame(c->L, c->minmax, c->range_max, c->V, pp, list);
+  *n = (int)list.size();
+  if (!out) return TVK_OK;
+  if ((int)list.size() > cap) return fail(c, TVK_ERR_ARG, "output too small");
+  for (size_t i = 0; i < list.size(); ++i) {
+    const PlannedBrick& b = list[i];
+    tvk_brick_rec& r = out[i];
+    r.key_index = b.key_index;
+    r.toc_index = b.toc_index;
+    for (int k = 0; k < 3; ++k) {
+      r.center[k] = b.center[k];
+      r.extension[k] = b.extension[k];
+      r.voxel_count[k] = b.voxel_count[k];
+      r.tex_min[k] = b.tex_min[k];
+      r.tex_max[k] = b.tex_max[k];
+    }
+    r.distance = b.distance;
+    r.is_empty = b.is_empty;
+  }
+  return TVK_OK;
+}
+
+int tvk_render(tvk_ctx* c, float* host_out, tvk_stats* st) {
+  if (!c) return TVK_ERR_ARG;
+  return render_impl(c, 1, nullptr, host_out, st);
+}
+
+int tvk_render_views(tvk_ctx* c, int n, const float* rots, float* host_out, tvk_stats* st) {
+  if (!c || !rots) return TVK_ERR_ARG;
+  return render_impl(c, n, rots, host_out, st);
+}
+
+int tvk_touched_bytes(tvk_ctx* c, uint64_t* bytes, uint64_t* bricks) {
+  if (!c || !bytes || !bricks) return TVK_ERR_ARG;
+  if (!c->touched_n) return fail(c, TVK_ERR_STATE, "no counted render yet");
+  std::vector<uint32_t> t(c->touched_n);
+  HIPCHK(c, hipMemcpyAsync(t.data(), c->d_touched, t.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t nt = c->L.toc.size();
+  uint64_t b = 0, n = 0;
+  for (size_t i = 0; i < t.size(); ++i)
+    if (t[i]) { ++n; b += c->L.toc[i % nt].count * (uint64_t)c->L.bpv; }
+  *bytes = b;
+  *bricks = n;
+  return TVK_OK;
+}
+
+int tvk_frame_device_ptr(tvk_ctx* c, void** p) {
+  if (!c || !p) return TVK_ERR_ARG;
+  *p = c->d_out;
+  return TVK_OK;
+}
+
+int tvk_event_record(tvk_ctx* c, int slot) {
+  if (!c || slot < 0 || slot >= 6) return TVK_ERR_ARG;
+  HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+  return TVK_OK;
+}
+
+int tvk_event_elapsed(tvk_ctx* c, int a, int b, float* ms) {
+  if (!c || !ms || a < 0 || b < 0 || a >= 8 || b >= 8) return TVK_ERR_ARG;
+  HIPCHK(c, hipEventSynchronize(c->ev[b]));
+  HIPCHK(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+  return TVK_OK;
+}
+
+int tvk_synchronize(tvk_ctx* c) {
+  if (!c) return TVK_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TVK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// Device-side bricking of a linear volume into the ExtendedOctree brick layout.
+//
+// The reference builds LOD 0 brick by brick (ExtendedOctreeConverter.cpp:288-462,
+// :1381-1414), the coarser levels by 2x box filtering of the previous level's
+// bricks (ExtendedOctreeConverter.inc:1-398) followed by FillOverlap
+// (ExtendedOctreeConverter.cpp:1203-1372), and per-brick min/max over every
+// stored voxel (ExtendedOctreeConverter.inc:408-444).  For even brick cores
+// these steps have a closed form, which is what runs here as bandwidth-bound
+// gathers:
+//   LOD k+1 voxel g = floor(mean of the in-range voxels of LOD k at 2g + {0,1}^3)
+//   brick voxel i   = LOD voxel (coords*core - overlap + i), 0 outside the
+//                     volume (clamped to the edge with bClampToEdge)
+// Bit-exactness against the procedural restatement is checked by the tests.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../tvk_types.hpp"
+
+namespace tvk {
+
+// grid: x = chunks of 256 voxels * ROWS, y = brick.  Each thread writes voxels
+// along x so stores and loads are coalesced.
+template <typename T>
+__global__ __launch_bounds__(256) void gather_bricks(const T* __restrict__ lin, uint64_t X, uint64_t Y,
+                                                     uint64_t Z, const GatherBrick* __restrict__ bricks,
+                                                     T* __restrict__ pool, uint32_t core_x, uint32_t core_y,
+                                                     uint32_t core_z, uint32_t ov, int clamp) {
+  const GatherBrick b = bricks[blockIdx.y];
+  const uint64_t nvox = (uint64_t)b.n[0] * b.n[1] * b.n[2];
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nvox; i += (uint64_t)gridDim.x * 256) {
+    const uint32_t x = (uint32_t)(i % b.n[0]);
+    const uint64_t r = i / b.n[0];
+    const uint32_t y = (uint32_t)(r % b.n[1]);
+    const uint32_t z = (uint32_t)(r / b.n[1]);
+    int64_t gx = (int64_t)b.c[0] * core_x - ov + x;
+    int64_t gy = (int64_t)b.c[1] * core_y - ov + y;
+    int64_t gz = (int64_t)b.c[2] * core_z - ov + z;
+    T v = 0;
+    if (clamp) {
+      gx = gx < 0 ? 0 : (gx >= (int64_t)X ? (int64_t)X - 1 : gx);
+      gy = gy < 0 ? 0 : (gy >= (int64_t)Y ? (int64_t)Y - 1 : gy);
+      gz = gz < 0 ? 0 : (gz >= (int64_t)Z ? (int64_t)Z - 1 : gz);
+      v = lin[(uint64_t)gx + X * ((uint64_t)gy + Y * (uint64_t)gz)];
+    } else if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int64_t)X && gy < (int64_t)Y && gz < (int64_t)Z) {
+      v = lin[(uint64_t)gx + X * ((uint64_t)gy + Y * (uint64_t)gz)];
+    }
+    pool[b.offset + i] = v;
+  }
+}
+
+// 2x box filter with the reference's mean (double accumulation, truncation);
+// an axis of size 1 is not halved (ExtendedOctree.cpp:201-213).
+template <typename T>
+__global__ __launch_bounds__(256) void downsample(const T* __restrict__ src, uint64_t SX, uint64_t SY, uint64_t SZ,
+                                                  T* __restrict__ dst, uint64_t DX, uint64_t DY, uint64_t DZ) {
+  const uint64_t n = DX * DY * DZ;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t x = i % DX, r = i / DX, y = r % DY, z = r / DY;
+    const uint64_t fx = SX > 1 ? 2 : 1, fy = SY > 1 ? 2 : 1, fz = SZ > 1 ? 2 : 1;
+    uint32_t sum = 0, cnt = 0;
+    for (uint64_t dz = 0; dz < fz; ++dz) {
+      const uint64_t zz = z * fz + dz;
+      if (zz >= SZ) continue;
+      for (uint64_t dy = 0; dy < fy; ++dy) {
+        const uint64_t yy = y * fy + dy;
+        if (yy >= SY) continue;
+        const T* row = src + SX * (yy + SY * zz);
+        for (uint64_t dx = 0; dx < fx; ++dx) {
+          const uint64_t xx = x * fx + dx;
+          if (xx >= SX) continue;
+          sum += row[xx];
+          ++cnt;
+        }
+      }
+    }
+    // double(sum)/cnt is exact (cnt is a power of two) and T() truncates
+    dst[i] = (T)(sum / cnt);
+  }
+}
+
+// per-brick min/max over all stored voxels; one workgroup per brick
+template <typename T>
+__global__ __launch_bounds__(256) void brick_minmax(const T* __restrict__ pool, const GatherBrick* __restrict__ bricks,
+                                                    uint32_t* __restrict__ out) {
+  const GatherBrick b = bricks[blockIdx.x];
+  const uint64_t nvox = (uint64_t)b.n[0] * b.n[1] * b.n[2];
+  uint32_t mn = 0xffffffffu, mx = 0;
+  for (uint64_t i = threadIdx.x; i < nvox; i += 256) {
+    const uint32_t v = pool[b.offset + i];
+    mn = min(mn, v);
+    mx = max(mx, v);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, 64));
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+  }
+  __shared__ uint32_t smn[4], smx[4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { smn[w] = mn; smx[w] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) { mn = min(mn, smn[k]); mx = max(mx, smx[k]); }
+    out[2 * blockIdx.x] = mn;
+    out[2 * blockIdx.x + 1] = mx;
+  }
+}
+
+// MaxGradientMagnitude over interior voxels: |(f(x-1)-f(x+1))/(2 norm), ...|
+// with zero (or clamped) values outside the volume, as the brick overlap holds them.
+template <typename T>
+__global__ __launch_bounds__(256) void max_gradient(const T* __restrict__ lin, uint64_t X, uint64_t Y, uint64_t Z,
+                                                    int clamp, double norm, double* __restrict__ out) {
+  const uint64_t n = X * Y * Z;
+  double best = 0.0;
+  auto at = [&](int64_t x, int64_t y, int64_t z) -> double {
+    if (clamp) {
+      x = x < 0 ? 0 : (x >= (int64_t)X ? (int64_t)X - 1 : x);
+      y = y < 0 ? 0 : (y >= (int64_t)Y ? (int64_t)Y - 1 : y);
+      z = z < 0 ? 0 : (z >= (int64_t)Z ? (int64_t)Z - 1 : z);
+    } else if (x < 0 || y < 0 || z < 0 || x >= (int64_t)X || y >= (int64_t)Y || z >= (int64_t)Z) {
+      return 0.0;
+    }
+    return (double)lin[(uint64_t)x + X * ((uint64_t)y + Y * (uint64_t)z)];
+  };
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const int64_t x = (int64_t)(i % X), r = (int64_t)(i / X), y = r % (int64_t)Y, z = r / (int64_t)Y;
+    const double gx = (at(x - 1, y, z) - at(x + 1, y, z)) / (norm * 2);
+    const double gy = (at(x, y - 1, z) - at(x, y + 1, z)) / (norm * 2);
+    const double gz = (at(x, y, z - 1) - at(x, y, z + 1)) / (norm * 2);
+    const double l = sqrt(gx * gx + gy * gy + gz * gz);
+    best = l > best ? l : best;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(best, o, 64);
+    best = t > best ? t : best;
+  }
+  __shared__ double sb[4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sb[w] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) best = sb[k] > best ? sb[k] : best;
+    out[blockIdx.x] = best;
+  }
+}
+
+// FillOverlap (ExtendedOctreeConverter.cpp:1203-1372) processes the bricks of a
+// coarse level in z,y,x order and fills each brick's overlap with ten region
+// copies from its neighbours, in a fixed order; the later neighbours are copied
+// before their own overlap exists (all zero at that point without clamping).
+// A brick only reads the final state of its left/top/front neighbours and the
+// pre-fill state of the others, so the bricks with equal x+y+z can run at once:
+// one launch per such wave reproduces the sequential result exactly.  The pool
+// holds the closed-form bricks on entry; the cores never change.
+template <typename T>
+__global__ __launch_bounds__(256) void fill_overlap_wave(T* __restrict__ pool, const GatherBrick* __restrict__ lod_bricks,
+                                                         uint32_t bx, uint32_t by, uint32_t bz, uint32_t ov,
+                                                         const uint32_t* __restrict__ wave) {
+  const uint32_t me = wave[blockIdx.y];
+  const GatherBrick B = lod_bricks[me];
+  const uint32_t x = B.c[0], y = B.c[1], z = B.c[2];
+  const int tx = (int)B.n[0], ty = (int)B.n[1], tz = (int)B.n[2];
+  const int o = (int)ov;
+  const uint64_t sx_ = 2ull * o * ty * tz, sy_ = 2ull * o * tx * tz, sz_ = 2ull * o * tx * ty;
+  const uint64_t total = sx_ + sy_ + sz_;
+  const bool hasR = x + 1 < bx, hasB = y + 1 < by, hasK = z + 1 < bz, hasL = x > 0, hasT = y > 0, hasF = z > 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
+    int vx, vy, vz;
+    if (i < sx_) {             // x slabs: 2*o x ty x tz
+      const uint64_t a = i % (2ull * o), r = i / (2ull * o);
+      vx = a < (uint64_t)o ? (int)a : tx - 2 * o + (int)a;
+      vy = (int)(r % ty); vz = (int)(r / ty);
+    } else if (i < sx_ + sy_) {  // y slabs: tx x 2*o x tz
+      const uint64_t j = i - sx_;
+      vx = (int)(j % tx);
+      const uint64_t r = j / tx, a = r % (2ull * o);
+      vy = a < (uint64_t)o ? (int)a : ty - 2 * o + (int)a;
+      vz = (int)(r / (2ull * o));
+    } else {                   // z slabs: tx x ty x 2*o
+      const uint64_t j = i - sx_ - sy_;
+      vx = (int)(j % tx);
+      const uint64_t r = j / tx;
+      vy = (int)(r % ty);
+      const uint64_t a = r / ty;
+      vz = a < (uint64_t)o ? (int)a : tz - 2 * o + (int)a;
+    }
+    const bool xh = vx >= tx - o, yh = vy >= ty - o, zh = vz >= tz - o, xl = vx < o, yl = vy < o, zl = vz < o;
+    // the ten copies in the reference's order; the last one covering the voxel wins
+    int sel = -1;
+    if (hasR && xh) sel = 0;
+    if (hasB && yh) sel = 1;
+    if (hasK && zh) sel = 2;
+    if (hasL && xl) sel = 3;
+    if (hasT && yl) sel = 4;
+    if (hasF && zl) sel = 5;
+    if (hasB && hasR && xh && yh) sel = 6;
+    if (hasR && hasK && xh && zh) sel = 7;
+    if (hasB && hasK && yh && zh) sel = 8;
+    if (hasR && hasB && hasK && xh && yh && zh) sel = 9;
+    T val = 0;
+    if (sel >= 0) {
+      const int dxs[10] = {1, 0, 0, -1, 0, 0, 1, 1, 0, 1};
+      const int dys[10] = {0, 1, 0, 0, -1, 0, 1, 0, 1, 1};
+      const int dzs[10] = {0, 0, 1, 0, 0, -1, 0, 1, 1, 1};
+      const int dx = dxs[sel], dy = dys[sel], dz = dzs[sel];
+      const uint32_t nidx = (x + dx) + bx * ((y + dy) + by * (z + dz));
+      const GatherBrick N = lod_bricks[nidx];
+      const int nx = (int)N.n[0], ny = (int)N.n[1], nz = (int)N.n[2];
+      const int px = dx > 0 ? o + vx - (tx - o) : (dx < 0 ? nx - 2 * o + vx : vx);
+      const int py = dy > 0 ? o + vy - (ty - o) : (dy < 0 ? ny - 2 * o + vy : vy);
+      const int pz = dz > 0 ? o + vz - (tz - o) : (dz < 0 ? nz - 2 * o + vz : vz);
+      const bool earlier = dx < 0 || dy < 0 || dz < 0;   // already filled: final state
+      const bool core = px >= o && px < nx - o && py >= o && py < ny - o && pz >= o && pz < nz - o;
+      if (earlier || core) val = pool[N.offset + (uint64_t)px + (uint64_t)nx * ((uint64_t)py + (uint64_t)ny * pz)];
+    }
+    pool[B.offset + (uint64_t)vx + (uint64_t)tx * ((uint64_t)vy + (uint64_t)ty * vz)] = val;
+  }
+}
+
+hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bricks, const uint32_t bricks[3],
+                               uint32_t ov, const uint32_t* d_wave, uint32_t wave_n, uint32_t max_shell,
+                               hipStream_t st) {
+  uint32_t gx = (max_shell + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  for (uint32_t first = 0; first < wave_n; first += 65535) {
+    const uint32_t cnt = wave_n - first < 65535 ? wave_n - first : 65535;
+    if (bpv == 1)
+      hipLaunchKernelGGL(fill_overlap_wave<uint8_t>, dim3(gx, cnt), dim3(256), 0, st, (uint8_t*)pool, d_lod_bricks,
+                         bricks[0], bricks[1], bricks[2], ov, d_wave + first);
+    else
+      hipLaunchKernelGGL(fill_overlap_wave<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (uint16_t*)pool,
+                         d_lod_bricks, bricks[0], bricks[1], bricks[2], ov, d_wave + first);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
+                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, hipStream_t st) {
+  uint32_t gx = (max_vox + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  for (uint32_t first = 0; first < nb; first += 65535) {
+    const uint32_t cnt = nb - first < 65535 ? nb - first : 65535;
+    dim3 grid(gx, cnt);
+    if (bpv == 1)
+      hipLaunchKernelGGL(gather_bricks<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)lin, dims[0], dims[1],
+                         dims[2], d_bricks + first, (uint8_t*)pool, core[0], core[1], core[2], ov, clamp);
+    else
+      hipLaunchKernelGGL(gather_bricks<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)lin, dims[0], dims[1],
+                         dims[2], d_bricks + first, (uint16_t*)pool, core[0], core[1], core[2], ov, clamp);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
+                             hipStream_t st) {
+  const uint64_t n = dd[0] * dd[1] * dd[2];
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (bpv == 1)
+    hipLaunchKernelGGL(downsample<uint8_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint8_t*)src, sd[0], sd[1],
+                       sd[2], (uint8_t*)dst, dd[0], dd[1], dd[2]);
+  else
+    hipLaunchKernelGGL(downsample<uint16_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint16_t*)src, sd[0],
+                       sd[1], sd[2], (uint16_t*)dst, dd[0], dd[1], dd[2]);
+  return hipGetLastError();
+}
+
+hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t* out,
+                         hipStream_t st) {
+  for (uint32_t first = 0; first < nb; first += 1u << 30) {
+    const uint32_t cnt = nb - first;
+    if (bpv == 1)
+      hipLaunchKernelGGL(brick_minmax<uint8_t>, dim3(cnt), dim3(256), 0, st, (const uint8_t*)pool, d_bricks + first,
+                         out + 2 * first);
+    else
+      hipLaunchKernelGGL(brick_minmax<uint16_t>, dim3(cnt), dim3(256), 0, st, (const uint16_t*)pool, d_bricks + first,
+                         out + 2 * first);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_max_gradient(const void* lin, int bpv, const uint64_t d[3], int clamp, double* out, uint32_t blocks,
+                               hipStream_t st) {
+  if (bpv == 1)
+    hipLaunchKernelGGL(max_gradient<uint8_t>, dim3(blocks), dim3(256), 0, st, (const uint8_t*)lin, d[0], d[1], d[2],
+                       clamp, 255.0, out);
+  else
+    hipLaunchKernelGGL(max_gradient<uint16_t>, dim3(blocks), dim3(256), 0, st, (const uint16_t*)lin, d[0], d[1], d[2],
+                       clamp, 65535.0, out);
+  return hipGetLastError();
+}
+
+}  // namespace tvk

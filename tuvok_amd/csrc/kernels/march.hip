@@ -1,0 +1,412 @@
+// Fixed-step brick raycaster for gfx950.
+//
+// One wavefront = one 8x8 pixel tile = 64 rays (one per lane); a 256-thread
+// workgroup owns a 16x16 pixel group.  Workgroups are remapped so that every
+// XCD (own L2) renders a contiguous band of the screen.
+//
+// Per tile, the visible non-empty bricks of the frame are scanned in
+// compositing order 64 at a time: every lane tests one brick's conservative
+// screen rectangle against the tile, a ballot yields the candidate mask, and
+// the wave walks the set bits in order (wave-uniform brick index -> scalar
+// loads of the 128-byte brick record).  For each brick every lane intersects
+// its own ray with the brick box and marches the segment exactly as
+// GLRaycaster does it per brick (GLRaycaster.cpp:383-478 with
+// GLRaycaster-*-FS.glsl): fp16-rounded eye-space entry, iStepCount =
+// int(L/step)+1, fixed texture-space increment, trilinear fetch from the
+// brick-contiguous pool with GL_CLAMP border emulation, TF lookup (1D table in
+// LDS, 2D table via L2), opacity correction, under-compositing, local
+// 'a >= 0.99' exit, then (ONE_MINUS_DST_ALPHA, ONE) blending into the ray's
+// accumulator.  With ert_eps > 0 a ray stops once its total transmittance is
+// below ert_eps and a wave leaves the brick loop once every lane has stopped.
+//
+// This file is compiled with -ffp-contract=off: the float operations are the
+// ones of the host planner and of the reference restatement, in the same
+// order, so step counts, sample positions and TF indices are reproducible.
+#include <hip/hip_runtime.h>
+#include "../tvk_types.hpp"
+
+namespace tvk {
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float a, float b, float c) { V3 r; r.x = a; r.y = b; r.z = c; return r; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return a / length(a); }
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+__device__ __forceinline__ float pow8(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x4; }
+__device__ __forceinline__ float fp16_round(float x) { return (float)(_Float16)x; }
+
+// value of a stored voxel as the unorm texture returns it
+template <typename T> struct Unorm;
+template <> struct Unorm<uint8_t> { static __device__ __forceinline__ float inv() { return 1.0f / 255.0f; } };
+template <> struct Unorm<uint16_t> { static __device__ __forceinline__ float inv() { return 1.0f / 65535.0f; } };
+
+// texture3D with GL_LINEAR and GL_CLAMP (border colour 0) or GL_CLAMP_TO_EDGE
+template <typename T>
+__device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p) {
+  const float inv = Unorm<T>::inv();
+  int i0[3];
+  float f[3];
+  int n[3] = {nx, ny, nz};
+  float pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float s = fminf(fmaxf(pc[a], 0.0f), 1.0f);
+    float u = s * (float)n[a] - 0.5f;
+    float fl = floorf(u);
+    i0[a] = (int)fl;
+    f[a] = u - fl;
+  }
+  // per-axis tap indices (clamped for the address) and validity (border)
+  int xa = i0[0], xb = i0[0] + 1, ya = i0[1], yb = i0[1] + 1, za = i0[2], zb = i0[2] + 1;
+  bool vxa = xa >= 0, vxb = xb < nx, vya = ya >= 0, vyb = yb < ny, vza = za >= 0, vzb = zb < nz;
+  if (edge) { vxa = vxb = vya = vyb = vza = vzb = true; }
+  xa = max(xa, 0); xb = min(xb, nx - 1); ya = max(ya, 0); yb = min(yb, ny - 1); za = max(za, 0); zb = min(zb, nz - 1);
+  xa = min(xa, nx - 1); xb = max(xb, 0); ya = min(ya, ny - 1); yb = max(yb, 0); za = min(za, nz - 1); zb = max(zb, 0);
+  const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+  const size_t ra = (size_t)ya * sy, rb = (size_t)yb * sy, pa = (size_t)za * sz, pb = (size_t)zb * sz;
+  float c000 = (vxa && vya && vza) ? (float)d[pa + ra + xa] * inv : 0.0f;
+  float c100 = (vxb && vya && vza) ? (float)d[pa + ra + xb] * inv : 0.0f;
+  float c010 = (vxa && vyb && vza) ? (float)d[pa + rb + xa] * inv : 0.0f;
+  float c110 = (vxb && vyb && vza) ? (float)d[pa + rb + xb] * inv : 0.0f;
+  float c001 = (vxa && vya && vzb) ? (float)d[pb + ra + xa] * inv : 0.0f;
+  float c101 = (vxb && vya && vzb) ? (float)d[pb + ra + xb] * inv : 0.0f;
+  float c011 = (vxa && vyb && vzb) ? (float)d[pb + rb + xa] * inv : 0.0f;
+  float c111 = (vxb && vyb && vzb) ? (float)d[pb + rb + xb] * inv : 0.0f;
+  float c00 = c000 * (1.0f - f[0]) + c100 * f[0];
+  float c10 = c010 * (1.0f - f[0]) + c110 * f[0];
+  float c01 = c001 * (1.0f - f[0]) + c101 * f[0];
+  float c11 = c011 * (1.0f - f[0]) + c111 * f[0];
+  float c0 = c00 * (1.0f - f[1]) + c10 * f[1];
+  float c1 = c01 * (1.0f - f[1]) + c11 * f[1];
+  return c0 * (1.0f - f[2]) + c1 * f[2];
+}
+
+// Volume3D.glsl:43-53 (6-tap central difference at +-1 brick voxel, halved)
+template <typename T>
+__device__ __forceinline__ V3 gradient(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 c, V3 h) {
+  float xp = sample_brick<T>(d, nx, ny, nz, edge, c + mk(+h.x, 0, 0));
+  float xm = sample_brick<T>(d, nx, ny, nz, edge, c + mk(-h.x, 0, 0));
+  float yp = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, -h.y, 0));
+  float ym = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, +h.y, 0));
+  float zp = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, 0, +h.z));
+  float zm = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, 0, -h.z));
+  return mk(xm - xp, yp - ym, zm - zp) / 2.0f;
+}
+
+__device__ __forceinline__ V3 reflect(V3 I, V3 N) { return I - N * (2.0f * dot(N, I)); }
+
+// lighting.glsl:33-50 (eye at the origin)
+__device__ __forceinline__ V3 lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {
+  V3 view = normalize(mk(0, 0, 0) - pos);
+  V3 R = normalize(reflect(view, n));
+  float dd = fmaxf(fabsf(dot(n, L)), 0.0f);
+  float ss = pow8(fmaxf(dot(R, L), 0.0f));
+  V3 c = amb + dif * dd + spc * ss;
+  return mk(clamp01(c.x), clamp01(c.y), clamp01(c.z));
+}
+
+// Compose-FS.glsl Lighting()
+__device__ __forceinline__ V3 iso_lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {
+  n.z = fabsf(n.z);
+  V3 view = normalize(mk(0, 0, 0) - pos);
+  V3 R = normalize(reflect(view, n));
+  float dd = fmaxf(fabsf(dot(n, L * -1.0f)), 0.0f);
+  float ss = pow8(fmaxf(dot(R, L), 0.0f));
+  V3 c = amb + dif * dd + spc * ss;
+  return mk(clamp01(c.x), clamp01(c.y), clamp01(c.z));
+}
+
+__device__ __forceinline__ V3 xform_point(V3 p, const float* M) {
+  // (p,1) * M, rows of 4
+  return mk(p.x * M[0] + p.y * M[4] + p.z * M[8] + 1.0f * M[12],
+            p.x * M[1] + p.y * M[5] + p.z * M[9] + 1.0f * M[13],
+            p.x * M[2] + p.y * M[6] + p.z * M[10] + 1.0f * M[14]);
+}
+__device__ __forceinline__ V3 xform_dir(V3 p, const float* M) {
+  return mk(p.x * M[0] + p.y * M[4] + p.z * M[8] + 0.0f * M[12],
+            p.x * M[1] + p.y * M[5] + p.z * M[9] + 0.0f * M[13],
+            p.x * M[2] + p.y * M[6] + p.z * M[10] + 0.0f * M[14]);
+}
+__device__ __forceinline__ V3 xform_e2t(V3 p, const float* E) {
+  // E holds rows 0..3 of the eye->texture matrix, 3 columns each
+  return mk(p.x * E[0] + p.y * E[3] + p.z * E[6] + 1.0f * E[9],
+            p.x * E[1] + p.y * E[4] + p.z * E[7] + 1.0f * E[10],
+            p.x * E[2] + p.y * E[5] + p.z * E[8] + 1.0f * E[11]);
+}
+
+__device__ __forceinline__ V3 normal_eye(V3 g, const float* nm, V3 ds) {
+  V3 s = g * ds;
+  V3 n = mk(nm[0] * s.x + nm[1] * s.y + nm[2] * s.z, nm[3] * s.x + nm[4] * s.y + nm[5] * s.z,
+            nm[6] * s.x + nm[7] * s.y + nm[8] * s.z);
+  float l = length(n);
+  if (l > 0.0f) n = n / l;
+  return n;
+}
+
+template <int MODE, bool LIT, typename T>
+__global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
+  extern __shared__ float4 s_tf[];  // 1D TF as floats (tf_n entries, MODE 0 only)
+
+  // XCD-aware remap: consecutive workgroups go round-robin over the 8 XCDs;
+  // give each XCD a contiguous range of 16x16 groups.
+  const uint32_t lin = blockIdx.x;
+  const uint32_t per_xcd = A.groups_padded >> 3;
+  const uint32_t g = (lin & 7u) * per_xcd + (lin >> 3);
+  if (g >= A.groups_total) return;  // uniform for the whole workgroup
+
+  if (MODE == 0) {
+    for (int i = threadIdx.x; i < A.tf_n; i += 256) {
+      const uint8_t* t = A.tf + 4 * i;
+      s_tf[i] = make_float4((float)t[0] / 255.0f, (float)t[1] / 255.0f, (float)t[2] / 255.0f, (float)t[3] / 255.0f);
+    }
+    __syncthreads();
+  }
+
+  int v = 0;
+  while (v + 1 < A.nviews && g >= A.views[v + 1].group_offset) ++v;
+  const ViewConst* VC = &A.views[v];
+  const uint32_t gl = g - VC->group_offset;
+  const uint32_t gx = gl % VC->groups_x, gy = gl / VC->groups_x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tx = (int)gx * 2 + (wave & 1), ty = (int)gy * 2 + (wave >> 1);
+  const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+  const int W = (int)VC->W, H = (int)VC->H;
+  const bool inside = px < W && py < H;
+  const int rx0 = tx * 8, ry0 = ty * 8, rx1 = tx * 8 + 7, ry1 = ty * 8 + 7;
+
+  // eye-space ray through the pixel centre (eye at the origin)
+  const float* ip = VC->inv_proj;
+  const float n0 = ((float)px + 0.5f) * 2.0f / (float)W - 1.0f;
+  const float n1 = ((float)py + 0.5f) * 2.0f / (float)H - 1.0f;
+  float e[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) e[c] = n0 * ip[c] + n1 * ip[4 + c] + -1.0f * ip[8 + c] + 1.0f * ip[12 + c];
+  const V3 d = mk(e[0] / e[3], e[1] / e[3], e[2] / e[3]);
+  const V3 o_m = xform_point(mk(0, 0, 0), VC->inv_mv);
+  const V3 d_m = xform_dir(d, VC->inv_mv);
+
+  const V3 amb = mk(A.amb[0], A.amb[1], A.amb[2]), dif = mk(A.dif[0], A.dif[1], A.dif[2]);
+  const V3 spc = mk(A.spc[0], A.spc[1], A.spc[2]), ldir = mk(A.ldir[0], A.ldir[1], A.ldir[2]);
+  const V3 dsc = mk(A.dscale[0], A.dscale[1], A.dscale[2]);
+  const float eps = A.ert_eps;
+  const bool edge = A.clamp_edge != 0;
+
+  float dst0 = 0.f, dst1 = 0.f, dst2 = 0.f, dst3 = 0.f;
+  bool done = !inside;
+  uint32_t ns = 0;
+  // isosurface state
+  float best_depth = 1.0f, hit_param = 0.0f;
+  bool have_hit = false;
+  V3 hit_pos = mk(0, 0, 0), hit_n = mk(0, 0, 0);
+
+  const FrameBrick* list = A.list + VC->list_offset;
+  const int nlist = (int)VC->nlist;
+  for (int base = 0; base < nlist; base += 64) {
+    const int j = base + lane;
+    bool cand = false;
+    if (j < nlist) {
+      const int16_t* r = list[j].rect;
+      cand = !(r[2] < rx0 || r[0] > rx1 || r[3] < ry0 || r[1] > ry1);
+    }
+    uint64_t mask = __ballot(cand);
+    while (mask) {
+      const int k = __ffsll((unsigned long long)mask) - 1;
+      mask &= mask - 1;
+      const FrameBrick* fb = &list[__builtin_amdgcn_readfirstlane(base + k)];
+      if (MODE != 2 && eps > 0.0f && (1.0f - dst3) < eps) done = true;
+      if (done) continue;
+      // slab test in model space (t along the eye-space direction d)
+      float tn = -INFINITY, tf = INFINITY;
+      bool ok = true;
+      {
+        const float bmn[3] = {fb->bmin[0], fb->bmin[1], fb->bmin[2]};
+        const float bmx[3] = {fb->bmax[0], fb->bmax[1], fb->bmax[2]};
+        const float om[3] = {o_m.x, o_m.y, o_m.z}, dm[3] = {d_m.x, d_m.y, d_m.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (dm[a] == 0.0f) {
+            if (om[a] < bmn[a] || om[a] > bmx[a]) ok = false;
+            continue;
+          }
+          float inv = 1.0f / dm[a];
+          float t0 = (bmn[a] - om[a]) * inv, t1 = (bmx[a] - om[a]) * inv;
+          tn = fmaxf(tn, fminf(t0, t1));
+          tf = fminf(tf, fmaxf(t0, t1));
+        }
+      }
+      ok = ok && tf > tn && tf > 0.0f;
+      const float te = fmaxf(tn, 1.0f);  // near-plane entry when the front face is clipped
+      if (A.touched) {
+        // every ray that reaches a brick takes at least one sample there
+        if (__ballot(ok && tf > te) != 0 && lane == __builtin_amdgcn_readfirstlane(lane))
+          A.touched[(size_t)v * A.ntoc + fb->toc] = 1u;
+      }
+      if (!ok || tf <= te) continue;
+      V3 entry = d * te;
+      const V3 exitp = d * tf;
+      entry = mk(fp16_round(entry.x), fp16_round(entry.y), fp16_round(entry.z));
+      const float* E = fb->e2t;
+      const V3 entryTex = xform_e2t(entry, E), exitTex = xform_e2t(exitp, E);
+      const V3 dir = exitp - entry;
+      const float L = length(dir);
+      const float rstep = fb->ray_step;
+      const int steps = (int)(L / rstep) + 1;
+      const V3 incTex = (exitTex - entryTex) / (L / rstep);
+      const T* __restrict__ bd = reinterpret_cast<const T*>(A.pool) + fb->offset;
+      const int nx = (int)fb->n[0], ny = (int)fb->n[1], nz = (int)fb->n[2];
+      const V3 vstep = mk(fb->vstep[0], fb->vstep[1], fb->vstep[2]);
+      const float sscale = fb->step_scale;
+      V3 pos = entryTex;
+
+      if (MODE == 2) {
+        // GLRaycaster-ISO-FS.glsl:56-105 + RefineIsosurface.glsl:37-52
+        bool found = false;
+        for (int i = 0; i < steps; ++i) {
+          ++ns;
+          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= A.iso_n) { found = true; break; }
+          pos = pos + incTex;
+        }
+        if (!found) continue;
+        V3 rd = incTex / 2.0f;
+        pos = pos - rd;
+        for (int kk = 0; kk < A.refine_steps; ++kk) {
+          rd = rd / 2.0f;
+          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= A.iso_n) pos = pos - rd; else pos = pos + rd;
+        }
+        const float param = length(pos - entryTex) / length(exitTex - entryTex);
+        const V3 he = entry * (1.0f - param) + exitp * param;
+        const float zf = VC->zfar, zn = VC->znear;
+        const float pxp = zf / (zf - zn), pyp = (zf * zn / (zn - zf));
+        float depth = pxp + (pyp / -he.z);
+        depth = fminf(fmaxf(depth, 0.0f), 1.0f);
+        if (depth < best_depth) {
+          best_depth = depth;
+          have_hit = true;
+          hit_pos = he;
+          hit_param = param;
+          hit_n = normal_eye(gradient<T>(bd, nx, ny, nz, edge, pos, vstep), VC->nm, dsc);
+        }
+        continue;
+      }
+
+      float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+      V3 cur = entry;
+      const V3 rayInc = dir / L * rstep;
+      for (int i = 0; i < steps; ++i) {
+        ++ns;
+        float s0, s1, s2, s3;
+        const float val = sample_brick<T>(bd, nx, ny, nz, edge, pos);
+        if (MODE == 0) {
+          const float cc = fminf(fmaxf(val * A.tf_scale, 0.0f), 1.0f);
+          const int ti = min((int)floorf(cc * (float)A.tf_n), A.tf_n - 1);
+          const float4 tv = s_tf[ti];
+          s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
+          if (LIT) {
+            const V3 gg = gradient<T>(bd, nx, ny, nz, edge, pos, vstep);
+            const V3 nn = normal_eye(gg, VC->nm, dsc);
+            const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
+            s0 = lc.x; s1 = lc.y; s2 = lc.z;
+          }
+          const float tt = 1.0f - s3;
+          s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+        } else {
+          const V3 gg = gradient<T>(bd, nx, ny, nz, edge, pos, vstep);
+          const float gm = length(gg);
+          const float sx = fminf(fmaxf(val * A.tf_scale, 0.0f), 1.0f);
+          const float sy = fminf(fmaxf(1.0f - gm * A.grad_scale, 0.0f), 1.0f);
+          const int xi = min((int)floorf(sx * (float)A.tf_w), A.tf_w - 1);
+          const int yi = min((int)floorf(sy * (float)A.tf_h), A.tf_h - 1);
+          const uint32_t w = *reinterpret_cast<const uint32_t*>(A.tf + 4 * ((size_t)yi * (size_t)A.tf_w + (size_t)xi));
+          s0 = (float)(w & 0xffu) / 255.0f;
+          s1 = (float)((w >> 8) & 0xffu) / 255.0f;
+          s2 = (float)((w >> 16) & 0xffu) / 255.0f;
+          s3 = (float)(w >> 24) / 255.0f;
+          if (LIT) {
+            const V3 nn = normal_eye(gg, VC->nm, dsc);
+            const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
+            const float tt = 1.0f - s3;
+            s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+            s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
+          } else {
+            const float tt = 1.0f - s3;
+            s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+          }
+        }
+        // UnderCompositing (Compositing.glsl:33-38)
+        const float wgt = (1.0f - c3) * s3;
+        c0 += s0 * wgt;
+        c1 += s1 * wgt;
+        c2 += s2 * wgt;
+        c3 += s3 * (1.0f - c3);
+        if (LIT) cur = cur + rayInc;
+        if (c3 >= 0.99f) break;
+        if (eps > 0.0f && (1.0f - dst3) * (1.0f - c3) < eps) break;
+        pos = pos + incTex;
+      }
+      // blend (ONE_MINUS_DST_ALPHA, ONE) into the RGBA32F target
+      const float t = 1.0f - dst3;
+      dst0 = c0 * t + dst0;
+      dst1 = c1 * t + dst1;
+      dst2 = c2 * t + dst2;
+      dst3 = c3 * t + dst3;
+    }
+    if (__ballot(!done) == 0) break;  // every ray of the tile has terminated
+  }
+
+  if (A.counter) {
+    uint32_t s = ns;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0 && s) atomicAdd(A.counter, (unsigned long long)s);
+  }
+  if (!inside) return;
+  float4 res;
+  if (MODE == 2) {
+    if (have_hit && hit_param != 0.0f) {
+      const V3 isoc = mk(A.iso_col[0], A.iso_col[1], A.iso_col[2]);
+      const V3 c = iso_lighting(hit_pos, hit_n, amb, dif * isoc, spc, ldir);
+      res = make_float4(c.x, c.y, c.z, 1.0f);
+    } else {
+      res = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  } else {
+    res = make_float4(dst0, dst1, dst2, dst3);
+  }
+  reinterpret_cast<float4*>(A.out)[(size_t)VC->out_offset + (size_t)py * (size_t)W + (size_t)px] = res;
+}
+
+// host-side launcher (extern "C++" within the library)
+hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st) {
+  dim3 grid(a.groups_padded), block(256);
+  const size_t lds = a.mode == 0 ? (size_t)a.tf_n * sizeof(float4) : 0;
+  const int key = a.mode * 4 + (a.lighting ? 2 : 0) + (bpv == 2 ? 1 : 0);
+  switch (key) {
+#define TVK_CASE(M, L, T, K) \
+    case K: hipLaunchKernelGGL((march_kernel<M, L, T>), grid, block, lds, st, a); break;
+    TVK_CASE(0, false, uint8_t, 0)
+    TVK_CASE(0, false, uint16_t, 1)
+    TVK_CASE(0, true, uint8_t, 2)
+    TVK_CASE(0, true, uint16_t, 3)
+    TVK_CASE(1, false, uint8_t, 4)
+    TVK_CASE(1, false, uint16_t, 5)
+    TVK_CASE(1, true, uint8_t, 6)
+    TVK_CASE(1, true, uint16_t, 7)
+    TVK_CASE(2, false, uint8_t, 8)
+    TVK_CASE(2, false, uint16_t, 9)
+    TVK_CASE(2, false, uint8_t, 10)
+    TVK_CASE(2, false, uint16_t, 11)
+#undef TVK_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tvk

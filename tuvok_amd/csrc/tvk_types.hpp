@@ -1,0 +1,67 @@
+// Structures shared by the host planner and the HIP kernels.
+#pragma once
+#include <stdint.h>
+
+namespace tvk {
+
+// One entry of a frame's visible, non-empty brick list, in compositing order.
+// Everything the march needs for the brick is here, so a wavefront reads it
+// with a few scalar loads.  128 bytes.
+struct FrameBrick {
+  float e2t[12];        // eye -> texture affine, rows 0..3 of the row-vector 4x4 (xyz columns)
+  float bmin[3];        // model-space box of the brick (GLRaycaster::RenderBox)
+  float bmax[3];
+  float ray_step;       // fRayStepsize
+  float step_scale;     // fStepScale (opacity-correction exponent)
+  float vstep[3];       // vVoxelStepsize (1 / stored voxel count)
+  uint32_t n[3];        // stored voxel counts (incl. overlap)
+  uint64_t offset;      // element offset of the brick in the pool
+  int16_t rect[4];      // conservative pixel rectangle x0,y0,x1,y1 (inclusive)
+  uint32_t toc;         // TOC index of the brick
+  uint32_t pad;
+};
+static_assert(sizeof(FrameBrick) == 128, "FrameBrick layout");
+
+// Per-view constants.
+struct ViewConst {
+  float inv_proj[16];   // inverse projection (row-vector convention)
+  float inv_mv[16];     // inverse modelview
+  float nm[9];          // gl_NormalMatrix, applied to column vectors
+  float znear, zfar;
+  uint32_t list_offset; // first FrameBrick of this view
+  uint32_t nlist;       // number of FrameBricks of this view
+  uint32_t W, H;
+  uint32_t out_offset;  // first pixel of this view in the output buffer
+  uint32_t tiles_x;     // 8x8 tiles per row
+  uint32_t groups_x;    // 16x16 block groups per row
+  uint32_t groups;      // 16x16 block groups of the view
+  uint32_t group_offset;// first 16x16 group of this view in the launch
+  uint32_t pad[2];
+};
+
+// One brick of a LOD for the bricking / min-max kernels.
+struct GatherBrick {
+  uint32_t c[3];        // brick coordinates
+  uint32_t n[3];        // stored size
+  uint64_t offset;      // element offset in the pool
+};
+
+struct MarchArgs {
+  const uint8_t* pool;
+  const FrameBrick* list;
+  const ViewConst* views;
+  const uint8_t* tf;            // RGBA8 table (1D: tf_n entries; 2D: tf_w x tf_h)
+  float* out;                   // RGBA32F
+  unsigned long long* counter;  // composited-sample counter or nullptr
+  uint32_t* touched;            // [view * ntoc + toc] = 1 once a sample is taken in the brick (or nullptr)
+  uint32_t ntoc;
+  int nviews;
+  int tf_n, tf_w, tf_h;
+  float tf_scale, grad_scale, iso_n, ert_eps;
+  float dscale[3], amb[3], dif[3], spc[3], ldir[3], iso_col[3];
+  int refine_steps, clamp_edge, lighting, mode;
+  uint32_t groups_total;        // sum of groups over the views
+  uint32_t groups_padded;       // rounded up to a multiple of 8 (XCD count)
+};
+
+}  // namespace tvk

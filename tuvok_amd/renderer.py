@@ -1,0 +1,316 @@
+"""HIPRaycaster: the AbstrRenderer-shaped front end of the HIP raycasting path.
+
+Method names follow Tuvok's AbstrRenderer (Renderer/AbstrRenderer.h:112-881)
+so code written against GLRaycaster/GLGridLeaper maps one to one:
+
+    ren = HIPRaycaster()
+    ren.LoadVolume(vol, brick=(66,)*3, overlap=1)      # RAWConverter + UVF load
+    ren.Set1DTrans(tf.GetByteArray())                   # AbstrRenderer::Set1DTrans
+    ren.SetRendermode(RM_1DTRANS); ren.SetUseLighting(False)
+    ren.SetViewParameters(45, 0.01, 1000); ren.SetViewPosition((0,0,2))
+    ren.Resize(512, 512)
+    img = ren.Paint()                                   # RGBA32F, premultiplied, GL row order
+
+Every call goes through the C ABI of libtvkhip.so; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import RM_1DTRANS, RM_2DTRANS, RM_ISOSURFACE, TvkError  # noqa: F401
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def rotation_y(angle):
+    """MATRIX4::RotationY (Basics/Vectors.h:994-1002), row-vector layout."""
+    c, s = np.float32(np.cos(angle)), np.float32(np.sin(angle))
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = c, -s, s, c
+    return m
+
+
+class HIPRaycaster:
+    def __init__(self, device=0):
+        self._L = _abi.load()
+        h = C.c_void_p()
+        rc = self._L.tvk_create(C.byref(h), device)
+        if rc != 0:
+            raise TvkError(rc, "tvk_create failed (no GPU?)")
+        self._h = h
+        self._p = _abi.Params()
+        self._L.tvk_default_params(C.byref(self._p))
+        self._p.lod = 0      # RT_CAPTURE renders LOD 0 (AbstrRenderer.cpp:1188-1190)
+        self._eye = np.array([0, 0, 1.6], np.float32)   # AbstrRenderer.cpp:63-69
+        self._at = np.array([0, 0, 0], np.float32)
+        self._up = np.array([0, 1, 0], np.float32)
+        self._fov, self._znear, self._zfar = 50.0, 0.01, 1000.0
+        self._w, self._h_px = 0, 0
+        self._rot = np.eye(4, dtype=np.float32)
+        self._user = None
+        self.last_stats = _abi.Stats()
+
+    # -- plumbing -----------------------------------------------------------
+    def _chk(self, rc):
+        if rc != 0:
+            raise TvkError(rc, self._L.tvk_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.tvk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- dataset (RAWConverter / UVFDataset / BrickedDataset) ---------------
+    def LoadVolume(self, vol, brick, overlap, clamp_to_edge=False, max_lods=0, keep_linear=False):
+        vol = np.ascontiguousarray(vol)
+        dt = {np.dtype(np.uint8): _abi.TVK_U8, np.dtype(np.uint16): _abi.TVK_U16}[vol.dtype]
+        dims = np.array(vol.shape[::-1], np.uint64)
+        b = np.array(brick, np.uint32)
+        self._chk(self._L.tvk_load_volume(self._h, vol.ctypes.data, dt, dims.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          b.ctypes.data_as(C.POINTER(C.c_uint32)), overlap, int(clamp_to_edge),
+                                          max_lods, int(keep_linear)))
+
+    def GenerateVolume(self, kind, seed, dtype, dims, brick, overlap, clamp_to_edge=False, max_lods=0,
+                       keep_linear=False):
+        dt = {np.dtype(np.uint8): _abi.TVK_U8, np.dtype(np.uint16): _abi.TVK_U16}[np.dtype(dtype)]
+        d = np.array(dims, np.uint64)
+        b = np.array(brick, np.uint32)
+        self._chk(self._L.tvk_generate_volume(self._h, kind, seed, dt, d.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                              b.ctypes.data_as(C.POINTER(C.c_uint32)), overlap, int(clamp_to_edge),
+                                              max_lods, int(keep_linear)))
+
+    def DownloadVolume(self):
+        d = self.GetDatasetDesc()
+        dt = np.uint8 if d.bit_width == 8 else np.uint16
+        out = np.empty((d.dims[2], d.dims[1], d.dims[0]), dt)
+        self._chk(self._L.tvk_download_volume(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def GetDatasetDesc(self):
+        d = _abi.DatasetDesc()
+        self._chk(self._L.tvk_dataset_info(self._h, C.byref(d)))
+        return d
+
+    def GetLODInfo(self, lod):
+        size = np.zeros(3, np.uint64)
+        bricks = np.zeros(3, np.uint64)
+        off = C.c_uint64()
+        asp = np.zeros(3, np.float64)
+        self._chk(self._L.tvk_lod_info(self._h, lod, size.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                       bricks.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(off),
+                                       asp.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(size=tuple(int(v) for v in size), bricks=tuple(int(v) for v in bricks), offset=off.value,
+                    aspect=tuple(float(v) for v in asp))
+
+    def GetBrickInfo(self, toc):
+        c = np.zeros(4, np.uint64)
+        s = np.zeros(3, np.uint32)
+        off = C.c_uint64()
+        self._chk(self._L.tvk_brick_info(self._h, toc, c.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         s.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(off)))
+        return dict(coords=tuple(int(v) for v in c), size=tuple(int(v) for v in s), offset=off.value)
+
+    def GetBrickMetadata(self, lod, index):
+        c = np.zeros(3, np.float32)
+        e = np.zeros(3, np.float32)
+        n = np.zeros(3, np.uint32)
+        self._chk(self._L.tvk_brick_md(self._h, lod, index, _fp(c), _fp(e), n.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return c, e, n
+
+    def MaxMinForKey(self, toc):
+        mm = np.zeros(2, np.float64)
+        self._chk(self._L.tvk_brick_minmax(self._h, toc, mm.ctypes.data_as(C.POINTER(C.c_double))))
+        return float(mm[0]), float(mm[1])
+
+    def GetBrick(self, toc):
+        info = self.GetBrickInfo(toc)
+        nx, ny, nz = info["size"]
+        dt = np.uint8 if self.GetDatasetDesc().bit_width == 8 else np.uint16
+        out = np.empty((nz, ny, nx), dt)
+        self._chk(self._L.tvk_get_brick(self._h, toc, out.ctypes.data, out.nbytes))
+        return out
+
+    def GetPool(self):
+        d = self.GetDatasetDesc()
+        out = np.empty(d.pool_bytes, np.uint8)
+        self._chk(self._L.tvk_get_pool(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def MaxGradientMagnitude(self):
+        v = C.c_float()
+        self._chk(self._L.tvk_max_gradient(self._h, C.byref(v)))
+        return v.value
+
+    # -- transfer functions ---------------------------------------------------
+    def Set1DTrans(self, rgba8):
+        t = np.ascontiguousarray(rgba8, np.uint8).reshape(-1, 4)
+        self._chk(self._L.tvk_set_tf1d(self._h, t.ctypes.data_as(C.POINTER(C.c_uint8)), t.shape[0]))
+
+    def Set2DTrans(self, rgba8):
+        t = np.ascontiguousarray(rgba8, np.uint8)
+        h, w = t.shape[:2]
+        self._chk(self._L.tvk_set_tf2d(self._h, t.ctypes.data_as(C.POINTER(C.c_uint8)), w, h))
+
+    # -- state (AbstrRenderer setters) -----------------------------------------
+    def _push(self):
+        self._chk(self._L.tvk_set_params(self._h, C.byref(self._p)))
+
+    def SetRendermode(self, mode):
+        self._p.mode = mode
+        self._push()
+
+    def SetUseLighting(self, on):
+        self._p.lighting = int(bool(on))
+        self._push()
+
+    def SetSampleRateModifier(self, srm):
+        self._p.srm = float(srm)
+        self._push()
+
+    def SetIsoValue(self, v):
+        self._p.isovalue = float(v)
+        self._push()
+
+    def SetLOD(self, lod):
+        """Fixed LOD (>= 0) or ComputeMinLODForCurrentView (-1)."""
+        self._p.lod = int(lod)
+        self._push()
+
+    def SetGradientScale(self, s):
+        self._p.grad_scale = float(s)
+        self._push()
+
+    def SetTFScale(self, s):
+        self._p.tf_scale = float(s)
+        self._push()
+
+    def SetRefineSteps(self, n):
+        self._p.refine_steps = int(n)
+        self._push()
+
+    def SetEarlyRayTermination(self, eps):
+        self._p.ert_eps = float(eps)
+        self._push()
+
+    def SetClampToEdge(self, on):
+        self._p.clamp_to_edge = int(bool(on))
+        self._push()
+
+    def SetParams(self, **kw):
+        for k, v in kw.items():
+            f = getattr(self._p, k)
+            if hasattr(f, "__len__"):
+                f[:] = list(v)
+            else:
+                setattr(self._p, k, v)
+        self._push()
+
+    def GetParams(self):
+        return self._p
+
+    def SetCountSamples(self, on):
+        self._chk(self._L.tvk_set_count_samples(self._h, int(bool(on))))
+
+    # -- view -----------------------------------------------------------------
+    def SetViewParameters(self, fov, znear, zfar):
+        self._fov, self._znear, self._zfar = float(fov), float(znear), float(zfar)
+        self._camera()
+
+    def SetViewPosition(self, eye, at=(0, 0, 0), up=(0, 1, 0)):
+        self._eye = np.asarray(eye, np.float32)
+        self._at = np.asarray(at, np.float32)
+        self._up = np.asarray(up, np.float32)
+        self._camera()
+
+    def Resize(self, w, h):
+        self._w, self._h_px = int(w), int(h)
+        self._camera()
+
+    def SetRotation(self, rot):
+        self._rot = np.ascontiguousarray(rot, np.float32).reshape(4, 4)
+        if self._w:
+            self._chk(self._L.tvk_set_rotation(self._h, _fp(self._rot)))
+
+    def SetUserMatrices(self, view, proj, rot=None, trans=None):
+        """AbstrRenderer::SetUserMatrices with explicit row-vector matrices."""
+        v = np.ascontiguousarray(view, np.float32)
+        p = np.ascontiguousarray(proj, np.float32)
+        r = np.ascontiguousarray(np.eye(4) if rot is None else rot, np.float32)
+        t = np.ascontiguousarray(np.eye(4) if trans is None else trans, np.float32)
+        self._user = (v, p, r, t)
+        self._chk(self._L.tvk_set_view_matrices(self._h, _fp(v), _fp(r), _fp(t), _fp(p), self._w, self._h_px,
+                                                self._fov, self._znear, self._zfar))
+
+    def _camera(self):
+        if not self._w:
+            return
+        self._chk(self._L.tvk_set_camera(self._h, _fp(self._eye), _fp(self._at), _fp(self._up), self._fov,
+                                         self._znear, self._zfar, self._w, self._h_px))
+        if not np.array_equal(self._rot, np.eye(4, dtype=np.float32)):
+            self._chk(self._L.tvk_set_rotation(self._h, _fp(self._rot)))
+
+    def GetViewMatrices(self):
+        v = np.zeros(16, np.float32)
+        p = np.zeros(16, np.float32)
+        self._chk(self._L.tvk_get_view_matrices(self._h, _fp(v), _fp(p)))
+        return v.reshape(4, 4), p.reshape(4, 4)
+
+    # -- planning / rendering -------------------------------------------------------
+    def CurrentLOD(self):
+        v = C.c_int()
+        self._chk(self._L.tvk_current_lod(self._h, C.byref(v)))
+        return v.value
+
+    def PlanFrame(self):
+        """The visible-brick list of BuildSubFrameBrickList, in compositing order."""
+        n = C.c_int()
+        self._chk(self._L.tvk_plan(self._h, None, 0, C.byref(n)))
+        arr = (_abi.BrickRec * max(n.value, 1))()
+        self._chk(self._L.tvk_plan(self._h, arr, n.value, C.byref(n)))
+        return [arr[i] for i in range(n.value)]
+
+    def Paint(self, download=True):
+        out = np.empty((self._h_px, self._w, 4), np.float32) if download else None
+        self._chk(self._L.tvk_render(self._h, _fp(out) if download else None, C.byref(self.last_stats)))
+        return out
+
+    def PaintViews(self, rotations, download=True):
+        rots = np.ascontiguousarray(np.stack([np.asarray(r, np.float32).reshape(4, 4) for r in rotations]))
+        n = rots.shape[0]
+        out = np.empty((n, self._h_px, self._w, 4), np.float32) if download else None
+        self._chk(self._L.tvk_render_views(self._h, n, _fp(rots), _fp(out) if download else None,
+                                           C.byref(self.last_stats)))
+        return out
+
+    def TouchedBytes(self):
+        """Stored bytes and count of the bricks the last counted render sampled (summed over views)."""
+        b, n = C.c_uint64(), C.c_uint64()
+        self._chk(self._L.tvk_touched_bytes(self._h, C.byref(b), C.byref(n)))
+        return b.value, n.value
+
+    # -- timing ---------------------------------------------------------------------
+    def EventRecord(self, slot):
+        self._chk(self._L.tvk_event_record(self._h, slot))
+
+    def EventElapsed(self, a, b):
+        ms = C.c_float()
+        self._chk(self._L.tvk_event_elapsed(self._h, a, b, C.byref(ms)))
+        return ms.value
+
+    def Synchronize(self):
+        self._chk(self._L.tvk_synchronize(self._h))

@@ -1,0 +1,126 @@
+"""Transfer functions with the byte semantics of the reference.
+
+TransferFunction1D / TransferFunction2D keep float RGBA tables and hand the
+renderer RGBA8 tables exactly as TransferFunction1D::GetByteArray
+(IO/TransferFunction1D.cpp:311-330, truncating '(uchar)(clamp(x,0,1)*255)')
+and TransferFunction2D::GetByteArray (IO/TransferFunction2D.cpp:222-264).
+Non-zero limits follow ComputeNonZeroLimits (TransferFunction1D.cpp:362-371,
+TransferFunction2D.cpp:378-397).
+"""
+import numpy as np
+
+
+def _to_bytes(rgba, used_range=255):
+    # (unsigned char)(max(0, min(v, 1)) * cUsedRange), evaluated in float32
+    v = np.clip(np.asarray(rgba, np.float32), np.float32(0), np.float32(1)) * np.float32(used_range)
+    return v.astype(np.uint8)
+
+
+class TransferFunction1D:
+    def __init__(self, size=256):
+        self.color = np.zeros((size, 4), np.float32)
+
+    def GetSize(self):
+        return self.color.shape[0]
+
+    def SetStdFunction(self, center=0.3, inv_gradient=0.3):
+        """TransferFunction1D::SetStdFunction (TransferFunction1D.cpp:81-113): smoothstep ramp."""
+        n = self.GetSize()
+        ic = int((n - 1) * float(min(max(0.0, center), 1.0)))
+        ig = int((n - 1) * float(min(max(0.0, inv_gradient), 1.0)))
+        rs = 0 if ig // 2 > ic else ic - ig // 2
+        re = n if ig // 2 + ic > n else ic + ig // 2
+        for comp in range(4):
+            self.color[:rs, comp] = 0
+            for i in range(rs, re):
+                t = np.float32(i - ic + ig // 2) / np.float32(ig)
+                self.color[i, comp] = t * t * (3 - 2 * t)   # MathTools smoothstep
+            self.color[re:, comp] = 1
+        return self
+
+    def Set(self, rgba8):
+        """TransferFunction1D::Set(rgba8) (TransferFunction1D.cpp:146-177)."""
+        b = np.asarray(rgba8, np.uint8).reshape(-1, 4)
+        lo = b.min(axis=0).astype(np.float32)
+        hi = b.max(axis=0).astype(np.float32)
+        out = np.zeros(b.shape, np.float32)
+        for c in range(3):
+            span = hi[c] - lo[c]
+            out[:, c] = 0 if span == 0 else (b[:, c].astype(np.float32) - lo[c]) / span
+        out[:, 3] = b[:, 3].astype(np.float32) / np.float32(255)
+        self.color = out
+        return self
+
+    def GetByteArray(self, used_range=255):
+        return _to_bytes(self.color, used_range)
+
+    def GetNonZeroLimits(self):
+        nz = np.nonzero(self.color[:, 3] != 0)[0]
+        if len(nz) == 0:
+            return (self.GetSize(), 0)
+        return (int(nz[0]), int(nz[-1]))
+
+
+class TransferFunction2D:
+    """2D TF over (value, 1 - |gradient| * scale); rows index the gradient axis."""
+
+    def __init__(self, width=256, height=256):
+        self.color = np.zeros((height, width, 4), np.float32)
+
+    def GetByteArray(self, used_range=255):
+        return _to_bytes(self.color, used_range)
+
+    def GetNonZeroLimits(self):
+        a = self.GetByteArray()[:, :, 3]
+        ys, xs = np.nonzero(a)
+        h, w = a.shape
+        if len(xs) == 0:
+            return (w, 0, h, 0)
+        return (int(xs.min()), int(xs.max()), int(ys.min()), int(ys.max()))
+
+
+def tf1d_blue_white_red(size=256, alpha_start=40, alpha_max=0.6):
+    """1D TF: opacity 0 below `alpha_start`, linear to alpha_max at size-1;
+    colour is a 3-stop blue-white-red map."""
+    tf = TransferFunction1D(size)
+    i = np.arange(size, dtype=np.float64)
+    t = i / (size - 1)
+    blue = np.array([0.0, 0.0, 1.0])
+    white = np.array([1.0, 1.0, 1.0])
+    red = np.array([1.0, 0.0, 0.0])
+    col = np.where((t < 0.5)[:, None], blue + (white - blue) * (2 * t)[:, None],
+                   white + (red - white) * (2 * t - 1)[:, None])
+    alpha = np.where(i < alpha_start, 0.0, alpha_max * (i - alpha_start) / (size - 1 - alpha_start))
+    tf.color[:, :3] = col.astype(np.float32)
+    tf.color[:, 3] = alpha.astype(np.float32)
+    return tf
+
+
+def tf2d_gaussian_lobes(width=256, height=256, lobes=None):
+    """2D TF made of Gaussian lobes in (value, gradient-row) space.
+
+    Each lobe: (value centre, gradient centre, value sigma, gradient sigma,
+    peak alpha, rgb), coordinates in [0,1] of the table axes.  Row y of the
+    table is looked up with 1 - |g| * gradScale, so y = 1 is zero gradient.
+    """
+    if lobes is None:
+        lobes = [
+            (0.30, 0.95, 0.05, 0.05, 0.05, (0.9, 0.7, 0.5)),
+            (0.55, 0.85, 0.06, 0.10, 0.10, (0.3, 0.6, 1.0)),
+            (0.75, 0.70, 0.05, 0.15, 0.30, (1.0, 0.3, 0.2)),
+            (0.915, 0.90, 0.02, 0.12, 0.90, (1.0, 1.0, 0.8)),
+        ]
+    tf = TransferFunction2D(width, height)
+    x = (np.arange(width) + 0.5) / width
+    y = (np.arange(height) + 0.5) / height
+    X, Y = np.meshgrid(x, y)
+    rgb = np.zeros((height, width, 3))
+    a = np.zeros((height, width))
+    for (vc, gc, vs, gs, amax, col) in lobes:
+        w = amax * np.exp(-0.5 * (((X - vc) / vs) ** 2 + ((Y - gc) / gs) ** 2))
+        rgb += w[..., None] * np.asarray(col)
+        a += w
+    rgb = np.where(a[..., None] > 0, rgb / np.maximum(a[..., None], 1e-12), 0)
+    tf.color[..., :3] = rgb.astype(np.float32)
+    tf.color[..., 3] = np.clip(a, 0, 1).astype(np.float32)
+    return tf
