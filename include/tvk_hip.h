@@ -95,7 +95,8 @@ typedef struct {
 typedef struct {
   int lod_count;
   uint64_t brick_count;  /* all LODs                                            */
-  uint64_t pool_bytes;   /* HBM bytes of the brick pool                         */
+  uint64_t pool_bytes;   /* HBM bytes of the brick pool (blocked layout)        */
+  uint64_t brick_bytes;  /* bytes of all bricks in the reference layout         */
   double range_min, range_max;   /* UVFDataset::GetRange                        */
   int bit_width;
   uint64_t dims[3];
@@ -138,6 +139,9 @@ int tvk_brick_info(tvk_ctx* ctx, uint64_t toc_index, uint64_t coords[4], uint32_
 int tvk_brick_md(tvk_ctx* ctx, int lod, uint64_t index, float center[3], float extents[3],
                  uint32_t n_voxels[3]);
 int tvk_brick_minmax(tvk_ctx* ctx, uint64_t toc_index, double minmax[2]);
+/* Brick payloads in the reference layout (x fastest, stored size incl. overlap);
+ * tvk_get_pool returns every brick in TOC order (brick_bytes in total) and
+ * tvk_brick_info's pool_offset is the byte offset of a brick in it. */
 int tvk_get_brick(tvk_ctx* ctx, uint64_t toc_index, void* host_out, size_t bytes);
 int tvk_get_pool(tvk_ctx* ctx, void* host_out, size_t bytes);
 /* MaxGradientMagnitude over LOD0 core voxels (Histogram2DDataBlock.cpp:349-390) */

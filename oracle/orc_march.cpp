@@ -113,10 +113,14 @@ inline float pow8(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x
 inline float opacity_correct(float a, float s) { float t = 1.0f - a; return 1.0f - (s == 1.0f ? t : std::pow(t, s)); }
 inline float clamp01(float v) { return std::min(std::max(v, 0.0f), 1.0f); }
 
+// GLSL normalize(): one reciprocal of the length, then three products (the GLSL
+// built-in is implementation defined; this is the form the HIP kernel uses)
+inline V3 normalize_rcp(const V3& a) { const float r = 1.0f / length(a); return a * r; }
+
 // lighting.glsl:33-50 with vEyePos = 0
 V3 lighting(const V3& pos, const V3& n, const V3& amb, const V3& diff, const V3& spec, const V3& L) {
-  V3 view = normalize(V3(0, 0, 0) - pos);
-  V3 R = normalize(reflect(view, n));
+  V3 view = normalize_rcp(V3(0, 0, 0) - pos);
+  V3 R = normalize_rcp(reflect(view, n));
   float d = std::max(std::fabs(dot(n, L)), 0.0f);
   float s = pow8(std::max(dot(R, L), 0.0f));
   V3 c = amb + diff * d + spec * s;
@@ -126,8 +130,8 @@ V3 lighting(const V3& pos, const V3& n, const V3& amb, const V3& diff, const V3&
 // Compose-FS.glsl Lighting()
 V3 iso_lighting(const V3& pos, V3 n, const V3& amb, const V3& diff, const V3& spec, const V3& L) {
   n.z = std::fabs(n.z);
-  V3 view = normalize(V3(0, 0, 0) - pos);
-  V3 R = normalize(reflect(view, n));
+  V3 view = normalize_rcp(V3(0, 0, 0) - pos);
+  V3 R = normalize_rcp(reflect(view, n));
   float d = std::max(std::fabs(dot(n, L * -1.0f)), 0.0f);
   float s = pow8(std::max(dot(R, L), 0.0f));
   V3 c = amb + diff * d + spec * s;
@@ -221,7 +225,7 @@ struct Frame {
          nm[1][0] * s.x + nm[1][1] * s.y + nm[1][2] * s.z,
          nm[2][0] * s.x + nm[2][1] * s.y + nm[2][2] * s.z);
     float l = length(n);
-    if (l > 0.0f) n = n / l;
+    if (l > 0.0f) { const float r = 1.0f / l; n = n * r; }
     return n;
   }
 

@@ -20,6 +20,8 @@ hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, c
                              hipStream_t st);
 hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t* out,
                          hipStream_t st);
+hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bricks, const uint64_t* d_ref_off,
+                            uint32_t nb, uint32_t max_vox, void* out, hipStream_t st);
 hipError_t launch_max_gradient(const void* lin, int bpv, const uint64_t d[3], int clamp, double* out, uint32_t blocks,
                                hipStream_t st);
 hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bricks, const uint32_t bricks[3],
@@ -48,7 +50,8 @@ struct tvk_ctx {
   int tf1d_n = 0, tf2d_w = 0, tf2d_h = 0;
   uint8_t* d_tf1d = nullptr;
   uint8_t* d_tf2d = nullptr;
-  size_t tf1d_cap = 0, tf2d_cap = 0;
+  float4* d_tf2f = nullptr;
+  size_t tf1d_cap = 0, tf2d_cap = 0, tf2f_cap = 0;
   // view + parameters
   ViewState V;
   bool has_view = false;
@@ -163,7 +166,7 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
     uint32_t maxv = 0;
     for (uint32_t b = 0; b < nb; ++b) {
       const TocBrick& t = L.toc[li.offset + b];
-      maxv = std::max<uint32_t>(maxv, t.size[0] * t.size[1] * t.size[2]);
+      maxv = std::max<uint32_t>(maxv, (uint32_t)t.count);
     }
     hipError_t e = launch_gather(src, bpv, li.size, d_gb + li.offset, nb, maxv, c->d_pool, core, L.overlap, c->clamp,
                                  c->stream);
@@ -308,6 +311,7 @@ void tvk_destroy(tvk_ctx* c) {
   free_dataset(c);
   if (c->d_tf1d) (void)hipFree(c->d_tf1d);
   if (c->d_tf2d) (void)hipFree(c->d_tf2d);
+  if (c->d_tf2f) (void)hipFree(c->d_tf2f);
   if (c->d_list) (void)hipFree(c->d_list);
   if (c->d_views) (void)hipFree(c->d_views);
   if (c->d_out) (void)hipFree(c->d_out);
@@ -447,6 +451,7 @@ int tvk_dataset_info(tvk_ctx* c, tvk_dataset_desc* o) {
   o->lod_count = (int)(c->L.toc.back().coords[3] + 1);
   o->brick_count = c->L.toc.size();
   o->pool_bytes = c->pool_bytes;
+  o->brick_bytes = c->L.ref_elems * (uint64_t)c->L.bpv;
   o->range_min = c->range_min;
   o->range_max = c->range_max;
   o->bit_width = bit_width(c);
@@ -476,7 +481,7 @@ int tvk_brick_info(tvk_ctx* c, uint64_t i, uint64_t coords[4], uint32_t size[3],
   const TocBrick& t = c->L.toc[(size_t)i];
   for (int k = 0; k < 4; ++k) if (coords) coords[k] = t.coords[k];
   for (int k = 0; k < 3; ++k) if (size) size[k] = t.size[k];
-  if (off) *off = t.offset * (uint64_t)c->L.bpv;
+  if (off) *off = t.ref_offset * (uint64_t)c->L.bpv;   // byte offset in the tvk_get_pool layout
   return TVK_OK;
 }
 
@@ -502,24 +507,58 @@ int tvk_brick_minmax(tvk_ctx* c, uint64_t i, double mm[2]) {
   return TVK_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Copies bricks [first, first+n) of the TOC to the host in the reference layout.
+int download_ref(tvk_ctx* c, size_t first, size_t n, void* host, size_t bytes) {
+  const Layout& L = c->L;
+  std::vector<GatherBrick> gb(n);
+  std::vector<uint64_t> ro(n);
+  uint64_t total = 0;
+  uint32_t maxv = 0;
+  for (size_t k = 0; k < n; ++k) {
+    const TocBrick& t = L.toc[first + k];
+    for (int a = 0; a < 3; ++a) { gb[k].c[a] = (uint32_t)t.coords[a]; gb[k].n[a] = t.size[a]; }
+    gb[k].offset = t.offset;
+    ro[k] = total;
+    total += t.real;
+    maxv = std::max<uint32_t>(maxv, (uint32_t)t.real);
+  }
+  if (bytes != total * (size_t)L.bpv) return fail(c, TVK_ERR_ARG, "size mismatch");
+  GatherBrick* d_gb = nullptr;
+  uint64_t* d_ro = nullptr;
+  void* d_out = nullptr;
+  hipError_t e = hipMalloc(&d_gb, n * sizeof(GatherBrick));
+  if (e == hipSuccess) e = hipMalloc(&d_ro, n * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&d_out, bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_gb, gb.data(), n * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_ro, ro.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_unswizzle(c->d_pool, L.bpv, d_gb, d_ro, (uint32_t)n, maxv, d_out, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(host, d_out, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  if (d_gb) (void)hipFree(d_gb);
+  if (d_ro) (void)hipFree(d_ro);
+  if (d_out) (void)hipFree(d_out);
+  if (e != hipSuccess) return hipfail(c, "brick download", e);
+  return TVK_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int tvk_get_brick(tvk_ctx* c, uint64_t i, void* host, size_t bytes) {
   if (!c || !host) return TVK_ERR_ARG;
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
   if (i >= c->L.toc.size()) return fail(c, TVK_ERR_RANGE, "unknown brick");
-  const TocBrick& t = c->L.toc[(size_t)i];
-  if (bytes != t.count * (size_t)c->L.bpv) return fail(c, TVK_ERR_ARG, "size mismatch");
-  HIPCHK(c, hipMemcpyAsync(host, (const uint8_t*)c->d_pool + t.offset * c->L.bpv, bytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return TVK_OK;
+  return download_ref(c, (size_t)i, 1, host, bytes);
 }
 
 int tvk_get_pool(tvk_ctx* c, void* host, size_t bytes) {
   if (!c || !host) return TVK_ERR_ARG;
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
-  if (bytes != c->pool_bytes) return fail(c, TVK_ERR_ARG, "size mismatch");
-  HIPCHK(c, hipMemcpyAsync(host, c->d_pool, bytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return TVK_OK;
+  return download_ref(c, 0, c->L.toc.size(), host, bytes);
 }
 
 int tvk_max_gradient(tvk_ctx* c, float* out) {
@@ -559,8 +598,15 @@ int tvk_set_tf2d(tvk_ctx* c, const uint8_t* rgba8, int w, int h) {
   c->tf2d_w = w;
   c->tf2d_h = h;
   int rc = ensure(c, &c->d_tf2d, &c->tf2d_cap, 4 * (size_t)w * h);
+  if (!rc) rc = ensure(c, &c->d_tf2f, &c->tf2f_cap, (size_t)w * h);
   if (rc) return rc;
+  // texture2D of an RGBA8 table returns byte/255 (GL_NEAREST): expand once
+  std::vector<float4> f((size_t)w * h);
+  for (size_t i = 0; i < f.size(); ++i)
+    f[i] = make_float4((float)rgba8[4 * i] / 255.0f, (float)rgba8[4 * i + 1] / 255.0f,
+                       (float)rgba8[4 * i + 2] / 255.0f, (float)rgba8[4 * i + 3] / 255.0f);
   HIPCHK(c, hipMemcpyAsync(c->d_tf2d, c->tf2d.data(), 4 * (size_t)w * h, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tf2f, f.data(), f.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TVK_OK;
 }
@@ -734,6 +780,7 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   a.views = c->d_views;
   a.mode = c->P.mode;
   a.tf = c->P.mode == TVK_RM_2DTRANS ? c->d_tf2d : c->d_tf1d;
+  a.tf2f = c->d_tf2f;
   a.tf_n = c->tf1d_n;
   a.tf_w = c->tf2d_w;
   a.tf_h = c->tf2d_h;
@@ -761,6 +808,11 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
     a.spc[i] = c->P.light_specular[i];
     a.ldir[i] = c->P.light_dir[i];
     a.iso_col[i] = c->P.iso_color[i];
+  }
+  if (c->P.mode == TVK_RM_2DTRANS) {
+    for (int x = 0; x < c->tf2d_w; ++x)
+      for (int y = 0; y < c->tf2d_h; ++y)
+        if (c->tf2d[4 * ((size_t)y * c->tf2d_w + x) + 3]) { a.tf2_cols[x >> 6] |= 1ull << (x & 63); break; }
   }
   a.refine_steps = c->P.refine_steps;
   a.clamp_edge = c->P.clamp_to_edge;
@@ -855,7 +907,7 @@ int tvk_touched_bytes(tvk_ctx* c, uint64_t* bytes, uint64_t* bricks) {
   const size_t nt = c->L.toc.size();
   uint64_t b = 0, n = 0;
   for (size_t i = 0; i < t.size(); ++i)
-    if (t[i]) { ++n; b += c->L.toc[i % nt].count * (uint64_t)c->L.bpv; }
+    if (t[i]) { ++n; b += c->L.toc[i % nt].real * (uint64_t)c->L.bpv; }
   *bytes = b;
   *bricks = n;
   return TVK_OK;

@@ -82,7 +82,8 @@ bool Layout::build(const uint64_t d[3], const uint32_t b[3], uint32_t ov, int by
   size_t keep = lods.size();
   if (max_lods > 0 && (size_t)max_lods < keep) keep = (size_t)max_lods;
   toc.clear();
-  uint64_t off = 0;
+  uint64_t off = 0, roff = 0;
+  const int lx = block_lx(bytes);
   for (size_t j = 0; j < keep; ++j) {
     const LodInfo& l = lods[j];
     for (uint64_t z = 0; z < l.bricks[2]; ++z)
@@ -91,13 +92,17 @@ bool Layout::build(const uint64_t d[3], const uint32_t b[3], uint32_t ov, int by
           TocBrick t;
           t.coords[0] = x; t.coords[1] = y; t.coords[2] = z; t.coords[3] = j;
           brick_size(t.coords, t.size);
-          t.count = (uint64_t)t.size[0] * t.size[1] * t.size[2];
+          t.real = (uint64_t)t.size[0] * t.size[1] * t.size[2];
+          t.count = padded_count(t.size, lx);
           t.offset = off;
+          t.ref_offset = roff;
           off += t.count;
+          roff += t.real;
           toc.push_back(t);
         }
   }
   pool_elems = off;
+  ref_elems = roff;
   return true;
 }
 

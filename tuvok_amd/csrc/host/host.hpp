@@ -24,8 +24,10 @@ struct LodInfo {
 struct TocBrick {
   uint64_t coords[4];   // x, y, z, lod
   uint32_t size[3];     // stored voxels incl. overlap
-  uint64_t offset;      // element offset in the pool
-  uint64_t count;       // elements
+  uint64_t offset;      // element offset in the pool (blocked layout)
+  uint64_t count;       // elements in the pool (padded to whole blocks)
+  uint64_t real;        // voxels of the brick (reference layout)
+  uint64_t ref_offset;  // element offset in the reference layout (TOC order, unpadded)
 };
 
 // BrickKey = <timestep, LOD, 1D index>; hash of the reference (Brick.h:63-73) so
@@ -57,7 +59,8 @@ struct Layout {
   std::vector<LodInfo> lods;
   std::vector<TocBrick> toc;
   BrickTable table;
-  uint64_t pool_elems = 0;
+  uint64_t pool_elems = 0;   // blocked, padded
+  uint64_t ref_elems = 0;    // reference layout
 
   // ExtendedOctree::ComputeMetadata (ExtendedOctree.cpp:188-243); keeps the first
   // max_lods levels (0 = all, like the reference).

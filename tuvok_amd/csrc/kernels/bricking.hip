@@ -17,20 +17,34 @@
 
 namespace tvk {
 
-// grid: x = chunks of 256 voxels * ROWS, y = brick.  Each thread writes voxels
-// along x so stores and loads are coalesced.
+template <typename T> __device__ __forceinline__ int lx_of() { return sizeof(T) == 1 ? 3 : 2; }
+
+// decode a storage index of the blocked brick layout into voxel coordinates
+__device__ __forceinline__ void unswz(uint64_t i, uint32_t nbx, uint32_t nby, int lx, uint32_t& x, uint32_t& y,
+                                      uint32_t& z) {
+  const uint64_t blk = i >> (lx + 4);
+  const uint32_t r = (uint32_t)(i & ((1u << (lx + 4)) - 1u));
+  x = (uint32_t)(blk % nbx) << lx | (r & ((1u << lx) - 1u));
+  const uint64_t q = blk / nbx;
+  y = (uint32_t)(q % nby) << 2 | ((r >> lx) & 3u);
+  z = (uint32_t)(q / nby) << 2 | (r >> (lx + 2));
+}
+
+// grid: x = chunks of the brick's storage, y = brick.  Threads walk the brick in
+// storage order (coalesced stores); padding voxels repeat the last real voxel.
 template <typename T>
 __global__ __launch_bounds__(256) void gather_bricks(const T* __restrict__ lin, uint64_t X, uint64_t Y,
                                                      uint64_t Z, const GatherBrick* __restrict__ bricks,
                                                      T* __restrict__ pool, uint32_t core_x, uint32_t core_y,
                                                      uint32_t core_z, uint32_t ov, int clamp) {
   const GatherBrick b = bricks[blockIdx.y];
-  const uint64_t nvox = (uint64_t)b.n[0] * b.n[1] * b.n[2];
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nvox; i += (uint64_t)gridDim.x * 256) {
-    const uint32_t x = (uint32_t)(i % b.n[0]);
-    const uint64_t r = i / b.n[0];
-    const uint32_t y = (uint32_t)(r % b.n[1]);
-    const uint32_t z = (uint32_t)(r / b.n[1]);
+  const int lx = lx_of<T>();
+  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2);
+  const uint64_t cnt = padded_count(b.n, lx);
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * 256) {
+    uint32_t x, y, z;
+    unswz(i, nbx, nby, lx, x, y, z);
+    x = min(x, b.n[0] - 1); y = min(y, b.n[1] - 1); z = min(z, b.n[2] - 1);
     int64_t gx = (int64_t)b.c[0] * core_x - ov + x;
     int64_t gy = (int64_t)b.c[1] * core_y - ov + y;
     int64_t gz = (int64_t)b.c[2] * core_z - ov + z;
@@ -44,6 +58,23 @@ __global__ __launch_bounds__(256) void gather_bricks(const T* __restrict__ lin, 
       v = lin[(uint64_t)gx + X * ((uint64_t)gy + Y * (uint64_t)gz)];
     }
     pool[b.offset + i] = v;
+  }
+}
+
+// blocked pool -> reference layout (x fastest, unpadded) at ref offsets
+template <typename T>
+__global__ __launch_bounds__(256) void unswizzle_bricks(const T* __restrict__ pool, const GatherBrick* __restrict__ bricks,
+                                                        const uint64_t* __restrict__ ref_off, T* __restrict__ out) {
+  const GatherBrick b = bricks[blockIdx.y];
+  const int lx = lx_of<T>();
+  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2);
+  const uint64_t n = (uint64_t)b.n[0] * b.n[1] * b.n[2];
+  const uint64_t ro = ref_off[blockIdx.y];
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint32_t x = (uint32_t)(i % b.n[0]);
+    const uint64_t r = i / b.n[0];
+    const uint32_t y = (uint32_t)(r % b.n[1]), z = (uint32_t)(r / b.n[1]);
+    out[ro + i] = pool[b.offset + swz(x, y, z, nbx, nby, lx)];
   }
 }
 
@@ -82,7 +113,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void brick_minmax(const T* __restrict__ pool, const GatherBrick* __restrict__ bricks,
                                                     uint32_t* __restrict__ out) {
   const GatherBrick b = bricks[blockIdx.x];
-  const uint64_t nvox = (uint64_t)b.n[0] * b.n[1] * b.n[2];
+  const uint64_t nvox = padded_count(b.n, lx_of<T>());   // padding repeats real voxels
   uint32_t mn = 0xffffffffu, mx = 0;
   for (uint64_t i = threadIdx.x; i < nvox; i += 256) {
     const uint32_t v = pool[b.offset + i];
@@ -160,6 +191,8 @@ __global__ __launch_bounds__(256) void fill_overlap_wave(T* __restrict__ pool, c
   const uint32_t x = B.c[0], y = B.c[1], z = B.c[2];
   const int tx = (int)B.n[0], ty = (int)B.n[1], tz = (int)B.n[2];
   const int o = (int)ov;
+  const int lx = lx_of<T>();
+  const uint32_t tbx = nblocks(B.n[0], lx), tby = nblocks(B.n[1], 2);
   const uint64_t sx_ = 2ull * o * ty * tz, sy_ = 2ull * o * tx * tz, sz_ = 2ull * o * tx * ty;
   const uint64_t total = sx_ + sy_ + sz_;
   const bool hasR = x + 1 < bx, hasB = y + 1 < by, hasK = z + 1 < bz, hasL = x > 0, hasT = y > 0, hasF = z > 0;
@@ -210,9 +243,21 @@ __global__ __launch_bounds__(256) void fill_overlap_wave(T* __restrict__ pool, c
       const int pz = dz > 0 ? o + vz - (tz - o) : (dz < 0 ? nz - 2 * o + vz : vz);
       const bool earlier = dx < 0 || dy < 0 || dz < 0;   // already filled: final state
       const bool core = px >= o && px < nx - o && py >= o && py < ny - o && pz >= o && pz < nz - o;
-      if (earlier || core) val = pool[N.offset + (uint64_t)px + (uint64_t)nx * ((uint64_t)py + (uint64_t)ny * pz)];
+      if (earlier || core)
+        val = pool[N.offset + swz((uint32_t)px, (uint32_t)py, (uint32_t)pz, nblocks(N.n[0], lx), nblocks(N.n[1], 2), lx)];
     }
-    pool[B.offset + (uint64_t)vx + (uint64_t)tx * ((uint64_t)vy + (uint64_t)ty * vz)] = val;
+    const uint64_t dst = B.offset + swz((uint32_t)vx, (uint32_t)vy, (uint32_t)vz, tbx, tby, lx);
+    pool[dst] = val;
+    // padding voxels beyond the brick's last voxel repeat it
+    const bool ex = vx == tx - 1 && (tx & ((1 << lx) - 1)), ey = vy == ty - 1 && (ty & 3), ez = vz == tz - 1 && (tz & 3);
+    if (ex || ey || ez) {
+      const int xe = ex ? (int)(tbx << lx) : vx + 1, ye = ey ? (int)(tby << 2) : vy + 1;
+      const int ze = ez ? (int)(nblocks(B.n[2], 2) << 2) : vz + 1;
+      for (int zz = vz; zz < ze; ++zz)
+        for (int yy = vy; yy < ye; ++yy)
+          for (int xx = vx; xx < xe; ++xx)
+            pool[B.offset + swz((uint32_t)xx, (uint32_t)yy, (uint32_t)zz, tbx, tby, lx)] = val;
+    }
   }
 }
 
@@ -261,6 +306,22 @@ hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, c
   else
     hipLaunchKernelGGL(downsample<uint16_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint16_t*)src, sd[0],
                        sd[1], sd[2], (uint16_t*)dst, dd[0], dd[1], dd[2]);
+  return hipGetLastError();
+}
+
+hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bricks, const uint64_t* d_ref_off,
+                            uint32_t nb, uint32_t max_vox, void* out, hipStream_t st) {
+  uint32_t gx = (max_vox + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  for (uint32_t first = 0; first < nb; first += 65535) {
+    const uint32_t cnt = nb - first < 65535 ? nb - first : 65535;
+    if (bpv == 1)
+      hipLaunchKernelGGL(unswizzle_bricks<uint8_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint8_t*)pool,
+                         d_bricks + first, d_ref_off + first, (uint8_t*)out);
+    else
+      hipLaunchKernelGGL(unswizzle_bricks<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint16_t*)pool,
+                         d_bricks + first, d_ref_off + first, (uint16_t*)out);
+  }
   return hipGetLastError();
 }
 

@@ -38,7 +38,8 @@ __device__ __forceinline__ V3 operator*(V3 a, float s) { return mk(a.x * s, a.y 
 __device__ __forceinline__ V3 operator/(V3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ V3 normalize(V3 a) { return a / length(a); }
+// GLSL normalize(): one correctly rounded reciprocal of the length, then three products
+__device__ __forceinline__ V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 __device__ __forceinline__ float pow8(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x4; }
 __device__ __forceinline__ float fp16_round(float x) { return (float)(_Float16)x; }
@@ -47,6 +48,25 @@ __device__ __forceinline__ float fp16_round(float x) { return (float)(_Float16)x
 template <typename T> struct Unorm;
 template <> struct Unorm<uint8_t> { static __device__ __forceinline__ float inv() { return 1.0f / 255.0f; } };
 template <> struct Unorm<uint16_t> { static __device__ __forceinline__ float inv() { return 1.0f / 65535.0f; } };
+
+// Element offset parts of the blocked brick layout (see tvk_types.hpp): the
+// offset of voxel (x,y,z) is ox(x) + oy(y) + oz(z).
+template <typename T> struct Blk { static constexpr int LX = sizeof(T) == 1 ? 3 : 2; };
+template <typename T>
+__device__ __forceinline__ uint32_t ox(int x) {
+  constexpr int L = Blk<T>::LX;
+  return ((uint32_t)(x >> L) << (L + 4)) + (uint32_t)(x & ((1 << L) - 1));
+}
+template <typename T>
+__device__ __forceinline__ uint32_t oy(int y, uint32_t nbx) {
+  constexpr int L = Blk<T>::LX;
+  return (((uint32_t)(y >> 2) * nbx) << (L + 4)) + ((uint32_t)(y & 3) << L);
+}
+template <typename T>
+__device__ __forceinline__ uint32_t oz(int z, uint32_t nbxy) {
+  constexpr int L = Blk<T>::LX;
+  return (((uint32_t)(z >> 2) * nbxy) << (L + 4)) + ((uint32_t)(z & 3) << (L + 2));
+}
 
 // texture3D with GL_LINEAR and GL_CLAMP (border colour 0) or GL_CLAMP_TO_EDGE
 template <typename T>
@@ -68,18 +88,19 @@ __device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, i
   int xa = i0[0], xb = i0[0] + 1, ya = i0[1], yb = i0[1] + 1, za = i0[2], zb = i0[2] + 1;
   bool vxa = xa >= 0, vxb = xb < nx, vya = ya >= 0, vyb = yb < ny, vza = za >= 0, vzb = zb < nz;
   if (edge) { vxa = vxb = vya = vyb = vza = vzb = true; }
-  xa = max(xa, 0); xb = min(xb, nx - 1); ya = max(ya, 0); yb = min(yb, ny - 1); za = max(za, 0); zb = min(zb, nz - 1);
-  xa = min(xa, nx - 1); xb = max(xb, 0); ya = min(ya, ny - 1); yb = max(yb, 0); za = min(za, nz - 1); zb = max(zb, 0);
-  const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
-  const size_t ra = (size_t)ya * sy, rb = (size_t)yb * sy, pa = (size_t)za * sz, pb = (size_t)zb * sz;
-  float c000 = (vxa && vya && vza) ? (float)d[pa + ra + xa] * inv : 0.0f;
-  float c100 = (vxb && vya && vza) ? (float)d[pa + ra + xb] * inv : 0.0f;
-  float c010 = (vxa && vyb && vza) ? (float)d[pa + rb + xa] * inv : 0.0f;
-  float c110 = (vxb && vyb && vza) ? (float)d[pa + rb + xb] * inv : 0.0f;
-  float c001 = (vxa && vya && vzb) ? (float)d[pb + ra + xa] * inv : 0.0f;
-  float c101 = (vxb && vya && vzb) ? (float)d[pb + ra + xb] * inv : 0.0f;
-  float c011 = (vxa && vyb && vzb) ? (float)d[pb + rb + xa] * inv : 0.0f;
-  float c111 = (vxb && vyb && vzb) ? (float)d[pb + rb + xb] * inv : 0.0f;
+  xa = min(max(xa, 0), nx - 1); xb = min(max(xb, 0), nx - 1); ya = min(max(ya, 0), ny - 1);
+  yb = min(max(yb, 0), ny - 1); za = min(max(za, 0), nz - 1); zb = min(max(zb, 0), nz - 1);
+  const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
+  const uint32_t xA = ox<T>(xa), xB = ox<T>(xb), yA = oy<T>(ya, nbx), yB = oy<T>(yb, nbx);
+  const uint32_t zA = oz<T>(za, nbxy), zB = oz<T>(zb, nbxy);
+  float c000 = (vxa && vya && vza) ? (float)d[zA + yA + xA] * inv : 0.0f;
+  float c100 = (vxb && vya && vza) ? (float)d[zA + yA + xB] * inv : 0.0f;
+  float c010 = (vxa && vyb && vza) ? (float)d[zA + yB + xA] * inv : 0.0f;
+  float c110 = (vxb && vyb && vza) ? (float)d[zA + yB + xB] * inv : 0.0f;
+  float c001 = (vxa && vya && vzb) ? (float)d[zB + yA + xA] * inv : 0.0f;
+  float c101 = (vxb && vya && vzb) ? (float)d[zB + yA + xB] * inv : 0.0f;
+  float c011 = (vxa && vyb && vzb) ? (float)d[zB + yB + xA] * inv : 0.0f;
+  float c111 = (vxb && vyb && vzb) ? (float)d[zB + yB + xB] * inv : 0.0f;
   float c00 = c000 * (1.0f - f[0]) + c100 * f[0];
   float c10 = c010 * (1.0f - f[0]) + c110 * f[0];
   float c01 = c001 * (1.0f - f[0]) + c101 * f[0];
@@ -100,6 +121,123 @@ __device__ __forceinline__ V3 gradient(const T* __restrict__ d, int nx, int ny, 
   float zm = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, 0, -h.z));
   return mk(xm - xp, yp - ym, zm - zp) / 2.0f;
 }
+
+// Centre sample plus Volume3D.glsl gradient from one 32-voxel cross.
+// Every tap keeps its own clamped coordinate, index and weight exactly as
+// sample_brick computes them; the voxels are shared when each tap's index is
+// the centre's +-1 (always, except where a coordinate clamps at the texture
+// edge), otherwise the generic six-sample gradient runs.  value() first loads
+// the centre cube only, so samples whose TF entry is fully transparent never
+// pay for the gradient.
+__device__ __forceinline__ float tri(float c000, float c100, float c010, float c110, float c001, float c101, float c011,
+                                     float c111, float f0, float f1, float f2) {
+  // sample_brick's order: x lerps, then y, then z
+  const float c00 = c000 * (1.0f - f0) + c100 * f0, c10 = c010 * (1.0f - f0) + c110 * f0;
+  const float c01 = c001 * (1.0f - f0) + c101 * f0, c11 = c011 * (1.0f - f0) + c111 * f0;
+  const float c0 = c00 * (1.0f - f1) + c10 * f1, c1 = c01 * (1.0f - f1) + c11 * f1;
+  return c0 * (1.0f - f2) + c1 * f2;
+}
+
+template <typename T>
+struct CrossSampler {
+  const T* __restrict__ d;
+  int n[3];
+  bool edge;
+  V3 p;
+  int ic[3];
+  float fc[3];
+  float c[2][2][2];
+  // element offset parts and border validity of indices ic-1 .. ic+2 per axis
+  uint32_t ofx[4], ofy[4], ofz[4];
+  bool okx[4], oky[4], okz[4];
+
+  __device__ __forceinline__ float V(int i, int j, int k) const {   // i,j,k in 0..3 -> index ic-1+i ...
+    return (okx[i] && oky[j] && okz[k]) ? (float)d[ofz[k] + ofy[j] + ofx[i]] * Unorm<T>::inv() : 0.0f;
+  }
+
+  __device__ __forceinline__ float value(const T* __restrict__ dd, int nx, int ny, int nz, bool e, V3 pos) {
+    d = dd; n[0] = nx; n[1] = ny; n[2] = nz; edge = e; p = pos;
+    const float pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float s = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = s * (float)n[a] - 0.5f, fl = floorf(u);
+      ic[a] = (int)fl;
+      fc[a] = u - fl;
+    }
+    const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int x = ic[0] - 1 + k, y = ic[1] - 1 + k, z = ic[2] - 1 + k;
+      okx[k] = edge || (x >= 0 && x < nx);
+      oky[k] = edge || (y >= 0 && y < ny);
+      okz[k] = edge || (z >= 0 && z < nz);
+      ofx[k] = ox<T>(min(max(x, 0), nx - 1));
+      ofy[k] = oy<T>(min(max(y, 0), ny - 1), nbx);
+      ofz[k] = oz<T>(min(max(z, 0), nz - 1), nbxy);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) c[i][j][k] = V(1 + i, 1 + j, 1 + k);
+    return tri(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], fc[0],
+               fc[1], fc[2]);
+  }
+
+  __device__ __forceinline__ V3 gradient(V3 h) const {
+    // per axis: minus tap (p - h) and plus tap (p + h); Volume3D.glsl's y "p" tap is at -h
+    const float pm[3] = {p.x + -h.x, p.y + -h.y, p.z + -h.z};
+    const float pp[3] = {p.x + h.x, p.y + h.y, p.z + h.z};
+    int im[3], iq[3];
+    float fm[3], fq[3];
+    bool shared = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float s = fminf(fmaxf(pm[a], 0.0f), 1.0f), u = s * (float)n[a] - 0.5f, fl = floorf(u);
+      im[a] = (int)fl; fm[a] = u - fl;
+      s = fminf(fmaxf(pp[a], 0.0f), 1.0f); u = s * (float)n[a] - 0.5f; fl = floorf(u);
+      iq[a] = (int)fl; fq[a] = u - fl;
+      shared = shared && im[a] == ic[a] - 1 && iq[a] == ic[a] + 1;
+    }
+    if (!shared) {
+      // rare (texture-edge clamping): the six taps one at a time, as Volume3D.glsl
+      float t[6];
+#pragma unroll 1
+      for (int k = 0; k < 6; ++k) {
+        const int a = k >> 1;
+        const float off = (k & 1) ? (a == 1 ? h.y : (a == 0 ? -h.x : -h.z)) : (a == 1 ? -h.y : (a == 0 ? h.x : h.z));
+        V3 q = p;
+        if (a == 0) q = p + mk(off, 0, 0); else if (a == 1) q = p + mk(0, off, 0); else q = p + mk(0, 0, off);
+        t[k] = sample_brick<T>(d, n[0], n[1], n[2], edge, q);
+      }
+      // t = {xp, xm, yp, ym, zp, zm} in Volume3D.glsl's naming
+      return mk(t[1] - t[0], t[2] - t[3], t[5] - t[4]) / 2.0f;
+    }
+    float xl[2][2], xr[2][2], yl[2][2], yr[2][2], zl[2][2], zr[2][2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        xl[j][k] = V(0, 1 + j, 1 + k); xr[j][k] = V(3, 1 + j, 1 + k);
+        yl[j][k] = V(1 + j, 0, 1 + k); yr[j][k] = V(1 + j, 3, 1 + k);
+        zl[j][k] = V(1 + j, 1 + k, 0); zr[j][k] = V(1 + j, 1 + k, 3);
+      }
+    const float fxm = tri(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
+                          fm[0], fc[1], fc[2]);
+    const float fxp = tri(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
+                          fq[0], fc[1], fc[2]);
+    const float fym = tri(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
+                          fc[0], fm[1], fc[2]);
+    const float fyp = tri(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
+                          fc[0], fq[1], fc[2]);
+    const float fzm = tri(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
+                          fc[0], fc[1], fm[2]);
+    const float fzp = tri(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
+                          fc[0], fc[1], fq[2]);
+    return mk(fxm - fxp, fym - fyp, fzm - fzp) / 2.0f;
+  }
+};
 
 __device__ __forceinline__ V3 reflect(V3 I, V3 N) { return I - N * (2.0f * dot(N, I)); }
 
@@ -147,11 +285,11 @@ __device__ __forceinline__ V3 normal_eye(V3 g, const float* nm, V3 ds) {
   V3 n = mk(nm[0] * s.x + nm[1] * s.y + nm[2] * s.z, nm[3] * s.x + nm[4] * s.y + nm[5] * s.z,
             nm[6] * s.x + nm[7] * s.y + nm[8] * s.z);
   float l = length(n);
-  if (l > 0.0f) n = n / l;
+  if (l > 0.0f) { const float r = 1.0f / l; n = n * r; }
   return n;
 }
 
-template <int MODE, bool LIT, typename T>
+template <int MODE, bool LIT, typename T, bool COUNT>
 __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
   extern __shared__ float4 s_tf[];  // 1D TF as floats (tf_n entries, MODE 0 only)
 
@@ -244,7 +382,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
       }
       ok = ok && tf > tn && tf > 0.0f;
       const float te = fmaxf(tn, 1.0f);  // near-plane entry when the front face is clipped
-      if (A.touched) {
+      if (COUNT) {
         // every ray that reaches a brick takes at least one sample there
         if (__ballot(ok && tf > te) != 0 && lane == __builtin_amdgcn_readfirstlane(lane))
           A.touched[(size_t)v * A.ntoc + fb->toc] = 1u;
@@ -303,14 +441,15 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
       for (int i = 0; i < steps; ++i) {
         ++ns;
         float s0, s1, s2, s3;
-        const float val = sample_brick<T>(bd, nx, ny, nz, edge, pos);
+        CrossSampler<T> cs;
+        const float val = cs.value(bd, nx, ny, nz, edge, pos);
         if (MODE == 0) {
           const float cc = fminf(fmaxf(val * A.tf_scale, 0.0f), 1.0f);
           const int ti = min((int)floorf(cc * (float)A.tf_n), A.tf_n - 1);
           const float4 tv = s_tf[ti];
           s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
-          if (LIT) {
-            const V3 gg = gradient<T>(bd, nx, ny, nz, edge, pos, vstep);
+          if (LIT && s3 != 0.0f) {   // a transparent sample composites to nothing: skip its shading
+            const V3 gg = cs.gradient(vstep);
             const V3 nn = normal_eye(gg, VC->nm, dsc);
             const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
             s0 = lc.x; s1 = lc.y; s2 = lc.z;
@@ -318,26 +457,28 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
           const float tt = 1.0f - s3;
           s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
         } else {
-          const V3 gg = gradient<T>(bd, nx, ny, nz, edge, pos, vstep);
-          const float gm = length(gg);
           const float sx = fminf(fmaxf(val * A.tf_scale, 0.0f), 1.0f);
-          const float sy = fminf(fmaxf(1.0f - gm * A.grad_scale, 0.0f), 1.0f);
           const int xi = min((int)floorf(sx * (float)A.tf_w), A.tf_w - 1);
-          const int yi = min((int)floorf(sy * (float)A.tf_h), A.tf_h - 1);
-          const uint32_t w = *reinterpret_cast<const uint32_t*>(A.tf + 4 * ((size_t)yi * (size_t)A.tf_w + (size_t)xi));
-          s0 = (float)(w & 0xffu) / 255.0f;
-          s1 = (float)((w >> 8) & 0xffu) / 255.0f;
-          s2 = (float)((w >> 16) & 0xffu) / 255.0f;
-          s3 = (float)(w >> 24) / 255.0f;
-          if (LIT) {
-            const V3 nn = normal_eye(gg, VC->nm, dsc);
-            const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
-            const float tt = 1.0f - s3;
-            s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
-            s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
+          if (((A.tf2_cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
+            // every row of this value column is transparent: alpha 0 whatever the gradient
+            s0 = s1 = s2 = s3 = 0.0f;
           } else {
-            const float tt = 1.0f - s3;
-            s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+            const V3 gg = cs.gradient(vstep);
+            const float gm = length(gg);
+            const float sy = fminf(fmaxf(1.0f - gm * A.grad_scale, 0.0f), 1.0f);
+            const int yi = min((int)floorf(sy * (float)A.tf_h), A.tf_h - 1);
+            const float4 tv = A.tf2f[(size_t)yi * (size_t)A.tf_w + (size_t)xi];   // byte/255, precomputed
+            s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
+            if (LIT) {
+              const V3 nn = normal_eye(gg, VC->nm, dsc);
+              const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
+              const float tt = 1.0f - s3;
+              s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+              s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
+            } else {
+              const float tt = 1.0f - s3;
+              s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+            }
           }
         }
         // UnderCompositing (Compositing.glsl:33-38)
@@ -361,7 +502,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
     if (__ballot(!done) == 0) break;  // every ray of the tile has terminated
   }
 
-  if (A.counter) {
+  if (COUNT) {
     uint32_t s = ns;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -384,27 +525,33 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
 }
 
 // host-side launcher (extern "C++" within the library)
+template <int M, bool L, typename T>
+static void launch_one(const MarchArgs& a, dim3 grid, size_t lds, hipStream_t st) {
+  if (a.counter)
+    hipLaunchKernelGGL((march_kernel<M, L, T, true>), grid, dim3(256), lds, st, a);
+  else
+    hipLaunchKernelGGL((march_kernel<M, L, T, false>), grid, dim3(256), lds, st, a);
+}
+
 hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st) {
-  dim3 grid(a.groups_padded), block(256);
+  dim3 grid(a.groups_padded);
   const size_t lds = a.mode == 0 ? (size_t)a.tf_n * sizeof(float4) : 0;
-  const int key = a.mode * 4 + (a.lighting ? 2 : 0) + (bpv == 2 ? 1 : 0);
-  switch (key) {
-#define TVK_CASE(M, L, T, K) \
-    case K: hipLaunchKernelGGL((march_kernel<M, L, T>), grid, block, lds, st, a); break;
-    TVK_CASE(0, false, uint8_t, 0)
-    TVK_CASE(0, false, uint16_t, 1)
-    TVK_CASE(0, true, uint8_t, 2)
-    TVK_CASE(0, true, uint16_t, 3)
-    TVK_CASE(1, false, uint8_t, 4)
-    TVK_CASE(1, false, uint16_t, 5)
-    TVK_CASE(1, true, uint8_t, 6)
-    TVK_CASE(1, true, uint16_t, 7)
-    TVK_CASE(2, false, uint8_t, 8)
-    TVK_CASE(2, false, uint16_t, 9)
-    TVK_CASE(2, false, uint8_t, 10)
-    TVK_CASE(2, false, uint16_t, 11)
-#undef TVK_CASE
-    default: return hipErrorInvalidValue;
+  const bool lit = a.lighting != 0 && a.mode != 2;
+  const bool u16 = bpv == 2;
+  switch (a.mode) {
+    case 0:
+      if (lit) { if (u16) launch_one<0, true, uint16_t>(a, grid, lds, st); else launch_one<0, true, uint8_t>(a, grid, lds, st); }
+      else { if (u16) launch_one<0, false, uint16_t>(a, grid, lds, st); else launch_one<0, false, uint8_t>(a, grid, lds, st); }
+      break;
+    case 1:
+      if (lit) { if (u16) launch_one<1, true, uint16_t>(a, grid, lds, st); else launch_one<1, true, uint8_t>(a, grid, lds, st); }
+      else { if (u16) launch_one<1, false, uint16_t>(a, grid, lds, st); else launch_one<1, false, uint8_t>(a, grid, lds, st); }
+      break;
+    case 2:
+      if (u16) launch_one<2, false, uint16_t>(a, grid, lds, st); else launch_one<2, false, uint8_t>(a, grid, lds, st);
+      break;
+    default:
+      return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

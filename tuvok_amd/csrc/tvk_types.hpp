@@ -1,8 +1,27 @@
 // Structures shared by the host planner and the HIP kernels.
 #pragma once
 #include <stdint.h>
+#include <hip/hip_runtime.h>
 
 namespace tvk {
+
+// In HBM a brick is stored as 128-byte blocks of 8x4x4 (uint8) or 4x4x4
+// (uint16) voxels, blocks in x-fastest order, each axis padded to whole
+// blocks with copies of the last voxel.  A ray bundle marching through a brick
+// then touches compact cache lines instead of one line per (y,z) row.  The
+// reference layout (x-fastest, unpadded) is what the download calls return.
+#ifndef TVK_HD
+#define TVK_HD __host__ __device__
+#endif
+TVK_HD inline int block_lx(int bpv) { return bpv == 1 ? 3 : 2; }   // log2 of the block width in x
+TVK_HD inline uint32_t nblocks(uint32_t n, int l) { return (n + (1u << l) - 1u) >> l; }
+TVK_HD inline uint64_t swz(uint32_t x, uint32_t y, uint32_t z, uint32_t nbx, uint32_t nby, int lx) {
+  return ((((uint64_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> lx)) << (lx + 4)) + ((uint64_t)(z & 3u) << (lx + 2)) +
+         ((uint64_t)(y & 3u) << lx) + (x & ((1u << lx) - 1u));
+}
+TVK_HD inline uint64_t padded_count(const uint32_t n[3], int lx) {
+  return (uint64_t)nblocks(n[0], lx) * nblocks(n[1], 2) * nblocks(n[2], 2) << (lx + 4);
+}
 
 // One entry of a frame's visible, non-empty brick list, in compositing order.
 // Everything the march needs for the brick is here, so a wavefront reads it
@@ -51,6 +70,7 @@ struct MarchArgs {
   const FrameBrick* list;
   const ViewConst* views;
   const uint8_t* tf;            // RGBA8 table (1D: tf_n entries; 2D: tf_w x tf_h)
+  const float4* tf2f;           // 2D table as float4 (byte / 255)
   float* out;                   // RGBA32F
   unsigned long long* counter;  // composited-sample counter or nullptr
   uint32_t* touched;            // [view * ntoc + toc] = 1 once a sample is taken in the brick (or nullptr)
@@ -60,6 +80,7 @@ struct MarchArgs {
   float tf_scale, grad_scale, iso_n, ert_eps;
   float dscale[3], amb[3], dif[3], spc[3], ldir[3], iso_col[3];
   int refine_steps, clamp_edge, lighting, mode;
+  uint64_t tf2_cols[64];        // bit x: value column x of the 2D TF has a non-zero alpha (w <= 4096)
   uint32_t groups_total;        // sum of groups over the views
   uint32_t groups_padded;       // rounded up to a multiple of 8 (XCD count)
 };

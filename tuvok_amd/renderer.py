@@ -147,7 +147,7 @@ class HIPRaycaster:
 
     def GetPool(self):
         d = self.GetDatasetDesc()
-        out = np.empty(d.pool_bytes, np.uint8)
+        out = np.empty(d.brick_bytes, np.uint8)
         self._chk(self._L.tvk_get_pool(self._h, out.ctypes.data, out.nbytes))
         return out
 
