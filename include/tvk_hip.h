@@ -130,6 +130,11 @@ int tvk_generate_volume(tvk_ctx* ctx, int kind, uint64_t seed, int dtype, const 
                         const uint32_t brick[3], uint32_t overlap, int clamp_to_edge,
                         int max_lods, int keep_linear);
 /* Copy the retained linear LOD0 volume back to the host. */
+/* Makes dst render src's dataset: the brick pool (and kept linear volume) in HBM
+ * is shared, not copied, and stays allocated while any context holds it; dst
+ * keeps its own TF, parameters and camera.  GPUMemMan shares one dataset's
+ * textures between renderers the same way (GPUMemMan.cpp:846-996). */
+int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src);
 int tvk_download_volume(tvk_ctx* ctx, void* host_out, size_t bytes);
 int tvk_dataset_info(tvk_ctx* ctx, tvk_dataset_desc* out);
 int tvk_lod_info(tvk_ctx* ctx, int lod, uint64_t size[3], uint64_t bricks[3], uint64_t* offset,
@@ -172,22 +177,106 @@ int tvk_plan(tvk_ctx* ctx, tvk_brick_rec* out, int cap, int* n);
 /* Render one frame.  host_out (may be NULL) receives W*H RGBA32F, premultiplied,
  * row 0 = bottom (GL window order). */
 int tvk_render(tvk_ctx* ctx, float* host_out, tvk_stats* stats);
-/* Render n views (rotation matrices rots[n*16]) back to back in one launch per view
- * on the context stream; host_out (may be NULL) receives n frames. */
+/* Render n views (rotation matrices rots[n*16]) of the current dataset, TF and
+ * camera in one kernel launch on the context stream; host_out (may be NULL)
+ * receives n frames. */
 int tvk_render_views(tvk_ctx* ctx, int n, const float* rots, float* host_out, tvk_stats* stats);
 /* Enable/disable the per-frame sample counter and touched-brick record (off by
  * default: costs an atomic per wave). */
 int tvk_set_count_samples(tvk_ctx* ctx, int on);
 /* Stored bytes (and count) of the bricks in which the last counted render took at
- * least one sample, summed over its views. */
-int tvk_touched_bytes(tvk_ctx* ctx, uint64_t* bytes, uint64_t* bricks);
+ * least one sample: summed over its views, and over the union of its views (each
+ * brick once). */
+int tvk_touched_bytes(tvk_ctx* ctx, uint64_t* bytes, uint64_t* bricks, uint64_t* unique_bytes,
+                      uint64_t* unique_bricks);
 /* Device pointer of the last rendered frame(s) (for zero-copy consumers). */
 int tvk_frame_device_ptr(tvk_ctx* ctx, void** dptr);
 
 /* ---- timing on the context stream ------------------------------------------- */
-int tvk_event_record(tvk_ctx* ctx, int slot);          /* slot in [0, 8) */
+int tvk_event_record(tvk_ctx* ctx, int slot);          /* slot in [0, 6); 6 and 7 bracket the last launch */
 int tvk_event_elapsed(tvk_ctx* ctx, int a, int b, float* ms);
 int tvk_synchronize(tvk_ctx* ctx);
+
+/* ---- batches ---------------------------------------------------------------
+ * Many render requests -- each one the dataset, TF, parameters and camera of a
+ * tvk_ctx with its own view rotation and image size -- rendered by persistent
+ * kernels that pull ray tiles from device work queues (one launch per kernel
+ * variant: render mode x lighting x voxel type), replayed as one hipGraph per
+ * frame.  No reference counterpart as one call: AbstrRenderer::Paint renders
+ * its RenderRegions one after the other (GLRenderer.cpp:571-667); each request's
+ * frame equals tvk_render of its context at that rotation and size.  A request
+ * keeps a pointer to its context: the context must outlive the batch's use of
+ * it, and is read (not modified) by tvk_batch_render. */
+typedef struct tvk_batch tvk_batch;
+typedef struct {
+  int tile_w, tile_h;    /* pixels per wave tile; tile_w * tile_h == 64          */
+  int groups_per_cu;     /* persistent workgroups per CU (0: occupancy limit)    */
+  int use_graph;         /* replay the frame as a captured hipGraph              */
+} tvk_tuning;
+int tvk_default_tuning(tvk_tuning* t);   /* 8x8 tiles, occupancy-sized grid, graph on */
+int tvk_batch_create(tvk_batch** out, int device);
+void tvk_batch_destroy(tvk_batch* b);
+const char* tvk_batch_last_error(const tvk_batch* b);
+int tvk_batch_clear(tvk_batch* b);
+/* rot: 4x4 rotation (NULL: the context's); w, h: image size (the context's
+ * projection is re-made for this aspect when it came from tvk_set_camera) */
+int tvk_batch_add(tvk_batch* b, tvk_ctx* ctx, const float rot[16], uint32_t w, uint32_t h, int* id);
+int tvk_batch_set_tuning(tvk_batch* b, const tvk_tuning* t);
+int tvk_batch_get_tuning(tvk_batch* b, tvk_tuning* t);
+int tvk_batch_set_count_samples(tvk_batch* b, int on);
+/* Plans every request, uploads the brick lists and request table, replays the
+ * frame graph; stats->ms is the device time of the launches. */
+int tvk_batch_render(tvk_batch* b, tvk_stats* stats);
+/* RGBA32F premultiplied frame of request id from the last render */
+int tvk_batch_frame(tvk_batch* b, int id, float* host_out);
+int tvk_batch_frame_device_ptr(tvk_batch* b, int id, void** dptr);
+/* Times every tile shape x grid size on the current requests (reps renders
+ * each after a warm-up), keeps and returns the fastest. */
+int tvk_batch_autotune(tvk_batch* b, int reps, tvk_tuning* best, float* best_ms);
+
+/* ---- step-proposal network ----------------------------------------------------
+ * No reference counterpart (the reference marches at a fixed step).  A small
+ * MLP, 24 features -> 64 -> 64 -> 2 with ReLU, BF16 weights and FP32
+ * accumulation on the matrix cores, proposes per ray and brick segment a step
+ * multiplier (output 0: log2(multiplier) / 3) and a skip flag (output 1: logit).
+ * Parameters are an FP32 vector of tvk_stepnet_param_count() floats:
+ * W1[24][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2] (row-major, in x out). */
+typedef struct tvk_stepnet tvk_stepnet;
+int tvk_stepnet_create(tvk_stepnet** out, int device, uint64_t seed);   /* He-uniform init */
+void tvk_stepnet_destroy(tvk_stepnet* s);
+const char* tvk_stepnet_last_error(const tvk_stepnet* s);
+int tvk_stepnet_param_count(void);
+int tvk_stepnet_set_params(tvk_stepnet* s, const float* theta, int n);  /* also resets Adam */
+int tvk_stepnet_get_params(tvk_stepnet* s, float* theta, int n);
+/* outputs (n x 2) of n feature rows (n x 24) */
+int tvk_stepnet_infer(tvk_stepnet* s, const float* feat, int n, float* out);
+/* training samples: features n x 24, targets n x 3 (t0 = log2(multiplier)/3, t1 = skip in {0,1}, weight) */
+int tvk_stepnet_set_data(tvk_stepnet* s, const float* feat, const float* tgt, int n);
+/* gradient of mean weighted loss over the first n samples; grad[count] = summed weighted loss */
+int tvk_stepnet_grad(tvk_stepnet* s, int n, float* grad);
+/* Adam over the loaded samples; losses[epochs] (may be NULL) gets each epoch's mean loss */
+int tvk_stepnet_train(tvk_stepnet* s, int n, int epochs, int batch, float lr, float weight_decay, uint64_t seed,
+                      float* losses);
+
+/* Learned step on a context: with a network attached, tvk_render and
+ * tvk_render_views (1D/2D TF modes) march every ray segment through a brick
+ * with the proposed step multiplier m (step m * fRayStepsize, opacity
+ * correction exponent m * fStepScale) or skip it; NULL detaches (fixed step). */
+typedef struct {
+  float margin;          /* multiplier = 2^clamp(floor(3*y0 + 0.5 - margin), 0, max_log2) */
+  float skip_threshold;  /* skip when sigmoid(y1) > threshold (>= 1: never skip)          */
+  int max_log2;          /* 0..3                                                          */
+  float fallback_bound;  /* fixed step where T_before * (1-(1-alpha_max)^n) exceeds this (<= 0: never) */
+} tvk_proposal_params;
+int tvk_set_step_proposal(tvk_ctx* ctx, tvk_stepnet* net, const tvk_proposal_params* p);
+/* Training samples from the fixed-step march: renders nviews views (rots[n*16])
+ * and records a hash-selected fraction `keep` of the ray segments (at most cap),
+ * each as 48 floats: 24 features, transmittance before the segment, fixed-step
+ * sample count, the segment's premultiplied RGBA at multipliers 1, 2, 4, 8, 6 pad. */
+int tvk_generate_segments(tvk_ctx* ctx, int nviews, const float* rots, uint32_t cap, float keep, uint32_t seed,
+                          float* out, uint32_t* n);
+/* The same proposal for every 1D/2D-TF request of a batch (NULL: fixed step). */
+int tvk_batch_set_step_proposal(tvk_batch* b, tvk_stepnet* net, const tvk_proposal_params* p);
 
 #ifdef __cplusplus
 }

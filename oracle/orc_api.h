@@ -124,6 +124,13 @@ float orc_round_fp16(float f);
 void orc_brick_shader_vars(const orc_ds*, const orc_params*, const orc_brick*,
                            float* ray_step, float* step_scale, float voxel_step[3]);
 
+/* ---- step-proposal network (no reference counterpart) -------------------- */
+/* FP64 forward/backward of the 24-64-64-2 MLP; bf16 = 1 rounds to bfloat16 at
+ * the points the HIP kernels do.  theta: orc_sn_param_count() floats. */
+int orc_sn_param_count(void);
+void orc_sn_forward(const float* theta, const float* x, int n, int bf16, double* y);
+void orc_sn_grad(const float* theta, const float* x, const float* tgt, int n, int bf16, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -352,3 +352,41 @@ def camera(width, height, eye=(0, 0, 2), fov=45.0, znear=0.01, zfar=1000.0, rot=
     return dict(view=look_at(eye), proj=perspective(fov, width / height, znear, zfar),
                 rot=np.eye(4, dtype=np.float32) if rot is None else np.asarray(rot, np.float32),
                 trans=np.eye(4, dtype=np.float32), width=width, height=height, fov=fov, znear=znear, zfar=zfar)
+
+
+# ---- step-proposal network ---------------------------------------------------------
+def _sn_lib():
+    L = lib()
+    if not getattr(L, "_sn_ready", False):
+        L.orc_sn_param_count.restype = C.c_int
+        L.orc_sn_forward.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.c_int,
+                                     C.POINTER(C.c_double)]
+        L.orc_sn_grad.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                                  C.c_int, C.POINTER(C.c_double)]
+        L._sn_ready = True
+    return L
+
+
+def sn_param_count():
+    return _sn_lib().orc_sn_param_count()
+
+
+def sn_forward(theta, x, bf16=True):
+    theta = np.ascontiguousarray(theta, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros((x.shape[0], 2), np.float64)
+    _sn_lib().orc_sn_forward(theta.ctypes.data_as(C.POINTER(C.c_float)), x.ctypes.data_as(C.POINTER(C.c_float)),
+                             x.shape[0], int(bf16), y.ctypes.data_as(C.POINTER(C.c_double)))
+    return y
+
+
+def sn_grad(theta, x, tgt, bf16=True):
+    """(gradient of the mean weighted loss, summed weighted loss)"""
+    theta = np.ascontiguousarray(theta, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    tgt = np.ascontiguousarray(tgt, np.float32)
+    g = np.zeros(theta.size + 1, np.float64)
+    _sn_lib().orc_sn_grad(theta.ctypes.data_as(C.POINTER(C.c_float)), x.ctypes.data_as(C.POINTER(C.c_float)),
+                          tgt.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0], int(bf16),
+                          g.ctypes.data_as(C.POINTER(C.c_double)))
+    return g[:-1], g[-1]

@@ -51,3 +51,14 @@ def test_cpp_front_end_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_batch_abi_defaults_and_null_arguments():
+    L = _abi.load()
+    t = _abi.Tuning()
+    assert L.tvk_default_tuning(ctypes.byref(t)) == 0
+    assert (t.tile_w, t.tile_h, t.groups_per_cu, t.use_graph) == (8, 8, 0, 1)
+    assert L.tvk_batch_create(None, 0) == -1
+    assert L.tvk_batch_render(None, None) == -1
+    assert L.tvk_batch_add(None, None, None, 8, 8, None) == -1
+    assert L.tvk_share_dataset(None, None) == -1

@@ -13,12 +13,19 @@ GEN_BLOBS, GEN_FBM_SHELLS, GEN_ELLIPSOID_SDF, GEN_PHANTOM = 1, 2, 3, 4
 # every symbol include/tvk_hip.h declares
 EXPORTS = [
     "tvk_create", "tvk_destroy", "tvk_last_error", "tvk_default_params", "tvk_load_volume",
-    "tvk_generate_volume", "tvk_download_volume", "tvk_dataset_info", "tvk_lod_info", "tvk_brick_info",
+    "tvk_generate_volume", "tvk_share_dataset", "tvk_download_volume", "tvk_dataset_info", "tvk_lod_info", "tvk_brick_info",
     "tvk_brick_md", "tvk_brick_minmax", "tvk_get_brick", "tvk_get_pool", "tvk_max_gradient",
     "tvk_set_tf1d", "tvk_set_tf2d", "tvk_set_view_matrices", "tvk_set_camera", "tvk_set_rotation",
     "tvk_get_view_matrices", "tvk_set_params", "tvk_get_params", "tvk_current_lod", "tvk_plan",
     "tvk_render", "tvk_render_views", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_frame_device_ptr",
     "tvk_event_record", "tvk_event_elapsed", "tvk_synchronize",
+    "tvk_default_tuning", "tvk_batch_create", "tvk_batch_destroy", "tvk_batch_last_error", "tvk_batch_clear",
+    "tvk_batch_add", "tvk_batch_set_tuning", "tvk_batch_get_tuning", "tvk_batch_set_count_samples",
+    "tvk_batch_render", "tvk_batch_frame", "tvk_batch_frame_device_ptr", "tvk_batch_autotune",
+    "tvk_stepnet_create", "tvk_stepnet_destroy", "tvk_stepnet_last_error", "tvk_stepnet_param_count",
+    "tvk_stepnet_set_params", "tvk_stepnet_get_params", "tvk_stepnet_infer", "tvk_stepnet_set_data",
+    "tvk_stepnet_grad", "tvk_stepnet_train", "tvk_set_step_proposal", "tvk_generate_segments",
+    "tvk_batch_set_step_proposal",
 ]
 
 
@@ -42,6 +49,15 @@ class DatasetDesc(C.Structure):
                 ("brick_bytes", C.c_uint64),
                 ("range_min", C.c_double), ("range_max", C.c_double), ("bit_width", C.c_int),
                 ("dims", C.c_uint64 * 3), ("brick_size", C.c_uint32 * 3), ("overlap", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_w", C.c_int), ("tile_h", C.c_int), ("groups_per_cu", C.c_int), ("use_graph", C.c_int)]
+
+
+class ProposalParams(C.Structure):
+    _fields_ = [("margin", C.c_float), ("skip_threshold", C.c_float), ("max_log2", C.c_int),
+                ("fallback_bound", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -71,6 +87,7 @@ def load(path=LIB_PATH):
         "tvk_generate_volume": (C.c_int, [vp, C.c_int, C.c_uint64, C.c_int, u64p, u32p, C.c_uint32, C.c_int,
                                           C.c_int, C.c_int]),
         "tvk_download_volume": (C.c_int, [vp, vp, C.c_size_t]),
+        "tvk_share_dataset": (C.c_int, [vp, vp]),
         "tvk_dataset_info": (C.c_int, [vp, C.POINTER(DatasetDesc)]),
         "tvk_lod_info": (C.c_int, [vp, C.c_int, u64p, u64p, u64p, dp]),
         "tvk_brick_info": (C.c_int, [vp, C.c_uint64, u64p, u32p, u64p]),
@@ -93,11 +110,38 @@ def load(path=LIB_PATH):
         "tvk_render": (C.c_int, [vp, fp, C.POINTER(Stats)]),
         "tvk_render_views": (C.c_int, [vp, C.c_int, fp, fp, C.POINTER(Stats)]),
         "tvk_set_count_samples": (C.c_int, [vp, C.c_int]),
-        "tvk_touched_bytes": (C.c_int, [vp, u64p, u64p]),
+        "tvk_touched_bytes": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
         "tvk_frame_device_ptr": (C.c_int, [vp, C.POINTER(vp)]),
         "tvk_event_record": (C.c_int, [vp, C.c_int]),
         "tvk_event_elapsed": (C.c_int, [vp, C.c_int, C.c_int, fp]),
         "tvk_synchronize": (C.c_int, [vp]),
+        "tvk_default_tuning": (C.c_int, [C.POINTER(Tuning)]),
+        "tvk_batch_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "tvk_batch_destroy": (None, [vp]),
+        "tvk_batch_last_error": (C.c_char_p, [vp]),
+        "tvk_batch_clear": (C.c_int, [vp]),
+        "tvk_batch_add": (C.c_int, [vp, vp, fp, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
+        "tvk_batch_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "tvk_batch_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "tvk_batch_set_count_samples": (C.c_int, [vp, C.c_int]),
+        "tvk_batch_render": (C.c_int, [vp, C.POINTER(Stats)]),
+        "tvk_batch_frame": (C.c_int, [vp, C.c_int, fp]),
+        "tvk_batch_frame_device_ptr": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+        "tvk_batch_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tuning), fp]),
+        "tvk_stepnet_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_uint64]),
+        "tvk_stepnet_destroy": (None, [vp]),
+        "tvk_stepnet_last_error": (C.c_char_p, [vp]),
+        "tvk_stepnet_param_count": (C.c_int, []),
+        "tvk_stepnet_set_params": (C.c_int, [vp, fp, C.c_int]),
+        "tvk_stepnet_get_params": (C.c_int, [vp, fp, C.c_int]),
+        "tvk_stepnet_infer": (C.c_int, [vp, fp, C.c_int, fp]),
+        "tvk_stepnet_set_data": (C.c_int, [vp, fp, fp, C.c_int]),
+        "tvk_stepnet_grad": (C.c_int, [vp, C.c_int, fp]),
+        "tvk_stepnet_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, fp]),
+        "tvk_set_step_proposal": (C.c_int, [vp, vp, C.POINTER(ProposalParams)]),
+        "tvk_batch_set_step_proposal": (C.c_int, [vp, vp, C.POINTER(ProposalParams)]),
+        "tvk_generate_segments": (C.c_int, [vp, C.c_int, fp, C.c_uint32, C.c_float, C.c_uint32, fp,
+                                            C.POINTER(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -22,8 +22,11 @@ SOURCES = [
     ("kernels/march.hip", ["-ffp-contract=off"]),
     ("kernels/bricking.hip", []),
     ("kernels/generate.hip", []),
+    ("kernels/stepnet.hip", []),
     ("host/host.cpp", ["-ffp-contract=off"]),
     ("api.cpp", ["-ffp-contract=off"]),
+    ("batch.cpp", ["-ffp-contract=off"]),
+    ("stepnet_api.cpp", []),
 ]
 
 COMMON = [

@@ -1,130 +1,27 @@
 // C ABI of the raycasting path (include/tvk_hip.h).
-#include <hip/hip_runtime.h>
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <new>
-#include <string>
-#include <vector>
-#include "tvk_hip.h"
-#include "host/host.hpp"
-#include "kernels/generate.hpp"
-#include "tvk_types.hpp"
-
-namespace tvk {
-hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st);
-hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
-                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, hipStream_t st);
-hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
-                             hipStream_t st);
-hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t* out,
-                         hipStream_t st);
-hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bricks, const uint64_t* d_ref_off,
-                            uint32_t nb, uint32_t max_vox, void* out, hipStream_t st);
-hipError_t launch_max_gradient(const void* lin, int bpv, const uint64_t d[3], int clamp, double* out, uint32_t blocks,
-                               hipStream_t st);
-hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bricks, const uint32_t bricks[3],
-                               uint32_t ov, const uint32_t* d_wave, uint32_t wave_n, uint32_t max_shell,
-                               hipStream_t st);
-}  // namespace tvk
+#include "ctx.hpp"
 
 using namespace tvk;
 
-struct tvk_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  // dataset
-  bool has_ds = false;
-  Layout L;
-  int clamp = 0;
-  void* d_pool = nullptr;
-  size_t pool_bytes = 0;
-  void* d_lin = nullptr;
-  size_t lin_bytes = 0;
-  std::vector<double> minmax;
-  double range_min = 0, range_max = 0;
-  // transfer functions
-  std::vector<uint8_t> tf1d, tf2d;
-  int tf1d_n = 0, tf2d_w = 0, tf2d_h = 0;
-  uint8_t* d_tf1d = nullptr;
-  uint8_t* d_tf2d = nullptr;
-  float4* d_tf2f = nullptr;
-  size_t tf1d_cap = 0, tf2d_cap = 0, tf2f_cap = 0;
-  // view + parameters
-  ViewState V;
-  bool has_view = false;
-  tvk_params P{};
-  // frame resources
-  FrameBrick* d_list = nullptr;
-  size_t list_cap = 0;
-  ViewConst* d_views = nullptr;
-  size_t views_cap = 0;
-  float* d_out = nullptr;
-  size_t out_cap = 0;
-  unsigned long long* d_counter = nullptr;
-  bool count = false;
-  uint32_t* d_touched = nullptr;
-  size_t touched_cap = 0, touched_n = 0;
-  hipEvent_t ev[8] = {};
-  std::vector<FrameBrick> h_list;
-  std::vector<ViewConst> h_views;
-  FrameBrick* h_list_pinned = nullptr;
-  size_t h_list_pinned_cap = 0;
-  ViewConst* h_views_pinned = nullptr;
-  size_t h_views_pinned_cap = 0;
-  int last_views = 0;
-  uint32_t last_w = 0, last_h = 0;
-  int last_lod = 0;
-};
-
 namespace {
 
-int fail(tvk_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  return code;
-}
-int hipfail(tvk_ctx* c, const char* what, hipError_t e) {
-  return fail(c, e == hipErrorOutOfMemory ? TVK_ERR_NOMEM : TVK_ERR_HIP,
-              std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPCHK(c, x)                                  \
-  do {                                                \
-    hipError_t e_ = (x);                              \
-    if (e_ != hipSuccess) return hipfail(c, #x, e_);  \
-  } while (0)
-
-template <typename T>
-int ensure(tvk_ctx* c, T** p, size_t* cap, size_t n) {
-  if (*cap >= n && *p) return TVK_OK;
-  if (*p) { (void)hipFree(*p); *p = nullptr; *cap = 0; }
-  size_t want = std::max<size_t>(n, 1);
-  HIPCHK(c, hipMalloc((void**)p, want * sizeof(T)));
-  *cap = want;
-  return TVK_OK;
-}
-
-template <typename T>
-int ensure_pinned(tvk_ctx* c, T** p, size_t* cap, size_t n) {
-  if (*cap >= n && *p) return TVK_OK;
-  if (*p) { (void)hipHostFree(*p); *p = nullptr; *cap = 0; }
-  size_t want = std::max<size_t>(n * 2, 64);
-  HIPCHK(c, hipHostMalloc((void**)p, want * sizeof(T), hipHostMallocDefault));
-  *cap = want;
-  return TVK_OK;
+std::vector<float4> unorm_table(const uint8_t* rgba8, size_t n) {
+  std::vector<float4> f(n);
+  for (size_t i = 0; i < n; ++i)
+    f[i] = make_float4((float)rgba8[4 * i] / 255.0f, (float)rgba8[4 * i + 1] / 255.0f,
+                       (float)rgba8[4 * i + 2] / 255.0f, (float)rgba8[4 * i + 3] / 255.0f);
+  return f;
 }
 
 void free_dataset(tvk_ctx* c) {
-  if (c->d_pool) (void)hipFree(c->d_pool);
-  if (c->d_lin) (void)hipFree(c->d_lin);
+  c->pool_hold.reset();   // frees the pool once no context shares it
+  c->lin_hold.reset();
   c->d_pool = c->d_lin = nullptr;
   c->pool_bytes = c->lin_bytes = 0;
   c->has_ds = false;
   c->minmax.clear();
 }
 
-int bit_width(const tvk_ctx* c) { return c->L.bpv == 1 ? 8 : 16; }
 
 // Brick the linear volume at d_lin (dims of c->L) into the pool.
 int build_pool(tvk_ctx* c, const void* d_lin) {
@@ -136,6 +33,7 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
     return fail(c, TVK_ERR_UNSUPPORTED, "coarser LODs need an even brick core (brick size - 2*overlap)");
   c->pool_bytes = (size_t)L.pool_elems * bpv;
   HIPCHK(c, hipMalloc(&c->d_pool, c->pool_bytes));
+  c->pool_hold = std::shared_ptr<void>(c->d_pool, [](void* p) { (void)hipFree(p); });
   // descriptors of every kept brick
   std::vector<GatherBrick> gb(L.toc.size());
   for (size_t i = 0; i < L.toc.size(); ++i) {
@@ -312,6 +210,8 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_tf1d) (void)hipFree(c->d_tf1d);
   if (c->d_tf2d) (void)hipFree(c->d_tf2d);
   if (c->d_tf2f) (void)hipFree(c->d_tf2f);
+  if (c->d_tf1f) (void)hipFree(c->d_tf1f);
+  if (c->d_tf2cols) (void)hipFree(c->d_tf2cols);
   if (c->d_list) (void)hipFree(c->d_list);
   if (c->d_views) (void)hipFree(c->d_views);
   if (c->d_out) (void)hipFree(c->d_out);
@@ -335,12 +235,13 @@ int tvk_load_volume(tvk_ctx* c, const void* host, int dtype, const uint64_t dims
   if (rc) return rc;
   const size_t bytes = (size_t)(dims[0] * dims[1] * dims[2]) * dtype;
   HIPCHK(c, hipMalloc(&c->d_lin, bytes));
+  c->lin_hold = std::shared_ptr<void>(c->d_lin, [](void* p) { (void)hipFree(p); });
   c->lin_bytes = bytes;
   HIPCHK(c, hipMemcpyAsync(c->d_lin, host, bytes, hipMemcpyHostToDevice, c->stream));
   rc = build_pool(c, c->d_lin);
   if (!keep_linear || rc) {
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(c->d_lin);
+    c->lin_hold.reset();
     c->d_lin = nullptr;
     c->lin_bytes = 0;
   }
@@ -356,6 +257,7 @@ int tvk_generate_volume(tvk_ctx* c, int kind, uint64_t seed, int dtype, const ui
   if (rc) return rc;
   const size_t bytes = (size_t)(dims[0] * dims[1] * dims[2]) * dtype;
   HIPCHK(c, hipMalloc(&c->d_lin, bytes));
+  c->lin_hold = std::shared_ptr<void>(c->d_lin, [](void* p) { (void)hipFree(p); });
   c->lin_bytes = bytes;
   SplitMix rng(seed);
   hipError_t e = hipSuccess;
@@ -427,12 +329,33 @@ int tvk_generate_volume(tvk_ctx* c, int kind, uint64_t seed, int dtype, const ui
   rc = build_pool(c, c->d_lin);
   if (!keep_linear || rc) {
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(c->d_lin);
+    c->lin_hold.reset();
     c->d_lin = nullptr;
     c->lin_bytes = 0;
   }
   if (rc) free_dataset(c);
   return rc;
+}
+
+int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src) {
+  if (!dst || !src || dst == src) return TVK_ERR_ARG;
+  if (!src->has_ds) return fail(dst, TVK_ERR_STATE, "source context has no dataset");
+  if (src->device != dst->device) return fail(dst, TVK_ERR_ARG, "contexts are on different devices");
+  HIPCHK(dst, hipStreamSynchronize(src->stream));   // the source's bricking is complete
+  free_dataset(dst);
+  dst->L = src->L;
+  dst->clamp = src->clamp;
+  dst->d_pool = src->d_pool;
+  dst->pool_hold = src->pool_hold;
+  dst->pool_bytes = src->pool_bytes;
+  dst->d_lin = src->d_lin;
+  dst->lin_hold = src->lin_hold;
+  dst->lin_bytes = src->lin_bytes;
+  dst->minmax = src->minmax;
+  dst->range_min = src->range_min;
+  dst->range_max = src->range_max;
+  dst->has_ds = true;
+  return TVK_OK;
 }
 
 int tvk_download_volume(tvk_ctx* c, void* host, size_t bytes) {
@@ -584,9 +507,14 @@ int tvk_set_tf1d(tvk_ctx* c, const uint8_t* rgba8, int n) {
   if (!rgba8 || n < 2 || n > 4096) return fail(c, TVK_ERR_ARG, "1D TF needs 2..4096 RGBA8 entries");
   c->tf1d.assign(rgba8, rgba8 + 4 * (size_t)n);
   c->tf1d_n = n;
+  c->tfst_dirty = true;
   int rc = ensure(c, &c->d_tf1d, &c->tf1d_cap, 4 * (size_t)n);
+  if (!rc) rc = ensure(c, &c->d_tf1f, &c->tf1f_cap, (size_t)n);
   if (rc) return rc;
+  // texture1D of the RGBA8 table returns byte/255 (GL_NEAREST): expand once
+  const std::vector<float4> f = unorm_table(rgba8, (size_t)n);
   HIPCHK(c, hipMemcpyAsync(c->d_tf1d, c->tf1d.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tf1f, f.data(), f.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TVK_OK;
 }
@@ -597,16 +525,20 @@ int tvk_set_tf2d(tvk_ctx* c, const uint8_t* rgba8, int w, int h) {
   c->tf2d.assign(rgba8, rgba8 + 4 * (size_t)w * h);
   c->tf2d_w = w;
   c->tf2d_h = h;
+  c->tfst_dirty = true;
   int rc = ensure(c, &c->d_tf2d, &c->tf2d_cap, 4 * (size_t)w * h);
   if (!rc) rc = ensure(c, &c->d_tf2f, &c->tf2f_cap, (size_t)w * h);
+  if (!rc && !c->d_tf2cols) { size_t cap = 0; rc = ensure(c, &c->d_tf2cols, &cap, 64); }
   if (rc) return rc;
   // texture2D of an RGBA8 table returns byte/255 (GL_NEAREST): expand once
-  std::vector<float4> f((size_t)w * h);
-  for (size_t i = 0; i < f.size(); ++i)
-    f[i] = make_float4((float)rgba8[4 * i] / 255.0f, (float)rgba8[4 * i + 1] / 255.0f,
-                       (float)rgba8[4 * i + 2] / 255.0f, (float)rgba8[4 * i + 3] / 255.0f);
+  const std::vector<float4> f = unorm_table(rgba8, (size_t)w * h);
+  uint64_t cols[64] = {0};
+  for (int x = 0; x < w; ++x)
+    for (int y = 0; y < h; ++y)
+      if (rgba8[4 * ((size_t)y * w + x) + 3]) { cols[x >> 6] |= 1ull << (x & 63); break; }
   HIPCHK(c, hipMemcpyAsync(c->d_tf2d, c->tf2d.data(), 4 * (size_t)w * h, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_tf2f, f.data(), f.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tf2cols, cols, sizeof(cols), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TVK_OK;
 }
@@ -645,6 +577,7 @@ int tvk_set_camera(tvk_ctx* c, const float eye[3], const float at[3], const floa
   c->V.znear = znear;
   c->V.zfar = zfar;
   c->has_view = true;
+  c->proj_from_fov = true;
   return TVK_OK;
 }
 
@@ -669,6 +602,7 @@ int tvk_set_params(tvk_ctx* c, const tvk_params* p) {
   if (!(p->srm > 0.0f)) return fail(c, TVK_ERR_ARG, "sample rate modifier must be > 0");
   if (p->refine_steps < 0 || p->refine_steps > 64) return fail(c, TVK_ERR_ARG, "refine_steps out of range");
   if (p->ert_eps < 0.0f || p->ert_eps >= 1.0f) return fail(c, TVK_ERR_ARG, "ert_eps must be in [0,1)");
+  if (p->mode != c->P.mode) c->tfst_dirty = true;
   c->P = *p;
   return TVK_OK;
 }
@@ -687,7 +621,7 @@ int tvk_set_count_samples(tvk_ctx* c, int on) {
 
 }  // extern "C"
 
-namespace {
+namespace tvk {
 
 int frame_lod(tvk_ctx* c, const ViewState& V, int* lod) {
   int l = c->P.lod;
@@ -725,7 +659,123 @@ float tf_scale_of(const tvk_ctx* c) {
   return float(iMaxValue / fMaxValue);
 }
 
-int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st) {
+// Per-entry alpha and colour of the active TF (a 2D TF contributes, per value
+// column, its largest alpha and that row's colour), with prefix sums and
+// sparse max tables so that any value range is summarised in O(1).
+void build_tf_stats(tvk_ctx* c) {
+  tvk_ctx::TfStats& T = c->tfst;
+  const bool two = c->P.mode == TVK_RM_2DTRANS;
+  const int n = two ? c->tf2d_w : c->tf1d_n;
+  T = tvk_ctx::TfStats{};
+  T.n = n;
+  c->tfst_dirty = false;
+  if (n <= 0) return;
+  std::vector<float> a((size_t)n), rgb(3 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const uint8_t* e = nullptr;
+    if (two) {
+      int best = 0;
+      for (int y = 0; y < c->tf2d_h; ++y)
+        if (c->tf2d[4 * ((size_t)y * n + i) + 3] > c->tf2d[4 * ((size_t)best * n + i) + 3]) best = y;
+      e = &c->tf2d[4 * ((size_t)best * n + i)];
+    } else {
+      e = &c->tf1d[4 * (size_t)i];
+    }
+    a[(size_t)i] = e[3] / 255.0f;
+    for (int k = 0; k < 3; ++k) rgb[3 * (size_t)i + k] = e[k] / 255.0f;
+  }
+  T.pa.assign((size_t)n + 1, 0.0);
+  T.pnz.assign((size_t)n + 1, 0.0);
+  T.ptv.assign((size_t)n, 0.0);
+  T.prgb.assign((size_t)n, 0.0);
+  std::vector<float> slope((size_t)std::max(n - 1, 1), 0.0f);
+  for (int i = 0; i < n; ++i) {
+    T.pa[(size_t)i + 1] = T.pa[(size_t)i] + a[(size_t)i];
+    T.pnz[(size_t)i + 1] = T.pnz[(size_t)i] + (a[(size_t)i] > 0.0f ? 1.0 : 0.0);
+    if (i + 1 < n) {
+      const float d = std::fabs(a[(size_t)i + 1] - a[(size_t)i]);
+      float dr = 0.0f;
+      for (int k = 0; k < 3; ++k) dr += std::fabs(rgb[3 * ((size_t)i + 1) + k] - rgb[3 * (size_t)i + k]);
+      slope[(size_t)i] = d;
+      T.ptv[(size_t)i + 1] = T.ptv[(size_t)i] + d;
+      T.prgb[(size_t)i + 1] = T.prgb[(size_t)i] + dr / 3.0f;
+    }
+  }
+  auto sparse = [](const std::vector<float>& v, std::vector<std::vector<float>>& t) {
+    t.clear();
+    t.push_back(v);
+    for (size_t w = 1; 2 * w <= v.size(); w *= 2) {
+      const std::vector<float>& p = t.back();
+      std::vector<float> q(v.size() - 2 * w + 1);
+      for (size_t i = 0; i < q.size(); ++i) q[i] = std::max(p[i], p[i + w]);
+      t.push_back(q);
+    }
+  };
+  sparse(a, T.amax);
+  sparse(slope, T.smax);
+}
+
+float range_max(const std::vector<std::vector<float>>& t, int lo, int hi) {   // inclusive, lo <= hi
+  int k = 0;
+  while ((2 << k) <= hi - lo + 1) ++k;
+  return std::max(t[(size_t)k][(size_t)lo], t[(size_t)k][(size_t)(hi - (1 << k) + 1)]);
+}
+
+void brick_tf_stats(tvk_ctx* c, FrameBrick& fb) {
+  if (c->tfst_dirty) build_tf_stats(c);
+  const tvk_ctx::TfStats& T = c->tfst;
+  for (float& v : fb.tfs) v = 0.0f;
+  if (T.n <= 0 || c->P.mode == TVK_RM_ISOSURFACE) return;
+  // TF coordinate and entry of a stored value, as the march computes them
+  const float inv = c->L.bpv == 1 ? 1.0f / 255.0f : 1.0f / 65535.0f;
+  const float sc = tf_scale_of(c);
+  auto coord = [&](double v) { return std::min(std::max((float)v * inv * sc, 0.0f), 1.0f); };
+  auto entry = [&](float cc) { return std::min((int)std::floor(cc * (float)T.n), T.n - 1); };
+  const float c0 = coord(c->minmax[2 * (size_t)fb.toc]), c1 = coord(c->minmax[2 * (size_t)fb.toc + 1]);
+  const int lo = entry(c0), hi = entry(c1);
+  const double cnt = (double)(hi - lo + 1);
+  fb.tfs[0] = c0;
+  fb.tfs[1] = c1;
+  fb.tfs[2] = (float)((T.pa[(size_t)hi + 1] - T.pa[(size_t)lo]) / cnt);
+  fb.tfs[3] = range_max(T.amax, lo, hi);
+  fb.tfs[4] = (float)(T.ptv[(size_t)hi] - T.ptv[(size_t)lo]);
+  fb.tfs[5] = (float)(T.prgb[(size_t)hi] - T.prgb[(size_t)lo]);
+  fb.tfs[6] = hi > lo ? range_max(T.smax, lo, hi - 1) : 0.0f;
+  fb.tfs[7] = (float)((T.pnz[(size_t)hi + 1] - T.pnz[(size_t)lo]) / cnt);
+}
+
+// Shading constants of the context's dataset, TF and parameters
+void fill_shade(const tvk_ctx* c, ShadeConst& S) {
+  S = ShadeConst{};
+  S.pool = (const uint8_t*)c->d_pool;
+  S.tf1 = c->d_tf1f;
+  S.tf2f = c->d_tf2f;
+  S.tf2_cols = c->d_tf2cols;
+  S.tf_n = c->tf1d_n;
+  S.tf_w = c->tf2d_w;
+  S.tf_h = c->tf2d_h;
+  S.tf_scale = tf_scale_of(c);
+  S.grad_scale = c->P.grad_scale;
+  S.iso_n = (float)((double)c->P.isovalue / (double)(1 << bit_width(c)));  // GetNormalizedIsovalue
+  S.ert_eps = c->P.ert_eps;
+  for (int i = 0; i < 3; ++i) {
+    S.dscale[i] = c->P.domain_scale[i];
+    S.amb[i] = c->P.light_ambient[i];
+    S.dif[i] = c->P.light_diffuse[i];
+    S.spc[i] = c->P.light_specular[i];
+    S.ldir[i] = c->P.light_dir[i];
+    S.iso_col[i] = c->P.iso_color[i];
+  }
+  S.refine_steps = c->P.refine_steps;
+  S.clamp_edge = c->P.clamp_to_edge;
+}
+
+// learn: 0 fixed step, 1 step proposal of c->net, 2 fixed step recording segment samples into *seg
+int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st, int learn = -1,
+                const SegOut* seg = nullptr) {
+  if (learn < 0) learn = (c->net && c->P.mode != TVK_RM_ISOSURFACE) ? 1 : 0;
+  if (learn > 0 && c->P.mode == TVK_RM_ISOSURFACE)
+    return fail(c, TVK_ERR_UNSUPPORTED, "step proposal is not available in isosurface mode");
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
   if (!c->has_view) return fail(c, TVK_ERR_STATE, "no view");
   if (nviews < 1) return fail(c, TVK_ERR_ARG, "nviews must be >= 1");
@@ -751,6 +801,8 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
     vc.list_offset = (uint32_t)c->h_list.size();
     make_frame_bricks(c->L, V, pp, planned, c->h_list);
     vc.nlist = (uint32_t)c->h_list.size() - vc.list_offset;
+    if (learn > 0)
+      for (size_t q = vc.list_offset; q < c->h_list.size(); ++q) brick_tf_stats(c, c->h_list[q]);
     vc.out_offset = (uint32_t)pixels;
     vc.group_offset = groups;
     groups += vc.groups;
@@ -775,15 +827,10 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   HIPCHK(c, hipMemcpyAsync(c->d_views, c->h_views_pinned, (size_t)nviews * sizeof(ViewConst), hipMemcpyHostToDevice,
                            c->stream));
   MarchArgs a{};
-  a.pool = (const uint8_t*)c->d_pool;
+  fill_shade(c, a.S);
   a.list = c->d_list;
   a.views = c->d_views;
   a.mode = c->P.mode;
-  a.tf = c->P.mode == TVK_RM_2DTRANS ? c->d_tf2d : c->d_tf1d;
-  a.tf2f = c->d_tf2f;
-  a.tf_n = c->tf1d_n;
-  a.tf_w = c->tf2d_w;
-  a.tf_h = c->tf2d_h;
   a.out = c->d_out;
   a.counter = c->count ? c->d_counter : nullptr;
   a.ntoc = (uint32_t)c->L.toc.size();
@@ -797,31 +844,17 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
     a.touched = c->d_touched;
   }
   a.nviews = nviews;
-  a.tf_scale = tf_scale_of(c);
-  a.grad_scale = c->P.grad_scale;
-  a.iso_n = (float)((double)c->P.isovalue / (double)(1 << bit_width(c)));  // GetNormalizedIsovalue
-  a.ert_eps = c->P.ert_eps;
-  for (int i = 0; i < 3; ++i) {
-    a.dscale[i] = c->P.domain_scale[i];
-    a.amb[i] = c->P.light_ambient[i];
-    a.dif[i] = c->P.light_diffuse[i];
-    a.spc[i] = c->P.light_specular[i];
-    a.ldir[i] = c->P.light_dir[i];
-    a.iso_col[i] = c->P.iso_color[i];
+  if (learn == 1) {
+    a.learn = c->learn;
+    if (!stepnet_device(c->net, &a.learn.net)) return fail(c, TVK_ERR_STATE, "no step network");
   }
-  if (c->P.mode == TVK_RM_2DTRANS) {
-    for (int x = 0; x < c->tf2d_w; ++x)
-      for (int y = 0; y < c->tf2d_h; ++y)
-        if (c->tf2d[4 * ((size_t)y * c->tf2d_w + x) + 3]) { a.tf2_cols[x >> 6] |= 1ull << (x & 63); break; }
-  }
-  a.refine_steps = c->P.refine_steps;
-  a.clamp_edge = c->P.clamp_to_edge;
+  if (learn == 2 && seg) a.seg = *seg;
   a.lighting = c->P.lighting;
   a.groups_total = groups;
   a.groups_padded = (groups + 7u) & ~7u;
   if (c->count) HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
-  if (a.groups_total > 0) HIPCHK(c, launch_march(a, c->L.bpv, c->stream));
+  if (a.groups_total > 0) HIPCHK(c, launch_march(a, c->L.bpv, c->stream, learn));
   HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
   c->last_views = nviews;
   c->last_w = c->V.W;
@@ -898,18 +931,83 @@ int tvk_render_views(tvk_ctx* c, int n, const float* rots, float* host_out, tvk_
   return render_impl(c, n, rots, host_out, st);
 }
 
-int tvk_touched_bytes(tvk_ctx* c, uint64_t* bytes, uint64_t* bricks) {
-  if (!c || !bytes || !bricks) return TVK_ERR_ARG;
+int tvk_set_step_proposal(tvk_ctx* c, tvk_stepnet* net, const tvk_proposal_params* p) {
+  if (!c) return TVK_ERR_ARG;
+  if (net && !p) return fail(c, TVK_ERR_ARG, "null proposal parameters");
+  c->net = net;
+  c->learn = LearnConst{};
+  if (net) {
+    if (p->max_log2 < 0 || p->max_log2 > 3) return fail(c, TVK_ERR_ARG, "max_log2 must be 0..3");
+    c->learn.margin = p->margin;
+    const double q = p->skip_threshold;
+    c->learn.skip_logit = q >= 1.0 ? INFINITY : (q <= 0.0 ? -INFINITY : (float)std::log(q / (1.0 - q)));
+    c->learn.max_log2 = p->max_log2;
+    c->learn.fallback = p->fallback_bound > 0.0f ? p->fallback_bound : INFINITY;
+  }
+  return TVK_OK;
+}
+
+int tvk_generate_segments(tvk_ctx* c, int nviews, const float* rots, uint32_t cap, float keep, uint32_t seed,
+                          float* host_out, uint32_t* n) {
+  if (!c || !rots || !host_out || !n || nviews < 1 || cap == 0) return TVK_ERR_ARG;
+  if (c->P.mode == TVK_RM_ISOSURFACE) return fail(c, TVK_ERR_UNSUPPORTED, "segments need a TF render mode");
+  if (!(keep > 0.0f) || keep > 1.0f) return fail(c, TVK_ERR_ARG, "keep fraction must be in (0,1]");
+  HIPCHK(c, hipSetDevice(c->device));
+  SegSample* d_out = nullptr;
+  unsigned int* d_cnt = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d_out, (size_t)cap * sizeof(SegSample)));
+  hipError_t e = hipMalloc((void**)&d_cnt, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned int), c->stream);
+  int rc = e == hipSuccess ? TVK_OK : hipfail(c, "segment buffers", e);
+  if (!rc) {
+    SegOut so;
+    so.out = d_out;
+    so.count = d_cnt;
+    so.cap = cap;
+    so.keep = keep >= 1.0f ? 0xffffffffu : (uint32_t)((double)keep * 4294967296.0);
+    so.seed = seed;
+    const float eps = c->P.ert_eps;
+    c->P.ert_eps = 0.0f;   // the recorded marches run without early termination
+    rc = render_impl(c, nviews, rots, nullptr, nullptr, 2, &so);
+    c->P.ert_eps = eps;
+  }
+  unsigned int cnt = 0;
+  if (!rc) {
+    e = hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cnt = std::min(cnt, cap);
+    if (e == hipSuccess && cnt)
+      e = hipMemcpy(host_out, d_out, (size_t)cnt * sizeof(SegSample), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hipfail(c, "segment download", e);
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(d_out);
+  if (d_cnt) (void)hipFree(d_cnt);
+  *n = rc ? 0 : cnt;
+  return rc;
+}
+
+int tvk_touched_bytes(tvk_ctx* c, uint64_t* bytes, uint64_t* bricks, uint64_t* unique_bytes,
+                      uint64_t* unique_bricks) {
+  if (!c || !bytes || !bricks || !unique_bytes || !unique_bricks) return TVK_ERR_ARG;
   if (!c->touched_n) return fail(c, TVK_ERR_STATE, "no counted render yet");
   std::vector<uint32_t> t(c->touched_n);
   HIPCHK(c, hipMemcpyAsync(t.data(), c->d_touched, t.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const size_t nt = c->L.toc.size();
-  uint64_t b = 0, n = 0;
+  std::vector<uint8_t> any(nt, 0);
+  uint64_t b = 0, n = 0, ub = 0, un = 0;
   for (size_t i = 0; i < t.size(); ++i)
-    if (t[i]) { ++n; b += c->L.toc[i % nt].real * (uint64_t)c->L.bpv; }
+    if (t[i]) {
+      const size_t k = i % nt;
+      const uint64_t bb = c->L.toc[k].real * (uint64_t)c->L.bpv;
+      ++n; b += bb;
+      if (!any[k]) { any[k] = 1; ++un; ub += bb; }
+    }
   *bytes = b;
   *bricks = n;
+  *unique_bytes = ub;
+  *unique_bricks = un;
   return TVK_OK;
 }
 

@@ -1,0 +1,158 @@
+// Internal state of a tvk_ctx and helpers shared by the C ABI sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+#include "tvk_hip.h"
+#include "host/host.hpp"
+#include "kernels/generate.hpp"
+#include "tvk_types.hpp"
+
+namespace tvk {
+hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st, int learn);
+hipError_t launch_batch(const BatchArgs& b, int mode, bool lit, int bpv, int groups, hipStream_t st, int learn);
+hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
+                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, hipStream_t st);
+hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
+                             hipStream_t st);
+hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t* out,
+                         hipStream_t st);
+hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bricks, const uint64_t* d_ref_off,
+                            uint32_t nb, uint32_t max_vox, void* out, hipStream_t st);
+hipError_t launch_max_gradient(const void* lin, int bpv, const uint64_t d[3], int clamp, double* out, uint32_t blocks,
+                               hipStream_t st);
+hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bricks, const uint32_t bricks[3],
+                               uint32_t ov, const uint32_t* d_wave, uint32_t wave_n, uint32_t max_shell,
+                               hipStream_t st);
+}  // namespace tvk
+
+
+// the opaque handle of the C ABI
+struct tvk_ctx {
+  using Layout = tvk::Layout;
+  using ViewState = tvk::ViewState;
+  using FrameBrick = tvk::FrameBrick;
+  using ViewConst = tvk::ViewConst;
+  using LearnConst = tvk::LearnConst;
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // dataset
+  bool has_ds = false;
+  Layout L;
+  int clamp = 0;
+  void* d_pool = nullptr;
+  std::shared_ptr<void> pool_hold;   // owns d_pool; shared by contexts that share the dataset
+  std::shared_ptr<void> lin_hold;    // owns d_lin
+  size_t pool_bytes = 0;
+  void* d_lin = nullptr;
+  size_t lin_bytes = 0;
+  std::vector<double> minmax;
+  double range_min = 0, range_max = 0;
+  // transfer functions
+  std::vector<uint8_t> tf1d, tf2d;
+  int tf1d_n = 0, tf2d_w = 0, tf2d_h = 0;
+  uint8_t* d_tf1d = nullptr;
+  uint8_t* d_tf2d = nullptr;
+  float4* d_tf1f = nullptr;    // tables as the sampler returns them (byte / 255)
+  float4* d_tf2f = nullptr;
+  uint64_t* d_tf2cols = nullptr;  // 64 words: value columns of the 2D TF with a non-zero alpha
+  size_t tf1d_cap = 0, tf2d_cap = 0, tf1f_cap = 0, tf2f_cap = 0;
+  // view + parameters
+  ViewState V;
+  bool has_view = false;
+  bool proj_from_fov = false;   // projection made by tvk_set_camera (re-made per request aspect)
+  tvk_params P{};
+  // frame resources
+  FrameBrick* d_list = nullptr;
+  size_t list_cap = 0;
+  ViewConst* d_views = nullptr;
+  size_t views_cap = 0;
+  float* d_out = nullptr;
+  size_t out_cap = 0;
+  unsigned long long* d_counter = nullptr;
+  bool count = false;
+  // step proposal (learned mode): the network and its decision thresholds
+  struct tvk_stepnet* net = nullptr;
+  tvk::LearnConst learn{};
+  // per-entry statistics of the active TF (brick features of the step proposal)
+  struct TfStats {
+    int n = 0;
+    std::vector<double> pa, ptv, prgb, pnz;   // prefix sums of alpha, |dalpha|, |drgb|, alpha > 0
+    std::vector<std::vector<float>> amax, smax;   // sparse tables: max alpha, max |dalpha|
+  } tfst;
+  bool tfst_dirty = true;
+  uint32_t* d_touched = nullptr;
+  size_t touched_cap = 0, touched_n = 0;
+  hipEvent_t ev[8] = {};
+  std::vector<FrameBrick> h_list;
+  std::vector<ViewConst> h_views;
+  FrameBrick* h_list_pinned = nullptr;
+  size_t h_list_pinned_cap = 0;
+  ViewConst* h_views_pinned = nullptr;
+  size_t h_views_pinned_cap = 0;
+  int last_views = 0;
+  uint32_t last_w = 0, last_h = 0;
+  int last_lod = 0;
+};
+
+
+namespace tvk {
+
+inline int fail(tvk_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+inline int hipfail(tvk_ctx* c, const char* what, hipError_t e) {
+  return fail(c, e == hipErrorOutOfMemory ? TVK_ERR_NOMEM : TVK_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(c, x)                                  \
+  do {                                                \
+    hipError_t e_ = (x);                              \
+    if (e_ != hipSuccess) return hipfail(c, #x, e_);  \
+  } while (0)
+
+template <typename T>
+int ensure(tvk_ctx* c, T** p, size_t* cap, size_t n) {
+  if (*cap >= n && *p) return TVK_OK;
+  if (*p) { (void)hipFree(*p); *p = nullptr; *cap = 0; }
+  size_t want = std::max<size_t>(n, 1);
+  HIPCHK(c, hipMalloc((void**)p, want * sizeof(T)));
+  *cap = want;
+  return TVK_OK;
+}
+
+template <typename T>
+int ensure_pinned(tvk_ctx* c, T** p, size_t* cap, size_t n) {
+  if (*cap >= n && *p) return TVK_OK;
+  if (*p) { (void)hipHostFree(*p); *p = nullptr; *cap = 0; }
+  size_t want = std::max<size_t>(n * 2, 64);
+  HIPCHK(c, hipHostMalloc((void**)p, want * sizeof(T), hipHostMallocDefault));
+  *cap = want;
+  return TVK_OK;
+}
+
+inline int bit_width(const tvk_ctx* c) { return c->L.bpv == 1 ? 8 : 16; }
+
+// LOD of a frame (fixed, or the screen-space-error minimum of the view)
+int frame_lod(tvk_ctx* c, const ViewState& V, int* lod);
+// planner parameters (mode, LOD, TF non-zero limits) of the context
+int plan_params(tvk_ctx* c, int lod, PlanParams& pp);
+// GLRenderer::CalculateScaling
+float tf_scale_of(const tvk_ctx* c);
+// TF statistics over a brick's value range (FrameBrick::tfs)
+void brick_tf_stats(tvk_ctx* c, FrameBrick& fb);
+// the step network as the kernels read it (false when s is null)
+bool stepnet_device(const struct tvk_stepnet* s, StepNetDev* out);
+// shading constants of the context's dataset, TF and parameters
+void fill_shade(const tvk_ctx* c, ShadeConst& S);
+
+}  // namespace tvk

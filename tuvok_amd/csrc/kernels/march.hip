@@ -24,6 +24,7 @@
 // order, so step counts, sample positions and TF indices are reproducible.
 #include <hip/hip_runtime.h>
 #include "../tvk_types.hpp"
+#include "stepnet.hpp"
 
 namespace tvk {
 
@@ -42,7 +43,13 @@ __device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
 __device__ __forceinline__ V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 __device__ __forceinline__ float pow8(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x4; }
-__device__ __forceinline__ float fp16_round(float x) { return (float)(_Float16)x; }
+// The entry point goes through an RGBA16F target: the fp32 value is rounded to
+// fp16.  The empty asm keeps the product that feeds it an fp32 value (otherwise
+// the mixed-precision FMA rounds the exact product to fp16 in one step).
+__device__ __forceinline__ float fp16_round(float x) {
+  asm volatile("" : "+v"(x));
+  return (float)(_Float16)x;
+}
 
 // value of a stored voxel as the unorm texture returns it
 template <typename T> struct Unorm;
@@ -93,14 +100,17 @@ __device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, i
   const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
   const uint32_t xA = ox<T>(xa), xB = ox<T>(xb), yA = oy<T>(ya, nbx), yB = oy<T>(yb, nbx);
   const uint32_t zA = oz<T>(za, nbxy), zB = oz<T>(zb, nbxy);
-  float c000 = (vxa && vya && vza) ? (float)d[zA + yA + xA] * inv : 0.0f;
-  float c100 = (vxb && vya && vza) ? (float)d[zA + yA + xB] * inv : 0.0f;
-  float c010 = (vxa && vyb && vza) ? (float)d[zA + yB + xA] * inv : 0.0f;
-  float c110 = (vxb && vyb && vza) ? (float)d[zA + yB + xB] * inv : 0.0f;
-  float c001 = (vxa && vya && vzb) ? (float)d[zB + yA + xA] * inv : 0.0f;
-  float c101 = (vxb && vya && vzb) ? (float)d[zB + yA + xB] * inv : 0.0f;
-  float c011 = (vxa && vyb && vzb) ? (float)d[zB + yB + xA] * inv : 0.0f;
-  float c111 = (vxb && vyb && vzb) ? (float)d[zB + yB + xB] * inv : 0.0f;
+  // the clamped addresses are always inside the brick: load unconditionally and
+  // select the border colour afterwards (no per-tap branch)
+  auto tap = [&](bool ok, uint32_t off) { const float v = (float)d[off] * inv; return ok ? v : 0.0f; };
+  float c000 = tap(vxa && vya && vza, zA + yA + xA);
+  float c100 = tap(vxb && vya && vza, zA + yA + xB);
+  float c010 = tap(vxa && vyb && vza, zA + yB + xA);
+  float c110 = tap(vxb && vyb && vza, zA + yB + xB);
+  float c001 = tap(vxa && vya && vzb, zB + yA + xA);
+  float c101 = tap(vxb && vya && vzb, zB + yA + xB);
+  float c011 = tap(vxa && vyb && vzb, zB + yB + xA);
+  float c111 = tap(vxb && vyb && vzb, zB + yB + xB);
   float c00 = c000 * (1.0f - f[0]) + c100 * f[0];
   float c10 = c010 * (1.0f - f[0]) + c110 * f[0];
   float c01 = c001 * (1.0f - f[0]) + c101 * f[0];
@@ -152,7 +162,9 @@ struct CrossSampler {
   bool okx[4], oky[4], okz[4];
 
   __device__ __forceinline__ float V(int i, int j, int k) const {   // i,j,k in 0..3 -> index ic-1+i ...
-    return (okx[i] && oky[j] && okz[k]) ? (float)d[ofz[k] + ofy[j] + ofx[i]] * Unorm<T>::inv() : 0.0f;
+    // offsets are clamped into the brick: unconditional load, then the border select
+    const float v = (float)d[ofz[k] + ofy[j] + ofx[i]] * Unorm<T>::inv();
+    return (okx[i] && oky[j] && okz[k]) ? v : 0.0f;
   }
 
   __device__ __forceinline__ float value(const T* __restrict__ dd, int nx, int ny, int nz, bool e, V3 pos) {
@@ -289,36 +301,24 @@ __device__ __forceinline__ V3 normal_eye(V3 g, const float* nm, V3 ds) {
   return n;
 }
 
-template <int MODE, bool LIT, typename T, bool COUNT>
-__global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
-  extern __shared__ float4 s_tf[];  // 1D TF as floats (tf_n entries, MODE 0 only)
-
-  // XCD-aware remap: consecutive workgroups go round-robin over the 8 XCDs;
-  // give each XCD a contiguous range of 16x16 groups.
-  const uint32_t lin = blockIdx.x;
-  const uint32_t per_xcd = A.groups_padded >> 3;
-  const uint32_t g = (lin & 7u) * per_xcd + (lin >> 3);
-  if (g >= A.groups_total) return;  // uniform for the whole workgroup
-
-  if (MODE == 0) {
-    for (int i = threadIdx.x; i < A.tf_n; i += 256) {
-      const uint8_t* t = A.tf + 4 * i;
-      s_tf[i] = make_float4((float)t[0] / 255.0f, (float)t[1] / 255.0f, (float)t[2] / 255.0f, (float)t[3] / 255.0f);
-    }
-    __syncthreads();
-  }
-
-  int v = 0;
-  while (v + 1 < A.nviews && g >= A.views[v + 1].group_offset) ++v;
-  const ViewConst* VC = &A.views[v];
-  const uint32_t gl = g - VC->group_offset;
-  const uint32_t gx = gl % VC->groups_x, gy = gl / VC->groups_x;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int tx = (int)gx * 2 + (wave & 1), ty = (int)gy * 2 + (wave >> 1);
-  const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+// One wave renders one tile of tw x th pixels (tw * th == 64, lane = x + tw * y)
+// with its top-left pixel at (tx0, ty0).  tf1 is the 1D TF as float4 (in LDS
+// or global memory), list the view's brick list, touched (COUNT only) the
+// view's touched-brick row.
+// LEARN: 0 fixed step; 1 step proposal by the network (LearnConst); 2 fixed step
+// that also records segment samples (SegOut) with the segment's RGBA at step
+// multipliers 1, 2, 4 and 8.  sn_lds: the wave's network scratch (LEARN > 0).
+template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
+__device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __restrict__ tf1, const ViewConst* VC,
+                                           const FrameBrick* __restrict__ list, float* __restrict__ out, int tx0,
+                                           int ty0, int tw, int th, unsigned long long* counter, uint32_t* touched,
+                                           const LearnConst* LC, const SegOut* SO, uint16_t* sn_lds,
+                                           uint32_t view_id) {
+  const int lane = threadIdx.x & 63;
+  const int px = tx0 + lane % tw, py = ty0 + lane / tw;
   const int W = (int)VC->W, H = (int)VC->H;
   const bool inside = px < W && py < H;
-  const int rx0 = tx * 8, ry0 = ty * 8, rx1 = tx * 8 + 7, ry1 = ty * 8 + 7;
+  const int rx0 = tx0, ry0 = ty0, rx1 = tx0 + tw - 1, ry1 = ty0 + th - 1;
 
   // eye-space ray through the pixel centre (eye at the origin)
   const float* ip = VC->inv_proj;
@@ -331,11 +331,11 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
   const V3 o_m = xform_point(mk(0, 0, 0), VC->inv_mv);
   const V3 d_m = xform_dir(d, VC->inv_mv);
 
-  const V3 amb = mk(A.amb[0], A.amb[1], A.amb[2]), dif = mk(A.dif[0], A.dif[1], A.dif[2]);
-  const V3 spc = mk(A.spc[0], A.spc[1], A.spc[2]), ldir = mk(A.ldir[0], A.ldir[1], A.ldir[2]);
-  const V3 dsc = mk(A.dscale[0], A.dscale[1], A.dscale[2]);
-  const float eps = A.ert_eps;
-  const bool edge = A.clamp_edge != 0;
+  const V3 amb = mk(S.amb[0], S.amb[1], S.amb[2]), dif = mk(S.dif[0], S.dif[1], S.dif[2]);
+  const V3 spc = mk(S.spc[0], S.spc[1], S.spc[2]), ldir = mk(S.ldir[0], S.ldir[1], S.ldir[2]);
+  const V3 dsc = mk(S.dscale[0], S.dscale[1], S.dscale[2]);
+  const float eps = S.ert_eps;
+  const bool edge = S.clamp_edge != 0;
 
   float dst0 = 0.f, dst1 = 0.f, dst2 = 0.f, dst3 = 0.f;
   bool done = !inside;
@@ -343,9 +343,9 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
   // isosurface state
   float best_depth = 1.0f, hit_param = 0.0f;
   bool have_hit = false;
-  V3 hit_pos = mk(0, 0, 0), hit_n = mk(0, 0, 0);
+  V3 hit_pos = mk(0, 0, 0), hit_tex = mk(0, 0, 0);
+  int hit_brick = 0;
 
-  const FrameBrick* list = A.list + VC->list_offset;
   const int nlist = (int)VC->nlist;
   for (int base = 0; base < nlist; base += 64) {
     const int j = base + lane;
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
       mask &= mask - 1;
       const FrameBrick* fb = &list[__builtin_amdgcn_readfirstlane(base + k)];
       if (MODE != 2 && eps > 0.0f && (1.0f - dst3) < eps) done = true;
-      if (done) continue;
+      if (LEARN == 0 && done) continue;
       // slab test in model space (t along the eye-space direction d)
       float tn = -INFINITY, tf = INFINITY;
       bool ok = true;
@@ -382,14 +382,14 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
       }
       ok = ok && tf > tn && tf > 0.0f;
       const float te = fmaxf(tn, 1.0f);  // near-plane entry when the front face is clipped
+      const bool active = !done && ok && tf > te;
       if (COUNT) {
         // every ray that reaches a brick takes at least one sample there
-        if (__ballot(ok && tf > te) != 0 && lane == __builtin_amdgcn_readfirstlane(lane))
-          A.touched[(size_t)v * A.ntoc + fb->toc] = 1u;
+        if (__ballot(active) != 0 && lane == __builtin_amdgcn_readfirstlane(lane)) touched[fb->toc] = 1u;
       }
-      if (!ok || tf <= te) continue;
-      V3 entry = d * te;
-      const V3 exitp = d * tf;
+      if (LEARN == 0 && !active) continue;
+      V3 entry = d * (active ? te : 1.0f);
+      const V3 exitp = d * (active ? tf : 2.0f);
       entry = mk(fp16_round(entry.x), fp16_round(entry.y), fp16_round(entry.z));
       const float* E = fb->e2t;
       const V3 entryTex = xform_e2t(entry, E), exitTex = xform_e2t(exitp, E);
@@ -398,26 +398,26 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
       const float rstep = fb->ray_step;
       const int steps = (int)(L / rstep) + 1;
       const V3 incTex = (exitTex - entryTex) / (L / rstep);
-      const T* __restrict__ bd = reinterpret_cast<const T*>(A.pool) + fb->offset;
+      const T* __restrict__ bd = reinterpret_cast<const T*>(S.pool) + fb->offset;
       const int nx = (int)fb->n[0], ny = (int)fb->n[1], nz = (int)fb->n[2];
       const V3 vstep = mk(fb->vstep[0], fb->vstep[1], fb->vstep[2]);
       const float sscale = fb->step_scale;
-      V3 pos = entryTex;
 
       if (MODE == 2) {
         // GLRaycaster-ISO-FS.glsl:56-105 + RefineIsosurface.glsl:37-52
+        V3 pos = entryTex;
         bool found = false;
         for (int i = 0; i < steps; ++i) {
           ++ns;
-          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= A.iso_n) { found = true; break; }
+          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= S.iso_n) { found = true; break; }
           pos = pos + incTex;
         }
         if (!found) continue;
         V3 rd = incTex / 2.0f;
         pos = pos - rd;
-        for (int kk = 0; kk < A.refine_steps; ++kk) {
+        for (int kk = 0; kk < S.refine_steps; ++kk) {
           rd = rd / 2.0f;
-          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= A.iso_n) pos = pos - rd; else pos = pos + rd;
+          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= S.iso_n) pos = pos - rd; else pos = pos + rd;
         }
         const float param = length(pos - entryTex) / length(exitTex - entryTex);
         const V3 he = entry * (1.0f - param) + exitp * param;
@@ -430,23 +430,32 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
           have_hit = true;
           hit_pos = he;
           hit_param = param;
-          hit_n = normal_eye(gradient<T>(bd, nx, ny, nz, edge, pos, vstep), VC->nm, dsc);
+          hit_tex = pos;   // the normal is taken here, after the brick loop
+          hit_brick = base + k;
         }
         continue;
       }
 
-      float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+      // the fixed-step march of this segment with the step multiplied by mult
+      // (mult 1: GLRaycaster's step; the opacity correction exponent scales with it)
+      auto segment = [&](float mult, float& c0, float& c1, float& c2, float& c3) {
+      const float rs = rstep * mult;
+      const int steps_m = (int)(L / rs) + 1;
+      const V3 inc_m = (exitTex - entryTex) / (L / rs);
+      const float ssc = sscale * mult;
+      c0 = 0.f; c1 = 0.f; c2 = 0.f; c3 = 0.f;
+      V3 pos = entryTex;
       V3 cur = entry;
-      const V3 rayInc = dir / L * rstep;
-      for (int i = 0; i < steps; ++i) {
+      const V3 rayInc = dir / L * rs;
+      for (int i = 0; i < steps_m; ++i) {
         ++ns;
         float s0, s1, s2, s3;
         CrossSampler<T> cs;
         const float val = cs.value(bd, nx, ny, nz, edge, pos);
         if (MODE == 0) {
-          const float cc = fminf(fmaxf(val * A.tf_scale, 0.0f), 1.0f);
-          const int ti = min((int)floorf(cc * (float)A.tf_n), A.tf_n - 1);
-          const float4 tv = s_tf[ti];
+          const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
+          const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
+          const float4 tv = tf1[ti];
           s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
           if (LIT && s3 != 0.0f) {   // a transparent sample composites to nothing: skip its shading
             const V3 gg = cs.gradient(vstep);
@@ -455,29 +464,29 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
             s0 = lc.x; s1 = lc.y; s2 = lc.z;
           }
           const float tt = 1.0f - s3;
-          s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+          s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
         } else {
-          const float sx = fminf(fmaxf(val * A.tf_scale, 0.0f), 1.0f);
-          const int xi = min((int)floorf(sx * (float)A.tf_w), A.tf_w - 1);
-          if (((A.tf2_cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
+          const float sx = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
+          const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
+          if (((S.tf2_cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
             // every row of this value column is transparent: alpha 0 whatever the gradient
             s0 = s1 = s2 = s3 = 0.0f;
           } else {
             const V3 gg = cs.gradient(vstep);
             const float gm = length(gg);
-            const float sy = fminf(fmaxf(1.0f - gm * A.grad_scale, 0.0f), 1.0f);
-            const int yi = min((int)floorf(sy * (float)A.tf_h), A.tf_h - 1);
-            const float4 tv = A.tf2f[(size_t)yi * (size_t)A.tf_w + (size_t)xi];   // byte/255, precomputed
+            const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
+            const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
+            const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];   // byte/255, precomputed
             s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
             if (LIT) {
               const V3 nn = normal_eye(gg, VC->nm, dsc);
               const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
               const float tt = 1.0f - s3;
-              s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+              s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
               s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
             } else {
               const float tt = 1.0f - s3;
-              s3 = 1.0f - (sscale == 1.0f ? tt : powf(tt, sscale));
+              s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
             }
           }
         }
@@ -490,8 +499,78 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
         if (LIT) cur = cur + rayInc;
         if (c3 >= 0.99f) break;
         if (eps > 0.0f && (1.0f - dst3) * (1.0f - c3) < eps) break;
-        pos = pos + incTex;
+        pos = pos + inc_m;
       }
+      };
+
+      float mult = 1.0f;
+      bool skip = false;
+      if (LEARN > 0) {
+        // step proposal features (all lanes: the network runs on the whole wave)
+        float f[SN_IN];
+        const float T0 = 1.0f - dst3;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f[q < 2 ? q : q + 1] = fb->tfs[q];
+        f[2] = fb->tfs[1] - fb->tfs[0];
+        f[9] = LIT ? 1.0f : 0.0f;
+        f[10] = MODE == 1 ? 1.0f : 0.0f;
+        V3 dv = mk(incTex.x * (float)nx, incTex.y * (float)ny, incTex.z * (float)nz);
+        const float dl = length(dv);
+        dv = dl > 0.0f ? dv * (1.0f / dl) : mk(0, 0, 0);
+        f[11] = fabsf(dv.x); f[12] = fabsf(dv.y); f[13] = fabsf(dv.z);
+        f[14] = (float)steps * (1.0f / 512.0f);
+        f[15] = T0;
+        f[16] = -log2f(fmaxf(T0, 1e-6f)) * (1.0f / 20.0f);
+        f[17] = sscale;
+        const float obound = 1.0f - powf(1.0f - fminf(fb->tfs[3], 1.0f), sscale * (float)steps);
+        f[18] = obound;
+        f[19] = T0 * obound;
+        f[20] = -entry.z * 0.25f;
+        f[21] = (float)nx * (1.0f / 256.0f);
+        f[22] = (fb->tfs[1] - fb->tfs[0]) * 16.0f / (float)max(steps, 1);
+        f[23] = 1.0f;
+        if (!active) {
+#pragma unroll
+          for (int q = 0; q < SN_IN; ++q) f[q] = 0.0f;
+        }
+        if (LEARN == 1) {
+          float y0, y1;
+          sn_eval_wave(LC->net, f, sn_lds, y0, y1);
+          const int k = max(0, min(LC->max_log2, (int)floorf(3.0f * y0 + 0.5f - LC->margin)));
+          mult = (float)(1 << k);
+          skip = y1 > LC->skip_logit;
+          if (f[19] > LC->fallback) { mult = 1.0f; skip = false; }   // bound exceeded: fixed step
+        } else {
+          if (active) {
+            // record this segment (hash-selected) with its RGBA at multipliers 1, 2, 4, 8
+            uint32_t h = (uint32_t)(px * 73856093) ^ (uint32_t)(py * 19349663) ^ (fb->toc * 83492791u) ^
+                         (view_id * 2654435761u) ^ SO->seed;
+            h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+            if (h < SO->keep) {
+              const unsigned int slot = atomicAdd(SO->count, 1u);
+              if (slot < SO->cap) {
+                SegSample* o = SO->out + slot;
+#pragma unroll
+                for (int q = 0; q < SN_IN; ++q) o->f[q] = f[q];
+                o->t_before = T0;
+                o->steps = (float)steps;
+#pragma unroll 1
+                for (int mi = 0; mi < 4; ++mi) {
+                  float a0, a1, a2, a3;
+                  const float dsave = dst3;
+                  dst3 = 0.0f;   // no early termination inside the recorded marches
+                  segment((float)(1 << mi), a0, a1, a2, a3);
+                  dst3 = dsave;
+                  o->c[mi][0] = a0; o->c[mi][1] = a1; o->c[mi][2] = a2; o->c[mi][3] = a3;
+                }
+              }
+            }
+          }
+        }
+        if (!active || skip) continue;
+      }
+      float c0, c1, c2, c3;
+      segment(mult, c0, c1, c2, c3);
       // blend (ONE_MINUS_DST_ALPHA, ONE) into the RGBA32F target
       const float t = 1.0f - dst3;
       dst0 = c0 * t + dst0;
@@ -506,13 +585,24 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
     uint32_t s = ns;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0 && s) atomicAdd(A.counter, (unsigned long long)s);
+    if (lane == 0 && s) atomicAdd(counter, (unsigned long long)s);
   }
   if (!inside) return;
   float4 res;
   if (MODE == 2) {
+    // Normal of the winning hit: per-lane brick, computed once all bricks are
+    // done (inside the brick loop the per-brick values of lanes that take
+    // different branches were kept in shared scalar state by the compiler).
+    V3 hit_n = mk(0, 0, 0);
+    if (have_hit) {
+      const FrameBrick* hb = &list[hit_brick];
+      const T* __restrict__ hd = reinterpret_cast<const T*>(S.pool) + hb->offset;
+      hit_n = normal_eye(gradient<T>(hd, (int)hb->n[0], (int)hb->n[1], (int)hb->n[2], edge, hit_tex,
+                                     mk(hb->vstep[0], hb->vstep[1], hb->vstep[2])),
+                         VC->nm, dsc);
+    }
     if (have_hit && hit_param != 0.0f) {
-      const V3 isoc = mk(A.iso_col[0], A.iso_col[1], A.iso_col[2]);
+      const V3 isoc = mk(S.iso_col[0], S.iso_col[1], S.iso_col[2]);
       const V3 c = iso_lighting(hit_pos, hit_n, amb, dif * isoc, spc, ldir);
       res = make_float4(c.x, c.y, c.z, 1.0f);
     } else {
@@ -521,34 +611,170 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
   } else {
     res = make_float4(dst0, dst1, dst2, dst3);
   }
-  reinterpret_cast<float4*>(A.out)[(size_t)VC->out_offset + (size_t)py * (size_t)W + (size_t)px] = res;
+  reinterpret_cast<float4*>(out)[(size_t)VC->out_offset + (size_t)py * (size_t)W + (size_t)px] = res;
 }
 
-// host-side launcher (extern "C++" within the library)
+// Multi-view launch: one 256-thread workgroup = 16x16 pixels = 4 waves of 8x8.
+template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
+__global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
+  extern __shared__ float4 s_tf[];  // 1D TF (tf_n entries, MODE 0 only)
+  __shared__ __align__(16) uint16_t s_sn[LEARN > 0 ? 4 * SN_WAVE_LDS : 8];   // step network scratch, one slice per wave
+
+  // XCD-aware remap: consecutive workgroups go round-robin over the 8 XCDs;
+  // give each XCD a contiguous range of 16x16 groups.
+  const uint32_t lin = blockIdx.x;
+  const uint32_t per_xcd = A.groups_padded >> 3;
+  const uint32_t g = (lin & 7u) * per_xcd + (lin >> 3);
+  if (g >= A.groups_total) return;  // uniform for the whole workgroup
+
+  if (MODE == 0) {
+    for (int i = threadIdx.x; i < A.S.tf_n; i += 256) s_tf[i] = A.S.tf1[i];
+    __syncthreads();
+  }
+
+  int v = 0;
+  while (v + 1 < A.nviews && g >= A.views[v + 1].group_offset) ++v;
+  const ViewConst* VC = &A.views[v];
+  const uint32_t gl = g - VC->group_offset;
+  const uint32_t gx = gl % VC->groups_x, gy = gl / VC->groups_x;
+  const int wave = threadIdx.x >> 6;
+  const int tx = (int)gx * 2 + (wave & 1), ty = (int)gy * 2 + (wave >> 1);
+  march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
+                                         A.counter, COUNT ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
+                                         &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v);
+}
+
+// Persistent batch launch: every wave pulls tiles from the per-XCD queues until
+// all are drained.  Requests in one launch share MODE, LIT and T.
+template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(
+    BatchArgs B, const BatchRequest* __restrict__ req) {
+  __shared__ __align__(16) uint16_t s_sn[LEARN > 0 ? 4 * SN_WAVE_LDS : 8];
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint32_t chunk = (B.tiles_total + 7u) >> 3;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t q = (xcd + k) & 7u;
+    const uint32_t lo = q * chunk, hi = min(B.tiles_total, lo + chunk);
+    if (lo >= hi) continue;
+    for (;;) {
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(&B.queues[q], 1u);
+      t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+      t += lo;
+      if (t >= hi) break;   // this chunk is drained (the counter keeps growing past it: harmless)
+      // request of tile t: last request whose tile_offset <= t
+      int r0 = 0, r1 = B.nreq - 1;
+      while (r0 < r1) {
+        const int m = (r0 + r1 + 1) >> 1;
+        if (req[m].tile_offset <= t) r0 = m; else r1 = m - 1;
+      }
+      const BatchRequest* R = &req[__builtin_amdgcn_readfirstlane(r0)];
+      const uint32_t lt = t - R->tile_offset;
+      const uint32_t ty = lt / R->tiles_x, tx = lt - ty * R->tiles_x;
+      march_tile<MODE, LIT, T, COUNT, LEARN>(R->S, R->S.tf1, &R->V, B.list + R->V.list_offset, B.out,
+                                             (int)(tx * B.tile_w), (int)(ty * B.tile_h), (int)B.tile_w,
+                                             (int)B.tile_h, B.counter, COUNT ? B.touched + R->touched_offset : nullptr,
+                                             &B.learn, nullptr, s_sn + (LEARN > 0 ? (threadIdx.x >> 6) * SN_WAVE_LDS : 0),
+                                             0u);
+    }
+  }
+}
+
+__global__ void fill_u32(unsigned int* p, unsigned int n, unsigned int v) {
+  const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// host-side launchers (extern "C++" within the library)
 template <int M, bool L, typename T>
-static void launch_one(const MarchArgs& a, dim3 grid, size_t lds, hipStream_t st) {
-  if (a.counter)
-    hipLaunchKernelGGL((march_kernel<M, L, T, true>), grid, dim3(256), lds, st, a);
+static void launch_one(const MarchArgs& a, dim3 grid, size_t lds, hipStream_t st, int learn) {
+  if (learn == 2 && M != 2)
+    hipLaunchKernelGGL((march_kernel<M, L, T, false, (M != 2 ? 2 : 0)>), grid, dim3(256), lds, st, a);
+  else if (learn == 1 && M != 2) {
+    if (a.counter)
+      hipLaunchKernelGGL((march_kernel<M, L, T, true, (M != 2 ? 1 : 0)>), grid, dim3(256), lds, st, a);
+    else
+      hipLaunchKernelGGL((march_kernel<M, L, T, false, (M != 2 ? 1 : 0)>), grid, dim3(256), lds, st, a);
+  } else if (a.counter)
+    hipLaunchKernelGGL((march_kernel<M, L, T, true, 0>), grid, dim3(256), lds, st, a);
   else
-    hipLaunchKernelGGL((march_kernel<M, L, T, false>), grid, dim3(256), lds, st, a);
+    hipLaunchKernelGGL((march_kernel<M, L, T, false, 0>), grid, dim3(256), lds, st, a);
 }
 
-hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st) {
+// learn: 0 fixed step, 1 step proposal (a.learn), 2 fixed step recording segment samples (a.seg)
+hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st, int learn) {
   dim3 grid(a.groups_padded);
-  const size_t lds = a.mode == 0 ? (size_t)a.tf_n * sizeof(float4) : 0;
+  const size_t lds = a.mode == 0 ? (size_t)a.S.tf_n * sizeof(float4) : 0;
   const bool lit = a.lighting != 0 && a.mode != 2;
   const bool u16 = bpv == 2;
+  if (learn != 0 && a.mode == 2) return hipErrorInvalidValue;
   switch (a.mode) {
     case 0:
-      if (lit) { if (u16) launch_one<0, true, uint16_t>(a, grid, lds, st); else launch_one<0, true, uint8_t>(a, grid, lds, st); }
-      else { if (u16) launch_one<0, false, uint16_t>(a, grid, lds, st); else launch_one<0, false, uint8_t>(a, grid, lds, st); }
+      if (lit) { if (u16) launch_one<0, true, uint16_t>(a, grid, lds, st, learn); else launch_one<0, true, uint8_t>(a, grid, lds, st, learn); }
+      else { if (u16) launch_one<0, false, uint16_t>(a, grid, lds, st, learn); else launch_one<0, false, uint8_t>(a, grid, lds, st, learn); }
       break;
     case 1:
-      if (lit) { if (u16) launch_one<1, true, uint16_t>(a, grid, lds, st); else launch_one<1, true, uint8_t>(a, grid, lds, st); }
-      else { if (u16) launch_one<1, false, uint16_t>(a, grid, lds, st); else launch_one<1, false, uint8_t>(a, grid, lds, st); }
+      if (lit) { if (u16) launch_one<1, true, uint16_t>(a, grid, lds, st, learn); else launch_one<1, true, uint8_t>(a, grid, lds, st, learn); }
+      else { if (u16) launch_one<1, false, uint16_t>(a, grid, lds, st, learn); else launch_one<1, false, uint8_t>(a, grid, lds, st, learn); }
       break;
     case 2:
-      if (u16) launch_one<2, false, uint16_t>(a, grid, lds, st); else launch_one<2, false, uint8_t>(a, grid, lds, st);
+      if (u16) launch_one<2, false, uint16_t>(a, grid, lds, st, 0); else launch_one<2, false, uint8_t>(a, grid, lds, st, 0);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int M, bool L, typename T>
+static void launch_batch_one(const BatchArgs& b, int groups, hipStream_t st, int learn) {
+  if (learn == 1 && M != 2) {
+    if (b.counter)
+      hipLaunchKernelGGL((batch_kernel<M, L, T, true, (M != 2 ? 1 : 0)>), dim3(groups), dim3(256), 0, st, b, b.req);
+    else
+      hipLaunchKernelGGL((batch_kernel<M, L, T, false, (M != 2 ? 1 : 0)>), dim3(groups), dim3(256), 0, st, b, b.req);
+  } else if (b.counter)
+    hipLaunchKernelGGL((batch_kernel<M, L, T, true, 0>), dim3(groups), dim3(256), 0, st, b, b.req);
+  else
+    hipLaunchKernelGGL((batch_kernel<M, L, T, false, 0>), dim3(groups), dim3(256), 0, st, b, b.req);
+}
+
+template <int M, bool L, typename T>
+static hipError_t occ_one(int* n) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, batch_kernel<M, L, T, false, 0>, 256, 0);
+}
+
+// workgroups of the persistent kernel that fit on one CU at once
+int batch_occupancy(int mode, bool lit, int bpv, int* n) {
+  const bool u16 = bpv == 2;
+  lit = lit && mode != 2;
+  if (mode == 0) return lit ? (u16 ? occ_one<0, true, uint16_t>(n) : occ_one<0, true, uint8_t>(n))
+                            : (u16 ? occ_one<0, false, uint16_t>(n) : occ_one<0, false, uint8_t>(n));
+  if (mode == 1) return lit ? (u16 ? occ_one<1, true, uint16_t>(n) : occ_one<1, true, uint8_t>(n))
+                            : (u16 ? occ_one<1, false, uint16_t>(n) : occ_one<1, false, uint8_t>(n));
+  return u16 ? occ_one<2, false, uint16_t>(n) : occ_one<2, false, uint8_t>(n);
+}
+
+// Clears the 8 queue counters with a kernel (graph-capturable) and launches the
+// persistent kernel with `groups` workgroups.
+hipError_t launch_batch(const BatchArgs& b, int mode, bool lit, int bpv, int groups, hipStream_t st, int learn) {
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, st, b.queues, 8u, 0u);
+  if (b.tiles_total == 0 || groups <= 0) return hipGetLastError();
+  const bool u16 = bpv == 2;
+  lit = lit && mode != 2;
+  if (learn != 0 && mode == 2) learn = 0;
+  switch (mode) {
+    case 0:
+      if (lit) { if (u16) launch_batch_one<0, true, uint16_t>(b, groups, st, learn); else launch_batch_one<0, true, uint8_t>(b, groups, st, learn); }
+      else { if (u16) launch_batch_one<0, false, uint16_t>(b, groups, st, learn); else launch_batch_one<0, false, uint8_t>(b, groups, st, learn); }
+      break;
+    case 1:
+      if (lit) { if (u16) launch_batch_one<1, true, uint16_t>(b, groups, st, learn); else launch_batch_one<1, true, uint8_t>(b, groups, st, learn); }
+      else { if (u16) launch_batch_one<1, false, uint16_t>(b, groups, st, learn); else launch_batch_one<1, false, uint8_t>(b, groups, st, learn); }
+      break;
+    case 2:
+      if (u16) launch_batch_one<2, false, uint16_t>(b, groups, st, 0); else launch_batch_one<2, false, uint8_t>(b, groups, st, 0);
       break;
     default:
       return hipErrorInvalidValue;

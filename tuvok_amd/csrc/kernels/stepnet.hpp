@@ -1,0 +1,138 @@
+// Step-proposal network: 24 features -> 64 -> 64 -> 2 (ReLU hidden layers),
+// BF16 weights and activations, FP32 accumulation on the gfx950 matrix cores
+// (v_mfma_f32_32x32x16_bf16).  Output 0 is the proposed log2 step multiplier
+// divided by 3 (multiplier 1..8), output 1 the logit of "skip this segment".
+//
+// Operand layouts of one 32x32x16 MFMA (lane l = 0..63, r = l % 32, h = l / 32):
+//   A (32 x 16):  lane l holds A[r][8h .. 8h+7]
+//   B (16 x 32):  lane l holds B[8h .. 8h+7][r]
+//   D (32 x 32):  lane l, element i (0..15) is D[8*(i/4) + 4h + i%4][r]
+// Every product below is written "NT": C = A * Bt^T with both A and Bt stored
+// row-major with K contiguous, so each operand is one 16-byte load per lane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tvk {
+
+constexpr int SN_IN = 24;     // features
+constexpr int SN_INP = 32;    // features padded to the MFMA K step
+constexpr int SN_HID = 64;
+constexpr int SN_OUT = 2;
+// fp32 parameter vector: W1[24][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]
+constexpr int SN_OFF_W1 = 0;
+constexpr int SN_OFF_B1 = SN_OFF_W1 + SN_IN * SN_HID;
+constexpr int SN_OFF_W2 = SN_OFF_B1 + SN_HID;
+constexpr int SN_OFF_B2 = SN_OFF_W2 + SN_HID * SN_HID;
+constexpr int SN_OFF_W3 = SN_OFF_B2 + SN_HID;
+constexpr int SN_OFF_B3 = SN_OFF_W3 + SN_HID * SN_OUT;
+constexpr int SN_NPARAM = SN_OFF_B3 + SN_OUT;   // 5890
+
+// Device copy of the network as the kernels read it (bf16 weights, K-contiguous).
+struct StepNetDev {
+  const uint16_t* w1t;   // [64][32]  W1[f][o] at [o][f], f >= 24 zero
+  const uint16_t* w2t;   // [64][64]  W2[i][o] at [o][i]
+  const uint16_t* w2;    // [64][64]  W2[i][o] at [i][o]  (backward)
+  const uint16_t* w3t;   // [32][64]  W3[i][o] at [o][i], o >= 2 zero
+  const float* b1;       // [64]
+  const float* b2;       // [64]
+  const float* b3;       // [2]
+};
+
+typedef __bf16 sn_bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short sn_u16x8 __attribute__((ext_vector_type(8)));
+typedef float sn_f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bf16 bits, round to nearest even (finite values)
+__host__ __device__ inline uint16_t bf16_bits(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_value(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+
+__device__ inline sn_f32x16 sn_mfma(sn_u16x8 a, sn_u16x8 b, sn_f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(sn_bf16x8, a), __builtin_bit_cast(sn_bf16x8, b),
+                                                 c, 0, 0, 0);
+}
+
+// one 32x32 tile of C = A * Bt^T, K a multiple of 16; rows m0.., n0..
+template <int K>
+__device__ inline sn_f32x16 sn_tile_nt(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, int m0, int n0,
+                                       int lane) {
+  sn_f32x16 acc;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+  const int r = lane & 31, kh = (lane >> 5) * 8;
+#pragma unroll
+  for (int ks = 0; ks < K / 16; ++ks) {
+    const sn_u16x8 a = *reinterpret_cast<const sn_u16x8*>(A + (size_t)(m0 + r) * lda + ks * 16 + kh);
+    const sn_u16x8 b = *reinterpret_cast<const sn_u16x8*>(Bt + (size_t)(n0 + r) * ldb + ks * 16 + kh);
+    acc = sn_mfma(a, b, acc);
+  }
+  return acc;
+}
+
+// row of D element i for this lane (the column is lane % 32)
+__device__ inline int sn_row(int i, int lane) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); }
+
+// orders LDS writes and reads of the lanes of one wave
+__device__ inline void sn_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS scratch per wave for sn_eval_wave: two buffers of 32 x 64 bf16
+constexpr int SN_WAVE_LDS = 2 * 32 * SN_HID;
+
+// Evaluates the network for the 64 rays of a wave (lane = ray) in two halves
+// of 32 rays.  f: this lane's features.  lds: SN_WAVE_LDS uint16 per wave.
+__device__ inline void sn_eval_wave(const StepNetDev& N, const float* f, uint16_t* lds, float& y0, float& y1) {
+  const int lane = threadIdx.x & 63;
+  uint16_t* buf0 = lds;                 // X [32][32], then H2 [32][64]
+  uint16_t* buf1 = lds + 32 * SN_HID;   // H1 [32][64]
+  float o0 = 0.0f, o1 = 0.0f;
+  for (int half = 0; half < 2; ++half) {
+    if ((lane >> 5) == half) {
+      uint16_t* x = buf0 + (lane & 31) * SN_INP;
+#pragma unroll
+      for (int k = 0; k < SN_INP; ++k) x[k] = k < SN_IN ? bf16_bits(f[k]) : (uint16_t)0;
+    }
+    sn_wave_sync();
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const sn_f32x16 acc = sn_tile_nt<SN_INP>(N.w1t, SN_INP, buf0, SN_INP, 32 * mt, 0, lane);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int o = 32 * mt + sn_row(i, lane);
+        buf1[(lane & 31) * SN_HID + o] = bf16_bits(fmaxf(acc[i] + N.b1[o], 0.0f));
+      }
+    }
+    sn_wave_sync();
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const sn_f32x16 acc = sn_tile_nt<SN_HID>(N.w2t, SN_HID, buf1, SN_HID, 32 * mt, 0, lane);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int o = 32 * mt + sn_row(i, lane);
+        buf0[(lane & 31) * SN_HID + o] = bf16_bits(fmaxf(acc[i] + N.b2[o], 0.0f));
+      }
+    }
+    sn_wave_sync();
+    const sn_f32x16 acc = sn_tile_nt<SN_HID>(N.w3t, SN_HID, buf0, SN_HID, 0, 0, lane);
+    // rows 0 and 1 (outputs) of column r sit in elements 0 and 1 of lanes 0..31
+    float a0 = acc[0] + N.b3[0], a1 = acc[1] + N.b3[1];
+    if (half == 0) { o0 = a0; o1 = a1; }
+    else {
+      a0 = __shfl_xor(a0, 32, 64);
+      a1 = __shfl_xor(a1, 32, 64);
+      if (lane >= 32) { o0 = a0; o1 = a1; }
+    }
+    sn_wave_sync();
+  }
+  y0 = o0;
+  y1 = o1;
+}
+
+}  // namespace tvk

@@ -2,6 +2,7 @@
 #pragma once
 #include <stdint.h>
 #include <hip/hip_runtime.h>
+#include "kernels/stepnet.hpp"
 
 namespace tvk {
 
@@ -38,8 +39,13 @@ struct FrameBrick {
   int16_t rect[4];      // conservative pixel rectangle x0,y0,x1,y1 (inclusive)
   uint32_t toc;         // TOC index of the brick
   uint32_t pad;
+  // transfer-function statistics over the brick's value range (step proposal
+  // features): vmin, vmax (TF coordinates), alpha mean, alpha max, alpha total
+  // variation, colour total variation, largest alpha step, fraction of the
+  // range with non-zero alpha
+  float tfs[8];
 };
-static_assert(sizeof(FrameBrick) == 128, "FrameBrick layout");
+static_assert(sizeof(FrameBrick) == 160, "FrameBrick layout");
 
 // Per-view constants.
 struct ViewConst {
@@ -65,24 +71,85 @@ struct GatherBrick {
   uint64_t offset;      // element offset in the pool
 };
 
+// Shading constants of one render request (a dataset, its TF and parameters).
+struct ShadeConst {
+  const uint8_t* pool;          // brick pool of the dataset
+  const float4* tf1;            // 1D TF as float4 (byte / 255), tf_n entries
+  const float4* tf2f;           // 2D TF as float4 (byte / 255), tf_w x tf_h
+  const uint64_t* tf2_cols;     // bit x: value column x of the 2D TF has a non-zero alpha (64 words)
+  int tf_n, tf_w, tf_h;
+  float tf_scale, grad_scale, iso_n, ert_eps;
+  float dscale[3], amb[3], dif[3], spc[3], ldir[3], iso_col[3];
+  int refine_steps, clamp_edge;
+};
+
+// Step proposal (learned mode): network and decision thresholds.
+struct LearnConst {
+  StepNetDev net;
+  float margin;        // subtracted from 3*y0 before rounding to a log2 multiplier
+  float skip_logit;    // skip a segment when y1 exceeds this logit
+  int max_log2;        // largest log2 multiplier (multiplier 1..8)
+  float fallback;      // fixed step when the segment's contribution bound exceeds this
+};
+
+// One training sample of the step proposal: a ray's segment through one brick.
+struct SegSample {
+  float f[24];         // features (the network input)
+  float t_before;      // transmittance of the ray before the segment
+  float steps;         // fixed-step sample count of the segment
+  float c[4][4];       // segment RGBA (premultiplied) at multipliers 1, 2, 4, 8
+  float pad[6];
+};
+static_assert(sizeof(SegSample) == 192, "SegSample layout");
+
+struct SegOut {
+  SegSample* out;
+  unsigned int* count;
+  uint32_t cap;
+  uint32_t keep;       // a segment is recorded when hash < keep (of 2^32)
+  uint32_t seed;
+};
+
 struct MarchArgs {
-  const uint8_t* pool;
+  ShadeConst S;
+  LearnConst learn;
+  SegOut seg;
   const FrameBrick* list;
   const ViewConst* views;
-  const uint8_t* tf;            // RGBA8 table (1D: tf_n entries; 2D: tf_w x tf_h)
-  const float4* tf2f;           // 2D table as float4 (byte / 255)
   float* out;                   // RGBA32F
   unsigned long long* counter;  // composited-sample counter or nullptr
   uint32_t* touched;            // [view * ntoc + toc] = 1 once a sample is taken in the brick (or nullptr)
   uint32_t ntoc;
   int nviews;
-  int tf_n, tf_w, tf_h;
-  float tf_scale, grad_scale, iso_n, ert_eps;
-  float dscale[3], amb[3], dif[3], spc[3], ldir[3], iso_col[3];
-  int refine_steps, clamp_edge, lighting, mode;
-  uint64_t tf2_cols[64];        // bit x: value column x of the 2D TF has a non-zero alpha (w <= 4096)
+  int lighting, mode;
   uint32_t groups_total;        // sum of groups over the views
   uint32_t groups_padded;       // rounded up to a multiple of 8 (XCD count)
+};
+
+// One request of a batch: its shading constants, view, and place in the batch.
+struct BatchRequest {
+  ShadeConst S;
+  ViewConst V;                  // V.list_offset / V.out_offset index the batch buffers
+  uint32_t tile_offset;         // first tile of the request in the batch tile space
+  uint32_t ntiles;              // tiles_x * tiles_y
+  uint32_t tiles_x;             // tiles per row (tile_w wide)
+  uint32_t touched_offset;      // first entry of the request's touched-brick row
+};
+
+// Persistent batch launch: waves pull tiles from 8 per-XCD queues (the tile
+// space is cut into 8 contiguous chunks; a wave drains its own XCD's chunk, then
+// the others in turn) and exit once every chunk is exhausted.
+struct BatchArgs {
+  LearnConst learn;
+  const BatchRequest* req;
+  const FrameBrick* list;
+  float* out;
+  unsigned int* queues;         // 8 counters, cleared by a fill kernel before the launch
+  unsigned long long* counter;  // composited-sample counter or nullptr
+  uint32_t* touched;            // per request: touched_offset + toc (or nullptr)
+  int nreq;
+  uint32_t tiles_total;
+  uint32_t tile_w, tile_h;      // wave tile shape, tile_w * tile_h == 64
 };
 
 }  // namespace tvk

@@ -94,6 +94,10 @@ class HIPRaycaster:
                                               b.ctypes.data_as(C.POINTER(C.c_uint32)), overlap, int(clamp_to_edge),
                                               max_lods, int(keep_linear)))
 
+    def ShareDataset(self, other):
+        """Render `other`'s dataset (its HBM brick pool is shared, not copied)."""
+        self._chk(self._L.tvk_share_dataset(self._h, other._h))
+
     def DownloadVolume(self):
         d = self.GetDatasetDesc()
         dt = np.uint8 if d.bit_width == 8 else np.uint16
@@ -297,11 +301,34 @@ class HIPRaycaster:
                                            C.byref(self.last_stats)))
         return out
 
-    def TouchedBytes(self):
-        """Stored bytes and count of the bricks the last counted render sampled (summed over views)."""
-        b, n = C.c_uint64(), C.c_uint64()
-        self._chk(self._L.tvk_touched_bytes(self._h, C.byref(b), C.byref(n)))
-        return b.value, n.value
+    # -- step proposal ------------------------------------------------------------
+    def SetStepProposal(self, net, margin=0.0, skip_threshold=1.0, max_log2=3, fallback_bound=0.0):
+        """Attach a StepNet (tuvok_amd.stepnet) to march with proposed step
+        multipliers; net=None restores the fixed step."""
+        if net is None:
+            self._chk(self._L.tvk_set_step_proposal(self._h, None, None))
+            self._net = None
+            return
+        p = _abi.ProposalParams(float(margin), float(skip_threshold), int(max_log2), float(fallback_bound))
+        self._chk(self._L.tvk_set_step_proposal(self._h, net._h, C.byref(p)))
+        self._net = net   # keep it alive while attached
+
+    def GenerateSegments(self, rotations, cap, keep=1.0, seed=0):
+        """Segment samples of the fixed-step march over the given views (SegSample
+        records: features, transmittance before, step count, RGBA at x1/x2/x4/x8)."""
+        rots = np.ascontiguousarray(np.stack([np.asarray(r, np.float32).reshape(4, 4) for r in rotations]))
+        out = np.zeros((int(cap), 48), np.float32)
+        n = C.c_uint32()
+        self._chk(self._L.tvk_generate_segments(self._h, rots.shape[0], _fp(rots), int(cap), float(keep), int(seed),
+                                                _fp(out), C.byref(n)))
+        return out[:n.value]
+
+    def TouchedBytes(self, unique=False):
+        """Stored bytes and count of the bricks the last counted render sampled:
+        summed over its views, or (unique=True) over their union."""
+        b, n, ub, un = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._chk(self._L.tvk_touched_bytes(self._h, C.byref(b), C.byref(n), C.byref(ub), C.byref(un)))
+        return (ub.value, un.value) if unique else (b.value, n.value)
 
     # -- timing ---------------------------------------------------------------------
     def EventRecord(self, slot):

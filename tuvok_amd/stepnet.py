@@ -1,0 +1,119 @@
+"""Learned step proposal: the network, its training data and its training.
+
+StepNet wraps the device network (libtvkhip.so: BF16 MFMA inference and
+training kernels).  Training data are segment samples produced by the
+fixed-step march itself (HIPRaycaster.GenerateSegments): for every recorded
+ray segment through a brick, the network input, the ray's transmittance
+before the segment and the segment's RGBA marched at 1, 2, 4 and 8 times the
+fine step.  labels() turns them into targets: the largest multiplier whose
+error against the fine march stays within tau, and whether the whole segment
+contributes at most tau_skip.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import TvkError
+
+N_FEAT = 24
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class StepNet:
+    def __init__(self, device=0, seed=0):
+        self._L = _abi.load()
+        h = C.c_void_p()
+        rc = self._L.tvk_stepnet_create(C.byref(h), device, int(seed))
+        if rc != 0:
+            raise TvkError(rc, "tvk_stepnet_create failed (no GPU?)")
+        self._h = h
+        self.n_param = self._L.tvk_stepnet_param_count()
+        self._n = 0
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise TvkError(rc, self._L.tvk_stepnet_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.tvk_stepnet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        th = np.zeros(self.n_param, np.float32)
+        self._chk(self._L.tvk_stepnet_get_params(self._h, _fp(th), self.n_param))
+        return th
+
+    def set_params(self, th):
+        th = np.ascontiguousarray(th, np.float32)
+        self._chk(self._L.tvk_stepnet_set_params(self._h, _fp(th), th.size))
+
+    def infer(self, feat):
+        feat = np.ascontiguousarray(feat, np.float32).reshape(-1, N_FEAT)
+        out = np.zeros((feat.shape[0], 2), np.float32)
+        self._chk(self._L.tvk_stepnet_infer(self._h, _fp(feat), feat.shape[0], _fp(out)))
+        return out
+
+    def set_data(self, feat, tgt):
+        feat = np.ascontiguousarray(feat, np.float32).reshape(-1, N_FEAT)
+        tgt = np.ascontiguousarray(tgt, np.float32).reshape(-1, 3)
+        self._chk(self._L.tvk_stepnet_set_data(self._h, _fp(feat), _fp(tgt), feat.shape[0]))
+        self._n = feat.shape[0]
+
+    def grad(self, n=None):
+        """(gradient of the mean weighted loss over the first n loaded samples, summed loss)"""
+        g = np.zeros(self.n_param + 1, np.float32)
+        self._chk(self._L.tvk_stepnet_grad(self._h, int(n or self._n), _fp(g)))
+        return g[:-1], float(g[-1])
+
+    def train(self, epochs=10, batch=4096, lr=2e-3, weight_decay=0.0, seed=0, n=None):
+        losses = np.zeros(int(epochs), np.float32)
+        self._chk(self._L.tvk_stepnet_train(self._h, int(n or self._n), int(epochs), int(batch), float(lr),
+                                            float(weight_decay), int(seed), _fp(losses)))
+        return losses
+
+
+def split_samples(rec):
+    """SegSample records (n x 48) -> features, transmittance before, steps, RGBA[n, 4 multipliers, 4]."""
+    rec = np.asarray(rec, np.float32)
+    return rec[:, :24], rec[:, 24], rec[:, 25], rec[:, 26:42].reshape(-1, 4, 4)
+
+
+def segment_errors(rec):
+    """Error of each multiplier against the fine march, as the pixel sees it:
+    transmittance before the segment x the largest channel difference."""
+    _, tb, _, c = split_samples(rec)
+    d = np.abs(c - c[:, :1, :]).max(axis=2)          # n x 4
+    return tb[:, None] * d
+
+
+def labels(rec, tau, tau_skip, weight_pos=1.0):
+    """Targets (n x 3): t0 = log2(largest safe multiplier)/3, t1 = skip, weight."""
+    _, tb, _, c = split_samples(rec)
+    e = segment_errors(rec)
+    k = np.zeros(len(rec), np.int64)
+    ok = np.ones(len(rec), bool)
+    for m in (1, 2, 3):
+        ok &= e[:, m] <= tau
+        k = np.where(ok, m, k)
+    contrib = tb * c[:, 0, :].max(axis=1)
+    skip = (contrib <= tau_skip).astype(np.float32)
+    w = np.where(skip > 0, 1.0, weight_pos).astype(np.float32)
+    return np.stack([k / 3.0, skip, w], axis=1).astype(np.float32)
+
+
+def decide(y, margin, skip_threshold, max_log2=3):
+    """The kernel's decision rule on network outputs y (n x 2): (log2 multiplier, skip)."""
+    k = np.clip(np.floor(3.0 * y[:, 0] + 0.5 - margin), 0, max_log2).astype(np.int64)
+    p = 1.0 / (1.0 + np.exp(-y[:, 1].astype(np.float64)))
+    return k, p > skip_threshold

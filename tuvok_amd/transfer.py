@@ -96,6 +96,21 @@ def tf1d_blue_white_red(size=256, alpha_start=40, alpha_max=0.6):
     return tf
 
 
+def tf1d_random(seed, size=256):
+    """A random 1D TF: opacity 0 below a random threshold, then a linear ramp to a
+    random peak; colour interpolated between two random stops."""
+    rng = np.random.default_rng(seed)
+    lo = int(rng.integers(20, 160))
+    amax = float(rng.uniform(0.2, 0.9))
+    c0, c1 = rng.uniform(0, 1, 3), rng.uniform(0, 1, 3)
+    tf = TransferFunction1D(size)
+    i = np.arange(size, dtype=np.float64)
+    t = (i / (size - 1))[:, None]
+    tf.color[:, :3] = (c0 + (c1 - c0) * t).astype(np.float32)
+    tf.color[:, 3] = np.where(i < lo, 0.0, amax * (i - lo) / (size - 1 - lo)).astype(np.float32)
+    return tf
+
+
 def tf2d_gaussian_lobes(width=256, height=256, lobes=None):
     """2D TF made of Gaussian lobes in (value, gradient-row) space.
 

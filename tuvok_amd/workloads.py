@@ -3,13 +3,15 @@
 BLOBS_256 is the CPU-runnable 1D-TF case (256^3 uint8 blobs, 64^3 cores + 1
 voxel overlap, 512x512, one view).  FBM_SHELLS_1024 is the lit 2D-TF case at
 scale (1024^3 uint16 fBm + shells, 128^3 cores + 1 voxel overlap, 1920x1080,
-16 views on an orbit).
+16 views on an orbit).  SDF_ISO_2048 is the isosurface case (2048^3 uint8
+signed distance of 500 ellipsoids, iso 128, 4 bisection steps).  batch48()
+lists the 48 uneven requests over three uint8 volumes rendered as one batch.
 """
 import numpy as np
 
 from . import _abi
 from .renderer import HIPRaycaster, rotation_y
-from .transfer import tf1d_blue_white_red, tf2d_gaussian_lobes
+from .transfer import tf1d_blue_white_red, tf1d_random, tf2d_gaussian_lobes
 
 BLOBS_256 = dict(
     name="blobs256_u8_1dtf_512x512",
@@ -22,6 +24,31 @@ FBM_SHELLS_1024 = dict(
     gen=_abi.GEN_FBM_SHELLS, seed=1, dtype=np.uint16, dims=(1024, 1024, 1024), brick=(130, 130, 130), overlap=1,
     mode=_abi.RM_2DTRANS, lighting=True, width=1920, height=1080, fov=45.0, eye=(0.0, 0.0, 2.0), views=16,
 )
+
+
+SDF_ISO_2048 = dict(
+    name="sdf2048_u8_iso128_refine4_1080p",
+    gen=_abi.GEN_ELLIPSOID_SDF, seed=2, dtype=np.uint8, dims=(2048, 2048, 2048), brick=(130, 130, 130), overlap=1,
+    mode=_abi.RM_ISOSURFACE, lighting=True, isovalue=128.0, refine_steps=4, width=1920, height=1080, fov=45.0,
+    eye=(0.0, 0.0, 2.0), views=1,
+)
+
+# the three volumes of the batch: (generator, seed, edge)
+BATCH_VOLUMES = [(_abi.GEN_BLOBS, 10, 256), (_abi.GEN_ELLIPSOID_SDF, 11, 1024), (_abi.GEN_ELLIPSOID_SDF, 12, 2048)]
+
+
+def batch48(n=48, seed=7, min_px=256, max_px=2048):
+    """Requests of the batched workload: (volume index, TF seed, rotation, width,
+    height).  Image edges are log-uniform in [min_px, max_px] (independently for
+    width and height), volumes are picked uniformly, every request has its own
+    random 1D TF and view rotation."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        w = int(round(np.exp(rng.uniform(np.log(min_px), np.log(max_px)))))
+        h = int(round(np.exp(rng.uniform(np.log(min_px), np.log(max_px)))))
+        out.append((int(rng.integers(0, len(BATCH_VOLUMES))), 1000 + k, rotation_y(rng.uniform(0, 2 * np.pi)), w, h))
+    return out
 
 
 def orbit(views):
@@ -39,15 +66,49 @@ def setup(cfg, device=0, keep_linear=False, max_lods=1, ert_eps=0.0, dims=None, 
                        max_lods=max_lods, keep_linear=keep_linear)
     if cfg["mode"] == _abi.RM_1DTRANS:
         ren.Set1DTrans(tf1d_blue_white_red().GetByteArray())
-    else:
+    elif cfg["mode"] == _abi.RM_2DTRANS:
         ren.Set2DTrans(tf2d_gaussian_lobes().GetByteArray())
         if keep_linear:
             g = ren.MaxGradientMagnitude()
             ren.SetGradientScale(1.0 / g if g > 0 else 1.0)
     ren.SetRendermode(cfg["mode"])
     ren.SetUseLighting(cfg["lighting"])
+    if cfg["mode"] == _abi.RM_ISOSURFACE:
+        ren.SetIsoValue(cfg["isovalue"])
+        ren.SetRefineSteps(cfg["refine_steps"])
     ren.SetEarlyRayTermination(ert_eps)
     ren.SetViewParameters(cfg["fov"], 0.01, 1000.0)
     ren.SetViewPosition(cfg["eye"])
     ren.Resize(width or cfg["width"], height or cfg["height"])
     return ren
+
+
+def setup_batch(requests, device=0, scale=1, ert_eps=0.0, lighting=False, fov=45.0, eye=(0.0, 0.0, 2.0)):
+    """Renderers and the RenderBatch of the batched workload: one HIPRaycaster
+    per volume holds its dataset, one per request shares it and holds the
+    request's 1D TF.  scale divides every volume edge (reduced test cases)."""
+    from .batch import RenderBatch
+    owners = {}
+    rens = []
+    batch = RenderBatch(device)
+    for (vi, tf_seed, rot, w, h) in requests:
+        if vi not in owners:
+            gen, seed, edge = BATCH_VOLUMES[vi]
+            e = max(16, edge // scale)
+            r0 = HIPRaycaster(device)
+            r0.GenerateVolume(gen, seed, np.uint8, (e, e, e), (130, 130, 130) if e >= 512 else (66, 66, 66), 1,
+                              max_lods=1)
+            owners[vi] = r0
+        r = HIPRaycaster(device)
+        r.ShareDataset(owners[vi])
+        r.Set1DTrans(tf1d_random(tf_seed).GetByteArray())
+        r.SetRendermode(_abi.RM_1DTRANS)
+        r.SetUseLighting(lighting)
+        r.SetEarlyRayTermination(ert_eps)
+        r.SetViewParameters(fov, 0.01, 1000.0)
+        r.SetViewPosition(eye)
+        r.Resize(w, h)
+        r.SetRotation(rot)
+        batch.Add(r, rot, w, h)
+        rens.append(r)
+    return batch, rens, owners
