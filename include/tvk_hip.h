@@ -234,6 +234,33 @@ int tvk_batch_frame_device_ptr(tvk_batch* b, int id, void** dptr);
  * each after a warm-up), keeps and returns the fastest. */
 int tvk_batch_autotune(tvk_batch* b, int reps, tvk_tuning* best, float* best_ms);
 
+/* ---- ray-guided traversal (GLGridLeaper) ----------------------------------------
+ * Single-pass traversal of GLGridLeaper (GLGridLeaper.cpp, GLGridLeaper-blend.glsl,
+ * GLVolumePool.cpp:484-656): each ray walks the brick grid in normalised volume
+ * coordinates with a level of detail per brick from its depth, leaps over empty
+ * and child-empty bricks, and falls back to the next coarser resident level
+ * when a brick is not resident, reporting it (GLHashTable) and recording where
+ * it left the requested resolution.  Needs every level in the pool (max_lods 0)
+ * and a brick grid that halves evenly.  Modes: 1D and 2D TF, lit or not. */
+typedef struct {
+  uint64_t samples;      /* composited samples (when counting is on)                */
+  uint64_t missing;      /* bricks reported missing by the pass                      */
+  uint64_t unfinished;   /* rays that left the requested resolution (resume later)   */
+  float ms;              /* device time of the pass                                   */
+} tvk_leap_stats;
+/* residency per TOC brick (NULL: every brick resident); the coarsest level is always resident */
+int tvk_leap_set_resident(tvk_ctx* ctx, const uint8_t* flags, size_t n);
+/* one pass: first = 1 starts every ray at the volume box, 0 resumes the unfinished
+ * rays of the previous pass (GLGridLeaper::Continue3DDraw) */
+int tvk_leap_render(tvk_ctx* ctx, int first, float* host_out, tvk_leap_stats* stats);
+/* bricks reported missing by the last pass (flags per TOC brick, may be NULL) */
+int tvk_leap_missing(tvk_ctx* ctx, uint8_t* flags, size_t n, uint64_t* count);
+/* makes up to max_bricks reported bricks resident (the bricks already sit in the
+ * HBM pool: paging is a metadata update, GLVolumePool::UploadBricks) */
+int tvk_leap_page_in(tvk_ctx* ctx, uint64_t max_bricks, uint64_t* paged);
+/* resume state of the last pass: per pixel 8 floats (normalised xyz, eye z or 1000 when finished, RGBA) */
+int tvk_leap_resume_state(tvk_ctx* ctx, float* host, size_t floats);
+
 /* ---- step-proposal network ----------------------------------------------------
  * No reference counterpart (the reference marches at a fixed step).  A small
  * MLP, 24 features -> 64 -> 64 -> 2 with ReLU, BF16 weights and FP32

@@ -131,6 +131,18 @@ int orc_sn_param_count(void);
 void orc_sn_forward(const float* theta, const float* x, int n, int bf16, double* y);
 void orc_sn_grad(const float* theta, const float* x, const float* tgt, int n, int bf16, double* grad);
 
+/* ---- a15: GridLeaper single-pass traversal -------------------------------- */
+/* One pass of GLGridLeaper-blend.glsl over the whole frame.  eye: m_vEye (the
+ * renderer's camera position).  resident: per TOC brick (NULL: all); the
+ * coarsest level is always resident.  first: rays start at the volume box
+ * (else at resume_in).  out: W*H RGBA (accRayColor); resume_out: W*H*8
+ * (rayResumePos xyzw, rayResumeColor; w = 1000 marks a finished ray);
+ * missing: per TOC brick, 1 where a brick was reported missing.  Returns -1 if
+ * the pool's brick grid is not the octree's (needs power-of-two-friendly sizes). */
+int orc_leap_pass(const orc_ds* ds, const orc_view* v, const orc_params* p, const float eye[3],
+                  const uint8_t* resident, int first, const float* resume_in, float* out, float* resume_out,
+                  uint8_t* missing, uint64_t* samples);
+
 #ifdef __cplusplus
 }
 #endif

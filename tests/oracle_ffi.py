@@ -390,3 +390,33 @@ def sn_grad(theta, x, tgt, bf16=True):
                           tgt.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0], int(bf16),
                           g.ctypes.data_as(C.POINTER(C.c_double)))
     return g[:-1], g[-1]
+
+
+# ---- GridLeaper traversal --------------------------------------------------------
+def leap_pass(ds, camera, params, eye, resident=None, first=True, resume_in=None):
+    """One traversal pass of the oracle: (rgba[H,W,4], resume[H,W,8], missing[n], samples)."""
+    L = lib()
+    if not getattr(L, "_leap_ready", False):
+        L.orc_leap_pass.restype = C.c_int
+        L.orc_leap_pass.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                                    C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_uint8), C.POINTER(C.c_uint64)]
+        L._leap_ready = True
+    keep = []
+    v = make_view(camera)
+    prm = make_params(params, keep)
+    W, H = camera["width"], camera["height"]
+    out = np.zeros((H, W, 4), np.float32)
+    res = np.zeros((H, W, 8), np.float32)
+    miss = np.zeros(ds.brick_count, np.uint8)
+    e = np.ascontiguousarray(eye, np.float32)
+    rin = np.ascontiguousarray(resume_in if resume_in is not None else res, np.float32)
+    r8 = None if resident is None else np.ascontiguousarray(resident, np.uint8)
+    ns = C.c_uint64()
+    rc = L.orc_leap_pass(ds.h, C.byref(v), C.byref(prm), e.ctypes.data_as(C.POINTER(C.c_float)),
+                         r8.ctypes.data_as(C.POINTER(C.c_uint8)) if r8 is not None else None, int(first),
+                         rin.ctypes.data_as(C.POINTER(C.c_float)), out.ctypes.data_as(C.POINTER(C.c_float)),
+                         res.ctypes.data_as(C.POINTER(C.c_float)), miss.ctypes.data_as(C.POINTER(C.c_uint8)),
+                         C.byref(ns))
+    assert rc == 0, "brick grid of the pool differs from the octree's"
+    return out, res, miss, ns.value

@@ -1,0 +1,114 @@
+"""Ray-guided traversal (GLGridLeaper) on the GPU against the oracle's
+restatement of GLGridLeaper-blend.glsl: every pass's frame, resume state and
+missing-brick report, for all-resident pools, for partially resident pools
+paged in over several passes, and for views far enough away that coarser
+levels are chosen."""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from helpers import err_stats, oracle_camera_from, oracle_params_from
+from tuvok_amd import (GEN_BLOBS, GEN_FBM_SHELLS, RM_1DTRANS, RM_2DTRANS, HIPRaycaster, rotation_y,
+                       tf1d_blue_white_red, tf2d_gaussian_lobes)
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(kind="blobs", eye=(0.1, -0.2, 2.0), w=96, h=80):
+    r = HIPRaycaster(0)
+    if kind == "blobs":
+        r.GenerateVolume(GEN_BLOBS, 0, np.uint8, (64, 64, 64), (18, 18, 18), 1, max_lods=0, keep_linear=True)
+        tf = tf1d_blue_white_red().GetByteArray()
+        r.Set1DTrans(tf)
+        r.SetRendermode(RM_1DTRANS)
+        p = dict(tf1d=tf)
+    else:
+        r.GenerateVolume(GEN_FBM_SHELLS, 1, np.uint16, (64, 64, 64), (18, 18, 18), 1, max_lods=0, keep_linear=True)
+        tf = tf2d_gaussian_lobes().GetByteArray()
+        r.Set2DTrans(tf)
+        r.SetRendermode(RM_2DTRANS)
+        r.SetGradientScale(1.0 / r.MaxGradientMagnitude())
+        p = dict(tf2d=tf)
+    r.SetViewParameters(45.0, 0.01, 1000.0)
+    r.SetViewPosition(eye)
+    r.Resize(w, h)
+    vol = r.DownloadVolume()
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    assert ds.brick_count == r.GetDatasetDesc().brick_count
+    return r, ds, p, np.asarray(eye, np.float32)
+
+
+def check_pass(r, ds, p, eye, first, resident, resume_in, rot=None):
+    img, st = r.LeapRender(first=first)
+    res = r.LeapResumeState()
+    miss = r.LeapMissing()
+    cam = oracle_camera_from(r, rot)
+    ref, rres, rmiss, ns = O.leap_pass(ds, cam, oracle_params_from(r, **p), eye, resident, first, resume_in)
+    mx, mean = err_stats(img, ref)
+    print(f"  pass first={first}: max {mx:.3g} mean {mean:.3g} missing {int(miss.sum())}/{int(rmiss.sum())} "
+          f"unfinished {st.unfinished}", flush=True)
+    assert np.array_equal(miss, rmiss)
+    assert mx <= 1.0 / 255 and mean < 1e-3
+    assert np.abs(res - rres).max() <= 1e-4
+    return img, res, miss
+
+
+@pytest.mark.parametrize("kind,lit", [("blobs", False), ("blobs", True), ("fbm", False), ("fbm", True)])
+def test_all_resident_pass_matches_oracle(kind, lit):
+    print(f"leap {kind} lit={lit}", flush=True)
+    r, ds, p, eye = setup(kind)
+    r.SetUseLighting(lit)
+    for rot in (None, rotation_y(0.9)):
+        if rot is not None:
+            r.SetRotation(rot)
+        img, res, miss = check_pass(r, ds, p, eye, True, None, None, rot)
+        assert miss.sum() == 0 and (res[..., 3] == 1000).all()
+        assert img[..., 3].max() > 0.05
+    r.close()
+
+
+def test_paging_passes_match_oracle_and_converge():
+    """Only the coarsest level resident at first: each pass falls back to coarser
+    data, reports the missing bricks and records resume points; paging them in
+    and resuming converges to the all-resident frame."""
+    r, ds, p, eye = setup("blobs", eye=(0.0, 0.0, 1.6))
+    r.SetUseLighting(False)
+    n = ds.brick_count
+    nl = ds.lod_count
+    coarse = np.array([ds.brick_info(i)["coords"][3] >= nl - 2 for i in range(n)], np.uint8)
+    r.LeapSetResident(coarse)
+    resident = coarse.copy()
+    resume = None
+    first = True
+    for k in range(12):
+        img, res, miss = check_pass(r, ds, p, eye, first, resident, resume)
+        if miss.sum() == 0:
+            break
+        r.LeapPageIn()
+        resident = np.maximum(resident, miss)
+        resume = res
+        first = False
+    assert k > 0, "the first pass should have reported missing bricks"
+    assert miss.sum() == 0
+    r.LeapSetResident(None)
+    full, st = r.LeapRender(first=True)
+    mx, mean = err_stats(img, full)
+    print(f"  converged vs all resident: max {mx:.3g} mean {mean:.3g} after {k + 1} passes", flush=True)
+    assert mx <= 2.0 / 255 and mean < 1e-3
+    r.close()
+
+
+def test_distant_view_uses_coarser_levels():
+    """Far from the camera the per-brick level rises: fewer samples than a near view
+    of the same volume size on screen would need, and still the oracle's frame."""
+    r, ds, p, eye = setup("blobs", eye=(0.0, 0.0, 12.0), w=128, h=128)
+    r.SetUseLighting(False)
+    r.SetCountSamples(True)
+    img, res, miss = check_pass(r, ds, p, eye, True, None, None)
+    img2, st = r.LeapRender(first=True)
+    far_samples = st.samples
+    r.SetViewPosition((0.0, 0.0, 1.6))
+    img3, st3 = r.LeapRender(first=True)
+    print(f"  samples far {far_samples} near {st3.samples}", flush=True)
+    assert far_samples < st3.samples
+    r.close()

@@ -25,7 +25,8 @@ EXPORTS = [
     "tvk_stepnet_create", "tvk_stepnet_destroy", "tvk_stepnet_last_error", "tvk_stepnet_param_count",
     "tvk_stepnet_set_params", "tvk_stepnet_get_params", "tvk_stepnet_infer", "tvk_stepnet_set_data",
     "tvk_stepnet_grad", "tvk_stepnet_train", "tvk_set_step_proposal", "tvk_generate_segments",
-    "tvk_batch_set_step_proposal",
+    "tvk_batch_set_step_proposal", "tvk_leap_set_resident", "tvk_leap_render", "tvk_leap_missing",
+    "tvk_leap_page_in", "tvk_leap_resume_state",
 ]
 
 
@@ -58,6 +59,10 @@ class Tuning(C.Structure):
 class ProposalParams(C.Structure):
     _fields_ = [("margin", C.c_float), ("skip_threshold", C.c_float), ("max_log2", C.c_int),
                 ("fallback_bound", C.c_float)]
+
+
+class LeapStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("missing", C.c_uint64), ("unfinished", C.c_uint64), ("ms", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -140,6 +145,11 @@ def load(path=LIB_PATH):
         "tvk_stepnet_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, fp]),
         "tvk_set_step_proposal": (C.c_int, [vp, vp, C.POINTER(ProposalParams)]),
         "tvk_batch_set_step_proposal": (C.c_int, [vp, vp, C.POINTER(ProposalParams)]),
+        "tvk_leap_set_resident": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_size_t]),
+        "tvk_leap_render": (C.c_int, [vp, C.c_int, fp, C.POINTER(LeapStats)]),
+        "tvk_leap_missing": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_size_t, u64p]),
+        "tvk_leap_page_in": (C.c_int, [vp, C.c_uint64, u64p]),
+        "tvk_leap_resume_state": (C.c_int, [vp, fp, C.c_size_t]),
         "tvk_generate_segments": (C.c_int, [vp, C.c_int, fp, C.c_uint32, C.c_float, C.c_uint32, fp,
                                             C.POINTER(C.c_uint32)]),
     }

@@ -23,10 +23,12 @@ SOURCES = [
     ("kernels/bricking.hip", []),
     ("kernels/generate.hip", []),
     ("kernels/stepnet.hip", []),
+    ("kernels/leap.hip", ["-ffp-contract=off"]),
     ("host/host.cpp", ["-ffp-contract=off"]),
     ("api.cpp", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),
     ("stepnet_api.cpp", []),
+    ("leap_api.cpp", ["-ffp-contract=off"]),
 ]
 
 COMMON = [

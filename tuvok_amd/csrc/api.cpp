@@ -210,6 +210,8 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_tf1d) (void)hipFree(c->d_tf1d);
   if (c->d_tf2d) (void)hipFree(c->d_tf2d);
   if (c->d_tf2f) (void)hipFree(c->d_tf2f);
+  for (void* q : {(void*)c->d_leap_meta, (void*)c->d_leap_missing, (void*)c->d_leap_bricks, (void*)c->d_leap_resume})
+    if (q) (void)hipFree(q);
   if (c->d_tf1f) (void)hipFree(c->d_tf1f);
   if (c->d_tf2cols) (void)hipFree(c->d_tf2cols);
   if (c->d_list) (void)hipFree(c->d_list);
@@ -557,6 +559,8 @@ int tvk_set_view_matrices(tvk_ctx* c, const float view[16], const float rot[16],
   c->V.znear = znear;
   c->V.zfar = zfar;
   c->has_view = true;
+  c->proj_from_fov = false;
+  c->leap_eye = point_mul(F3(0, 0, 0), inverse(c->V.view));   // camera position of the view matrix
   return TVK_OK;
 }
 
@@ -578,6 +582,7 @@ int tvk_set_camera(tvk_ctx* c, const float eye[3], const float at[3], const floa
   c->V.zfar = zfar;
   c->has_view = true;
   c->proj_from_fov = true;
+  c->leap_eye = F3(eye[0], eye[1], eye[2]);
   return TVK_OK;
 }
 

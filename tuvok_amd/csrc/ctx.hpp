@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <new>
 #include <string>
@@ -89,6 +90,18 @@ struct tvk_ctx {
     std::vector<std::vector<float>> amax, smax;   // sparse tables: max alpha, max |dalpha|
   } tfst;
   bool tfst_dirty = true;
+  // ray-guided traversal (GridLeaper): residency, metadata, miss table, resume state
+  std::vector<uint8_t> leap_resident;   // per TOC brick (empty: all resident)
+  uint32_t* d_leap_meta = nullptr;
+  size_t leap_meta_cap = 0;
+  uint32_t* d_leap_missing = nullptr;
+  size_t leap_missing_cap = 0;
+  tvk::GatherBrick* d_leap_bricks = nullptr;
+  size_t leap_bricks_cap = 0;
+  float4* d_leap_resume = nullptr;
+  size_t leap_resume_cap = 0;
+  size_t leap_pixels = 0;
+  tvk::F3 leap_eye = tvk::F3(0.0f, 0.0f, 1.6f);   // m_vEye
   uint32_t* d_touched = nullptr;
   size_t touched_cap = 0, touched_n = 0;
   hipEvent_t ev[8] = {};

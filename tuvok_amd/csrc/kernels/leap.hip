@@ -1,0 +1,338 @@
+// GridLeaper-style single-pass traversal for gfx950: one ray per lane, 8x8
+// tiles per wave, rays walk the brick grid continuously in normalised volume
+// coordinates with a level of detail chosen per brick from the depth
+// (GLGridLeaper-blend.glsl:65-228 with the volume-pool functions of
+// GLVolumePool.cpp:484-656).  Empty bricks are leapt over (child-empty bricks
+// at coarser levels give longer leaps); a non-resident brick is reported in
+// the missing table and the next coarser resident level is used, and the ray
+// records where it first left the requested resolution so that a later pass
+// resumes from there once the brick is paged in.
+//
+// A brick is sampled in its own voxel coordinates in the blocked HBM pool (the
+// reference samples the same point through a 3D texture atlas).  The float
+// operations are those of the oracle in the same order (-ffp-contract=off).
+#include <hip/hip_runtime.h>
+#include "../tvk_types.hpp"
+#include "common.hpp"
+
+namespace tvk {
+namespace {
+
+enum : uint32_t { BI_MISSING = 0, BI_CHILD_EMPTY = 1, BI_EMPTY = 2, BI_FLAG_COUNT = 3 };
+
+// texture3D (GL_LINEAR, GL_CLAMP border 0) at voxel position b of a stored brick
+template <typename T>
+__device__ __forceinline__ float sample_vox(const T* __restrict__ d, int nx, int ny, int nz, V3 b) {
+  const float inv = Unorm<T>::inv();
+  const int n[3] = {nx, ny, nz};
+  const float bc[3] = {b.x, b.y, b.z};
+  int i0[3];
+  float f[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float s = fminf(fmaxf(bc[a], 0.0f), (float)n[a]);
+    const float u = s - 0.5f, fl = floorf(u);
+    i0[a] = (int)fl;
+    f[a] = u - fl;
+  }
+  const int xa = i0[0], xb = i0[0] + 1, ya = i0[1], yb = i0[1] + 1, za = i0[2], zb = i0[2] + 1;
+  const bool vxa = xa >= 0, vxb = xb < nx, vya = ya >= 0, vyb = yb < ny, vza = za >= 0, vzb = zb < nz;
+  const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
+  const uint32_t xA = ox<T>(min(max(xa, 0), nx - 1)), xB = ox<T>(min(max(xb, 0), nx - 1));
+  const uint32_t yA = oy<T>(min(max(ya, 0), ny - 1), nbx), yB = oy<T>(min(max(yb, 0), ny - 1), nbx);
+  const uint32_t zA = oz<T>(min(max(za, 0), nz - 1), nbxy), zB = oz<T>(min(max(zb, 0), nz - 1), nbxy);
+  auto tap = [&](bool ok, uint32_t off) { const float v = (float)d[off] * inv; return ok ? v : 0.0f; };
+  const float c000 = tap(vxa && vya && vza, zA + yA + xA), c100 = tap(vxb && vya && vza, zA + yA + xB);
+  const float c010 = tap(vxa && vyb && vza, zA + yB + xA), c110 = tap(vxb && vyb && vza, zA + yB + xB);
+  const float c001 = tap(vxa && vya && vzb, zB + yA + xA), c101 = tap(vxb && vya && vzb, zB + yA + xB);
+  const float c011 = tap(vxa && vyb && vzb, zB + yB + xA), c111 = tap(vxb && vyb && vzb, zB + yB + xB);
+  const float c00 = c000 * (1.0f - f[0]) + c100 * f[0];
+  const float c10 = c010 * (1.0f - f[0]) + c110 * f[0];
+  const float c01 = c001 * (1.0f - f[0]) + c101 * f[0];
+  const float c11 = c011 * (1.0f - f[0]) + c111 * f[0];
+  const float c0 = c00 * (1.0f - f[1]) + c10 * f[1];
+  const float c1 = c01 * (1.0f - f[1]) + c11 * f[1];
+  return c0 * (1.0f - f[2]) + c1 * f[2];
+}
+
+// GLGridLeaper-GradientTools.glsl ComputeGradient with one-voxel taps
+template <typename T>
+__device__ __forceinline__ V3 gradient_vox(const T* __restrict__ d, int nx, int ny, int nz, V3 c) {
+  const float xp = sample_vox<T>(d, nx, ny, nz, c + mk(+1.0f, 0, 0));
+  const float xm = sample_vox<T>(d, nx, ny, nz, c + mk(-1.0f, 0, 0));
+  const float yp = sample_vox<T>(d, nx, ny, nz, c + mk(0, -1.0f, 0));
+  const float ym = sample_vox<T>(d, nx, ny, nz, c + mk(0, +1.0f, 0));
+  const float zp = sample_vox<T>(d, nx, ny, nz, c + mk(0, 0, +1.0f));
+  const float zm = sample_vox<T>(d, nx, ny, nz, c + mk(0, 0, -1.0f));
+  return mk(xm - xp, yp - ym, zm - zp) / 2.0f;
+}
+
+// lighting.glsl:33-42 with the model-space eye
+__device__ __forceinline__ V3 lighting_eye(V3 eye, V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {
+  const V3 view = normalize(eye - pos);
+  const V3 R = normalize(reflect(view, n));
+  const float dd = fmaxf(fabsf(dot(n, L)), 0.0f);
+  const float ss = pow8(fmaxf(dot(R, L), 0.0f));
+  const V3 c = amb + dif * dd + spc * ss;
+  return mk(clamp01(c.x), clamp01(c.y), clamp01(c.z));
+}
+
+__device__ __forceinline__ V3 xf_point(V3 p, const float* M) {
+  return mk(p.x * M[0] + p.y * M[4] + p.z * M[8] + 1.0f * M[12], p.x * M[1] + p.y * M[5] + p.z * M[9] + 1.0f * M[13],
+            p.x * M[2] + p.y * M[6] + p.z * M[10] + 1.0f * M[14]);
+}
+__device__ __forceinline__ V3 xf_dir(V3 p, const float* M) {
+  return mk(p.x * M[0] + p.y * M[4] + p.z * M[8] + 0.0f * M[12], p.x * M[1] + p.y * M[5] + p.z * M[9] + 0.0f * M[13],
+            p.x * M[2] + p.y * M[6] + p.z * M[10] + 0.0f * M[14]);
+}
+
+__device__ __forceinline__ uint32_t brick_id(const LeapArgs& A, uint32_t x, uint32_t y, uint32_t z, uint32_t l) {
+  return A.lodoff[l] + x + y * A.ilayout[l][0] + z * A.ilayout[l][0] * A.ilayout[l][1];
+}
+
+// GLVolumePool.cpp:653-656, the floor of the logarithm read from the exponent
+__device__ __forceinline__ uint32_t compute_lod(const LeapArgs& A, float depth) {
+  const float x = A.lod_factor * (-depth) / A.l0err;
+  const uint32_t u = __float_as_uint(x);
+  const int e = (int)((u >> 23) & 0xffu) - 127;
+  const uint32_t lod = (x >= 1.0f && e > 0) ? (uint32_t)e : 0u;
+  return min((uint32_t)(A.nlod - 1), lod);
+}
+
+template <int MODE, bool LIT, typename T>
+__global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t tile = blockIdx.x * 4 + wave;
+  const int tx = (int)(tile % A.tiles_x), ty = (int)(tile / A.tiles_x);
+  const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+  if (px >= A.W || py >= A.H) return;   // no wave-level operations below
+  const size_t pix = (size_t)py * (size_t)A.W + (size_t)px;
+  const ShadeConst& S = A.S;
+
+  // pixel ray through the near plane
+  const float* ip = A.inv_proj;
+  const float n0 = ((float)px + 0.5f) * 2.0f / (float)A.W - 1.0f;
+  const float n1 = ((float)py + 0.5f) * 2.0f / (float)A.H - 1.0f;
+  float e[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) e[c] = n0 * ip[c] + n1 * ip[4 + c] + -1.0f * ip[8 + c] + 1.0f * ip[12 + c];
+  const V3 d = mk(e[0] / e[3], e[1] / e[3], e[2] / e[3]);
+  const V3 o_n = xf_point(mk(0, 0, 0), A.emm), d_n = xf_dir(d, A.emm);
+  float tn = -INFINITY, tfar = INFINITY;
+  bool hit = true;
+  {
+    const float on[3] = {o_n.x, o_n.y, o_n.z}, dn[3] = {d_n.x, d_n.y, d_n.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (dn[a] == 0.0f) {
+        if (on[a] < 0.0f || on[a] > 1.0f) hit = false;
+        continue;
+      }
+      const float inv = 1.0f / dn[a];
+      const float t0 = (0.0f - on[a]) * inv, t1 = (1.0f - on[a]) * inv;
+      tn = fmaxf(tn, fminf(t0, t1));
+      tfar = fminf(tfar, fmaxf(t0, t1));
+    }
+  }
+  // ray start: the box front face (or the near plane) in the first pass, the
+  // resume state afterwards
+  V3 entryN;
+  float entryDepth;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  bool live;
+  if (A.first) {
+    const float te = fmaxf(tn, 1.0f);
+    live = hit && tfar > tn && tfar > te;
+    const V3 pe = d * te;
+    entryN = xf_point(pe, A.emm);
+    entryDepth = pe.z;
+  } else {
+    const float4 rp = A.resume[2 * pix], rc = A.resume[2 * pix + 1];
+    entryN = mk(rp.x, rp.y, rp.z);
+    entryDepth = rp.w;
+    live = rp.w != 1000.0f;
+    acc[0] = rc.x; acc[1] = rc.y; acc[2] = rc.z; acc[3] = rc.w;
+  }
+  float4 rpos = make_float4(entryN.x, entryN.y, entryN.z, entryDepth);
+  float4 rcol = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  uint32_t ns = 0;
+  bool optimal = true;
+  if (live) {
+    const V3 pe = d * tfar;
+    const V3 exitN = xf_point(pe, A.emm);
+    const float exitDepth = pe.z;
+    const V3 direction = exitN - entryN;
+    const float rayLength = length(direction);
+    const V3 vsz = mk(A.vsz[0], A.vsz[1], A.vsz[2]);
+    const V3 dv = normalize(direction * vsz);
+    const V3 step = dv / (2.0f * A.srm);
+    const float stepSize = length(step);
+    const float voxelSize = 0.125f / 2000.0f;
+    const float oc = 1.0f / A.srm;
+    const V3 dsc = mk(S.dscale[0], S.dscale[1], S.dscale[2]);
+    const V3 amb = mk(S.amb[0], S.amb[1], S.amb[2]), dif = mk(S.dif[0], S.dif[1], S.dif[2]);
+    const V3 spc = mk(S.spc[0], S.spc[1], S.spc[2]);
+    const V3 meye = mk(A.meye[0], A.meye[1], A.meye[2]), mlight = mk(A.mlight[0], A.mlight[1], A.mlight[2]);
+    const V3 inner = mk(A.inner[0], A.inner[1], A.inner[2]);
+    const float ov = (float)A.overlap;
+    float tt = 0.0f;
+    V3 cur = entryN;
+    uint32_t last[4] = {0u, 0u, 0u, 9999u};
+    bool done = false;
+    if (rayLength > voxelSize) {
+      for (int j = 0; j < 100 && !done; ++j) {
+        const float depth = entryDepth * (1.0f - tt) + exitDepth * tt;
+        uint32_t lod = compute_lod(A, depth);
+        const V3 ne = mk(fminf(fmaxf(cur.x, 0.0f), 1.0f), fminf(fmaxf(cur.y, 0.0f), 1.0f),
+                         fminf(fmaxf(cur.z, 0.0f), 1.0f));
+        bool found_req = true;
+        uint32_t bc[4];
+        auto coords = [&](uint32_t l) {
+          bc[0] = (uint32_t)(ne.x * A.layout[l][0]);
+          bc[1] = (uint32_t)(ne.y * A.layout[l][1]);
+          bc[2] = (uint32_t)(ne.z * A.layout[l][2]);
+          bc[3] = l;
+        };
+        coords(lod);
+        uint32_t info = A.meta[brick_id(A, bc[0], bc[1], bc[2], lod)];
+        if (info == BI_MISSING) {
+          A.missing[brick_id(A, bc[0], bc[1], bc[2], lod)] = 1u;   // ReportMissingBrick
+          found_req = false;
+          do {
+            ++lod;
+            coords(lod);
+            info = A.meta[brick_id(A, bc[0], bc[1], bc[2], lod)];
+          } while (info == BI_MISSING && lod + 1 < (uint32_t)A.nlod);
+        }
+        const bool bEmpty = info <= BI_EMPTY;
+        if (bEmpty) {
+          for (uint32_t lr = lod + 1; lr < (uint32_t)(A.nlod - 1); ++lr) {
+            const uint32_t s0 = bc[0], s1 = bc[1], s2 = bc[2], s3 = bc[3];
+            coords(lr);
+            const uint32_t li = A.meta[brick_id(A, bc[0], bc[1], bc[2], lr)];
+            if (li == BI_CHILD_EMPTY) {
+              info = li;
+              lod = lr;
+            } else {
+              bc[0] = s0; bc[1] = s1; bc[2] = s2; bc[3] = s3;
+              break;
+            }
+          }
+        }
+        const V3 c0 = mk((float)bc[0] / A.layout[lod][0], (float)bc[1] / A.layout[lod][1],
+                         (float)bc[2] / A.layout[lod][2]);
+        const V3 c1 = mk((float)(bc[0] + 1) / A.layout[lod][0], (float)(bc[1] + 1) / A.layout[lod][1],
+                         (float)(bc[2] + 1) / A.layout[lod][2]);
+        V3 bexit;
+        {
+          const V3 div = mk(1.0f / direction.x, 1.0f / direction.y, 1.0f / direction.z);
+          const float tix = ((div.x >= 0.0f ? c1.x : c0.x) - ne.x) * div.x;
+          const float tiy = ((div.y >= 0.0f ? c1.y : c0.y) - ne.y) * div.y;
+          const float tiz = ((div.z >= 0.0f ? c1.z : c0.z) - ne.z) * div.z;
+          bexit = ne + direction * fminf(fminf(tix, tiy), tiz);
+        }
+        if (!found_req && optimal) {
+          optimal = false;
+          rpos = make_float4(cur.x, cur.y, cur.z, depth);
+          rcol = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        if (!bEmpty && (last[0] != bc[0] || last[1] != bc[1] || last[2] != bc[2] || last[3] != bc[3])) {
+          const GatherBrick& gb = A.bricks[info - BI_FLAG_COUNT];
+          const T* __restrict__ bd = reinterpret_cast<const T*>(S.pool) + gb.offset;
+          const int nx = (int)gb.n[0], ny = (int)gb.n[1], nz = (int)gb.n[2];
+          const V3 sc = mk(inner.x / (c1.x - c0.x), inner.y / (c1.y - c0.y), inner.z / (c1.z - c0.z));
+          const V3 tr = mk(ov, ov, ov) - c0 * sc;
+          const V3 pin = ne * sc + tr, pout = bexit * sc + tr;
+          int steps = (int)ceilf(length(pout - pin) / stepSize);
+          steps = min(steps, (int)ceilf(length((exitN - cur) * sc) / stepSize));
+          V3 b = pin;
+          for (int i = 0; i < steps; ++i) {
+            ++ns;
+            const float data = sample_vox<T>(bd, nx, ny, nz, b);
+            float c[4];
+            if (MODE == 0) {
+              const float cc = fminf(fmaxf(data * S.tf_scale, 0.0f), 1.0f);
+              const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
+              const float4 tv = S.tf1[ti];
+              c[0] = tv.x; c[1] = tv.y; c[2] = tv.z; c[3] = tv.w;
+              if (LIT) {
+                V3 g = gradient_vox<T>(bd, nx, ny, nz, b) * dsc;
+                const float l = length(g);
+                if (l > 0.0f) g = mk(g.x / l, g.y / l, g.z / l);
+                const V3 lc = lighting_eye(meye, (b - tr) / sc, g, amb, mk(c[0], c[1], c[2]) * dif, spc, mlight);
+                c[0] = lc.x; c[1] = lc.y; c[2] = lc.z;
+              }
+            } else {
+              const V3 g = gradient_vox<T>(bd, nx, ny, nz, b);
+              const float gm = length(g);
+              const float sx = fminf(fmaxf(data * S.tf_scale, 0.0f), 1.0f);
+              const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
+              const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
+              const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
+              const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];
+              c[0] = tv.x; c[1] = tv.y; c[2] = tv.z; c[3] = tv.w;
+              if (LIT) {
+                const V3 nrm = dsc * (gm > 0.0f ? g / gm : g);
+                const V3 lc = lighting_eye(meye, (b - tr) / sc, nrm, amb, mk(c[0], c[1], c[2]) * dif, spc, mlight);
+                c[0] = lc.x; c[1] = lc.y; c[2] = lc.z;
+              }
+            }
+            c[3] = 1.0f - (oc == 1.0f ? 1.0f - c[3] : powf(1.0f - c[3], oc));
+            // UnderCompositing (Compositing.glsl:33-38)
+            acc[0] += c[0] * (1.0f - acc[3]) * c[3];
+            acc[1] += c[1] * (1.0f - acc[3]) * c[3];
+            acc[2] += c[2] * (1.0f - acc[3]) * c[3];
+            acc[3] += c[3] * (1.0f - acc[3]);
+            if (acc[3] > 0.99f) { done = true; break; }
+            b = b + step;
+          }
+          if (!done) cur = (b - tr) / sc;
+        } else {
+          cur = bexit + direction * voxelSize / rayLength;
+        }
+        if (done) break;
+        last[0] = bc[0]; last[1] = bc[1]; last[2] = bc[2]; last[3] = bc[3];
+        tt = length(entryN - bexit) / rayLength;
+        if (tt > 0.9999f) break;
+      }
+    }
+    // TerminateRay
+    if (optimal) {
+      rpos.w = 1000.0f;
+      rcol = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    }
+  } else {
+    rpos.w = 1000.0f;
+  }
+  reinterpret_cast<float4*>(A.out)[pix] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  A.resume[2 * pix] = rpos;
+  A.resume[2 * pix + 1] = rcol;
+  if (A.counter && ns) atomicAdd(A.counter, (unsigned long long)ns);
+}
+
+template <int M, bool L, typename T>
+void launch_leap_one(const LeapArgs& a, uint32_t blocks, hipStream_t st) {
+  hipLaunchKernelGGL((leap_kernel<M, L, T>), dim3(blocks), dim3(256), 0, st, a);
+}
+
+}  // namespace
+
+hipError_t launch_leap(const LeapArgs& a, int bpv, hipStream_t st) {
+  const uint32_t tiles_y = ((uint32_t)a.H + 7u) / 8u;
+  const uint32_t tiles = a.tiles_x * tiles_y;
+  const uint32_t blocks = (tiles + 3u) / 4u;
+  if (blocks == 0) return hipSuccess;
+  const bool u16 = bpv == 2, lit = a.lighting != 0;
+  if (a.mode == 0) {
+    if (lit) { if (u16) launch_leap_one<0, true, uint16_t>(a, blocks, st); else launch_leap_one<0, true, uint8_t>(a, blocks, st); }
+    else { if (u16) launch_leap_one<0, false, uint16_t>(a, blocks, st); else launch_leap_one<0, false, uint8_t>(a, blocks, st); }
+  } else if (a.mode == 1) {
+    if (lit) { if (u16) launch_leap_one<1, true, uint16_t>(a, blocks, st); else launch_leap_one<1, true, uint8_t>(a, blocks, st); }
+    else { if (u16) launch_leap_one<1, false, uint16_t>(a, blocks, st); else launch_leap_one<1, false, uint8_t>(a, blocks, st); }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tvk

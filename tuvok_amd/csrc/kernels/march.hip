@@ -25,55 +25,9 @@
 #include <hip/hip_runtime.h>
 #include "../tvk_types.hpp"
 #include "stepnet.hpp"
+#include "common.hpp"
 
 namespace tvk {
-
-struct V3 {
-  float x, y, z;
-};
-__device__ __forceinline__ V3 mk(float a, float b, float c) { V3 r; r.x = a; r.y = b; r.z = c; return r; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ V3 operator/(V3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
-// GLSL normalize(): one correctly rounded reciprocal of the length, then three products
-__device__ __forceinline__ V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
-__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
-__device__ __forceinline__ float pow8(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x4; }
-// The entry point goes through an RGBA16F target: the fp32 value is rounded to
-// fp16.  The empty asm keeps the product that feeds it an fp32 value (otherwise
-// the mixed-precision FMA rounds the exact product to fp16 in one step).
-__device__ __forceinline__ float fp16_round(float x) {
-  asm volatile("" : "+v"(x));
-  return (float)(_Float16)x;
-}
-
-// value of a stored voxel as the unorm texture returns it
-template <typename T> struct Unorm;
-template <> struct Unorm<uint8_t> { static __device__ __forceinline__ float inv() { return 1.0f / 255.0f; } };
-template <> struct Unorm<uint16_t> { static __device__ __forceinline__ float inv() { return 1.0f / 65535.0f; } };
-
-// Element offset parts of the blocked brick layout (see tvk_types.hpp): the
-// offset of voxel (x,y,z) is ox(x) + oy(y) + oz(z).
-template <typename T> struct Blk { static constexpr int LX = sizeof(T) == 1 ? 3 : 2; };
-template <typename T>
-__device__ __forceinline__ uint32_t ox(int x) {
-  constexpr int L = Blk<T>::LX;
-  return ((uint32_t)(x >> L) << (L + 4)) + (uint32_t)(x & ((1 << L) - 1));
-}
-template <typename T>
-__device__ __forceinline__ uint32_t oy(int y, uint32_t nbx) {
-  constexpr int L = Blk<T>::LX;
-  return (((uint32_t)(y >> 2) * nbx) << (L + 4)) + ((uint32_t)(y & 3) << L);
-}
-template <typename T>
-__device__ __forceinline__ uint32_t oz(int z, uint32_t nbxy) {
-  constexpr int L = Blk<T>::LX;
-  return (((uint32_t)(z >> 2) * nbxy) << (L + 4)) + ((uint32_t)(z & 3) << (L + 2));
-}
 
 // texture3D with GL_LINEAR and GL_CLAMP (border colour 0) or GL_CLAMP_TO_EDGE
 template <typename T>
@@ -251,7 +205,6 @@ struct CrossSampler {
   }
 };
 
-__device__ __forceinline__ V3 reflect(V3 I, V3 N) { return I - N * (2.0f * dot(N, I)); }
 
 // lighting.glsl:33-50 (eye at the origin)
 __device__ __forceinline__ V3 lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {

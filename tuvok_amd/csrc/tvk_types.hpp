@@ -152,4 +152,26 @@ struct BatchArgs {
   uint32_t tile_w, tile_h;      // wave tile shape, tile_w * tile_h == 64
 };
 
+// GridLeaper single-pass traversal (one pass of the ray-guided march).
+constexpr int LEAP_MAX_LOD = 16;
+struct LeapArgs {
+  ShadeConst S;                // pool, TF tables, scales, lights
+  const uint32_t* meta;        // per brick ID: 0 missing, 1 child empty, 2 empty, 3 + TOC index resident
+  const GatherBrick* bricks;   // per TOC brick: stored size and pool offset
+  uint32_t* missing;           // per brick ID: set to 1 when reported missing
+  float4* resume;              // 2 per pixel: (normalised position, eye z; w = 1000 finished), colour
+  float* out;                  // RGBA32F accumulated colour
+  unsigned long long* counter; // samples (or nullptr)
+  float emm[16];               // eye -> normalised volume coordinates
+  float inv_proj[16];
+  float layout[LEAP_MAX_LOD][3];     // float brick layout per level
+  uint32_t ilayout[LEAP_MAX_LOD][3]; // integer brick layout per level
+  uint32_t lodoff[LEAP_MAX_LOD];     // first brick ID of each level
+  float inner[3];              // inner (core) brick size
+  float vsz[3];                // LOD-0 volume size in voxels
+  float meye[3], mlight[3];    // model-space eye and light direction
+  float lod_factor, l0err, srm;
+  int nlod, W, H, first, lighting, mode, overlap;
+  uint32_t tiles_x;
+};
 }  // namespace tvk

@@ -301,6 +301,50 @@ class HIPRaycaster:
                                            C.byref(self.last_stats)))
         return out
 
+    # -- ray-guided traversal (GLGridLeaper) ----------------------------------------
+    def LeapSetResident(self, flags=None):
+        """Residency per TOC brick (None: every brick resident)."""
+        if flags is None:
+            self._chk(self._L.tvk_leap_set_resident(self._h, None, 0))
+            return
+        f = np.ascontiguousarray(flags, np.uint8)
+        self._chk(self._L.tvk_leap_set_resident(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8)), f.size))
+
+    def LeapRender(self, first=True):
+        """One traversal pass; returns (RGBA32F frame, LeapStats)."""
+        out = np.empty((self._h_px, self._w, 4), np.float32)
+        st = _abi.LeapStats()
+        self._chk(self._L.tvk_leap_render(self._h, int(bool(first)), _fp(out), C.byref(st)))
+        return out, st
+
+    def LeapMissing(self):
+        n = self.GetDatasetDesc().brick_count
+        f = np.zeros(n, np.uint8)
+        cnt = C.c_uint64()
+        self._chk(self._L.tvk_leap_missing(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8)), n, C.byref(cnt)))
+        return f
+
+    def LeapPageIn(self, max_bricks=(1 << 62)):
+        k = C.c_uint64()
+        self._chk(self._L.tvk_leap_page_in(self._h, int(max_bricks), C.byref(k)))
+        return k.value
+
+    def LeapResumeState(self):
+        out = np.empty((self._h_px, self._w, 8), np.float32)
+        self._chk(self._L.tvk_leap_resume_state(self._h, _fp(out), out.size))
+        return out
+
+    def LeapConverge(self, max_passes=32, page_per_pass=(1 << 62)):
+        """Passes until no brick is reported missing (GLGridLeaper's Continue3DDraw
+        loop), paging the reported bricks in between.  Returns (frame, [stats])."""
+        img, st = self.LeapRender(first=True)
+        stats = [st]
+        while st.missing and len(stats) < max_passes:
+            self.LeapPageIn(page_per_pass)
+            img, st = self.LeapRender(first=False)
+            stats.append(st)
+        return img, stats
+
     # -- step proposal ------------------------------------------------------------
     def SetStepProposal(self, net, margin=0.0, skip_threshold=1.0, max_log2=3, fallback_bound=0.0):
         """Attach a StepNet (tuvok_amd.stepnet) to march with proposed step
