@@ -60,6 +60,47 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
                        f"({W}x{hi - lo} rays, {ns_total} samples, {dt_total:.1f} s, {threads} threads)")
 
 
+def learned_leg(ren, views, cfg):
+    """Learned step vs fixed step on the same volume, TF and views, at the
+    0.25-voxel reference step (srm 2): segment samples from 48 random views at
+    480x270, training on the device, calibration on 3 other views, then the 16
+    views timed with and without the proposal and compared pixel by pixel."""
+    from tuvok_amd.stepnet import calibrate, train_proposal
+    from tuvok_amd.renderer import rotation_y
+    rng = np.random.default_rng(123)
+    W, H = cfg["width"], cfg["height"]
+    ren.SetSampleRateModifier(2.0)
+    t0 = time.perf_counter()
+    ren.Resize(480, 270)
+    net, info = train_proposal(ren, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 48)], epochs=20)
+    ren.Resize(W, H)
+    train_s = time.perf_counter() - t0
+    print(f"[bench] learned: {info} in {train_s:.1f} s", file=sys.stderr, flush=True)
+    prop, tried = calibrate(ren, net, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 3)])
+    if prop is None:
+        return dict(error="no proposal setting met 0.9/255 on the calibration views", train=info)
+    ren.SetStepProposal(None)
+    fixed = ren.PaintViews(views)
+    t_fixed = []
+    for _ in range(3):
+        ren.PaintViews(views, download=False)
+        t_fixed.append(ren.last_stats.ms)
+    ren.SetStepProposal(net, **prop)
+    learned = ren.PaintViews(views)
+    t_learned = []
+    for _ in range(3):
+        ren.PaintViews(views, download=False)
+        t_learned.append(ren.last_stats.ms)
+    ren.SetStepProposal(None)
+    ren.SetSampleRateModifier(1.0)
+    d = np.abs(learned - fixed)
+    fms, lms = float(np.median(t_fixed)), float(np.median(t_learned))
+    return dict(reference_step_voxels=0.25, views=len(views), fixed_fps=round(len(views) / (fms / 1e3), 2),
+                learned_fps=round(len(views) / (lms / 1e3), 2), speedup=round(fms / lms, 3),
+                max_err=float(d.max()), mean_err=float(d.mean()), max_err_x255=round(float(d.max()) * 255, 3),
+                proposal=prop, train_s=round(train_s, 1), train=info)
+
+
 PROFILE = os.path.join(ROOT, "profiles", "march_fbm1024_lit2dtf_1080p_x16.md")
 
 
@@ -87,6 +128,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-step leg")
     ap.add_argument("--size", type=int, default=1024, help="volume edge (default: the 1024^3 workload)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -142,6 +184,13 @@ def main():
     value = n_ref / (ms_step / 1e3) / 1e9
 
     prof = profile_figures(k_ms)
+    learned = None
+    if not args.no_learned:
+        try:
+            learned = learned_leg(ren, views, cfg)
+        except Exception as e:   # the fixed-step line stays valid; report what failed
+            learned = dict(error=f"{type(e).__name__}: {e}")
+        print(f"[bench] learned: {learned}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         vol = ren.DownloadVolume()
@@ -178,6 +227,7 @@ def main():
                      "valu_busy": prof.get("valu_busy") if prof else None,
                      "lane_util": prof.get("lane_util") if prof else None},
         "cpu_baseline": cpu,
+        "learned_step": learned,
     }
     print(json.dumps(out))
 

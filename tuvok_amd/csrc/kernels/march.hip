@@ -23,6 +23,7 @@
 // ones of the host planner and of the reference restatement, in the same
 // order, so step counts, sample positions and TF indices are reproducible.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "../tvk_types.hpp"
 #include "stepnet.hpp"
 #include "common.hpp"
@@ -102,7 +103,10 @@ __device__ __forceinline__ float tri(float c000, float c100, float c010, float c
   return c0 * (1.0f - f2) + c1 * f2;
 }
 
-template <typename T>
+// FAST: every tap of the 4x4x4 neighbourhood lies inside the brick for every
+// lane of the wave (decided by a ballot before the sample), so no coordinate
+// clamps or border selects are needed; the values are the same.
+template <typename T, bool FAST = false>
 struct CrossSampler {
   const T* __restrict__ d;
   int n[3];
@@ -118,6 +122,7 @@ struct CrossSampler {
   __device__ __forceinline__ float V(int i, int j, int k) const {   // i,j,k in 0..3 -> index ic-1+i ...
     // offsets are clamped into the brick: unconditional load, then the border select
     const float v = (float)d[ofz[k] + ofy[j] + ofx[i]] * Unorm<T>::inv();
+    if (FAST) return v;
     return (okx[i] && oky[j] && okz[k]) ? v : 0.0f;
   }
 
@@ -134,12 +139,18 @@ struct CrossSampler {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int x = ic[0] - 1 + k, y = ic[1] - 1 + k, z = ic[2] - 1 + k;
-      okx[k] = edge || (x >= 0 && x < nx);
-      oky[k] = edge || (y >= 0 && y < ny);
-      okz[k] = edge || (z >= 0 && z < nz);
-      ofx[k] = ox<T>(min(max(x, 0), nx - 1));
-      ofy[k] = oy<T>(min(max(y, 0), ny - 1), nbx);
-      ofz[k] = oz<T>(min(max(z, 0), nz - 1), nbxy);
+      if (FAST) {
+        ofx[k] = ox<T>(x);
+        ofy[k] = oy<T>(y, nbx);
+        ofz[k] = oz<T>(z, nbxy);
+      } else {
+        okx[k] = edge || (x >= 0 && x < nx);
+        oky[k] = edge || (y >= 0 && y < ny);
+        okz[k] = edge || (z >= 0 && z < nz);
+        ofx[k] = ox<T>(min(max(x, 0), nx - 1));
+        ofy[k] = oy<T>(min(max(y, 0), ny - 1), nbx);
+        ofz[k] = oz<T>(min(max(z, 0), nz - 1), nbxy);
+      }
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -398,13 +409,13 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
       const float ssc = sscale * mult;
       c0 = 0.f; c1 = 0.f; c2 = 0.f; c3 = 0.f;
       V3 pos = entryTex;
+      V3 pos_s = entryTex;   // the position shade() samples
       V3 cur = entry;
       const V3 rayInc = dir / L * rs;
-      for (int i = 0; i < steps_m; ++i) {
-        ++ns;
-        float s0, s1, s2, s3;
-        CrossSampler<T> cs;
-        const float val = cs.value(bd, nx, ny, nz, edge, pos);
+      // TF, gradient and shading of the sample at pos (the cross sampler's two forms)
+      auto shade = [&](auto fast_c, float& s0, float& s1, float& s2, float& s3) {
+        CrossSampler<T, decltype(fast_c)::value> cs;
+        const float val = cs.value(bd, nx, ny, nz, edge, pos_s);
         if (MODE == 0) {
           const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
           const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
@@ -443,6 +454,25 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             }
           }
         }
+      };
+      for (int i = 0; i < steps_m; ++i) {
+        ++ns;
+        float s0, s1, s2, s3;
+        // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
+        bool inner = true;
+        {
+          const float pc[3] = {pos.x, pos.y, pos.z};
+          const int nn3[3] = {nx, ny, nz};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
+            const int ia = (int)floorf(u);
+            inner = inner && ia >= 1 && ia + 2 < nn3[a];
+          }
+        }
+        pos_s = pos;
+        if (__ballot(!inner) == 0) shade(std::true_type{}, s0, s1, s2, s3);
+        else shade(std::false_type{}, s0, s1, s2, s3);
         // UnderCompositing (Compositing.glsl:33-38)
         const float wgt = (1.0f - c3) * s3;
         c0 += s0 * wgt;

@@ -117,3 +117,41 @@ def decide(y, margin, skip_threshold, max_log2=3):
     k = np.clip(np.floor(3.0 * y[:, 0] + 0.5 - margin), 0, max_log2).astype(np.int64)
     p = 1.0 / (1.0 + np.exp(-y[:, 1].astype(np.float64)))
     return k, p > skip_threshold
+
+
+def train_proposal(ren, rotations, cap=2_000_000, keep=1.0, tau=0.25 / 255, tau_skip=None, epochs=20, batch=4096,
+                   lr=2e-3, seed=0, net=None, device=0):
+    """Segment samples of the renderer's fixed-step march over the given views,
+    their labels, and a network trained on them with the device kernels."""
+    rec = ren.GenerateSegments(rotations, cap, keep, seed)
+    if len(rec) == 0:
+        raise RuntimeError("no segment samples (empty views?)")
+    tgt = labels(rec, tau, tau / 2 if tau_skip is None else tau_skip)
+    net = net or StepNet(device, seed=seed)
+    net.set_data(split_samples(rec)[0], tgt)
+    losses = net.train(epochs=epochs, batch=batch, lr=lr, seed=seed)
+    info = dict(segments=int(len(rec)), mean_safe_log2=float((tgt[:, 0] * 3).mean()),
+                skip_fraction=float(tgt[:, 1].mean()), loss_first=float(losses[0]), loss_last=float(losses[-1]))
+    return net, info
+
+
+def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0),
+              skips=(0.99, 0.9), max_log2=3):
+    """The most aggressive proposal setting (least device time) whose frames stay
+    within `bound` of the fixed-step frames on the given views.  Returns
+    (setting or None, [(setting, max error, ms)])."""
+    ren.SetStepProposal(None)
+    refs = ren.PaintViews(rotations)
+    tried, best = [], None
+    for m in margins:
+        for s in skips:
+            prop = dict(margin=m, skip_threshold=s, max_log2=max_log2)
+            ren.SetStepProposal(net, **prop)
+            imgs = ren.PaintViews(rotations)
+            err = float(np.abs(imgs - refs).max())
+            ms = float(ren.last_stats.ms)
+            tried.append((prop, err, ms))
+            if err <= bound and (best is None or ms < best[1]):
+                best = (prop, ms)
+    ren.SetStepProposal(None)
+    return (best[0] if best else None), tried
