@@ -193,3 +193,31 @@ def test_batch48_full_size():
         assert np.array_equal(batch.Frame(i), rens[i].Paint()), i
     print(f"batch48: {px} pixels", flush=True)
     batch.close()
+
+
+def test_lod64g_batch_full_size():
+    """4096x4096x2048 uint16 (64 GiB) with a 5-level pyramid in the HBM pool, 32
+    requests mixing 3840x2160 and 1920x1080 at different distances: each request
+    takes its screen-space-error level, the batch renders them in one frame and
+    two of them equal their single renders."""
+    print("lod64g: generating 64 GiB volume + pyramid", flush=True)
+    reqs = workloads.lod_requests()
+    batch, rens, owner = workloads.setup_lod_batch(reqs)
+    d = owner.GetDatasetDesc()
+    print(f"lod64g: pool {d.pool_bytes / 2**30:.1f} GiB, {d.brick_count} bricks, {d.lod_count} levels", flush=True)
+    assert d.lod_count == 5
+    lods = [r.CurrentLOD() for r in rens]
+    print(f"lod64g: levels {sorted(set(lods))}", flush=True)
+    assert len(set(lods)) >= 2 and max(lods) <= 4
+    st = batch.Render()
+    st = batch.Render()
+    px = sum(w * h for (_, _, w, h) in reqs)
+    print(f"lod64g: {px / 1e6:.1f} Mpixels in {st.ms:.1f} ms ({px / st.ms / 1e3:.1f} Mpixels/s)", flush=True)
+    for i in (0, 1, 30, 31):
+        f = batch.Frame(i)
+        a = f[..., 3]
+        assert np.isfinite(f).all() and a.min() >= 0 and a.max() <= 1 + 1e-6
+        assert (f[..., :3] <= a[..., None] + 1e-5).all()
+        if i in (0, 31):
+            assert np.array_equal(f, rens[i].Paint()), i
+    batch.close()

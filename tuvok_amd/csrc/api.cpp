@@ -630,8 +630,9 @@ namespace tvk {
 
 int frame_lod(tvk_ctx* c, const ViewState& V, int* lod) {
   int l = c->P.lod;
-  if (l < 0) l = min_lod_for_view(c->L, V, c->P.screen_space_error);
   const int kept = (int)(c->L.toc.back().coords[3] + 1);
+  // ComputeMinLODForCurrentView, limited to the levels the pool holds
+  if (l < 0) l = std::min(min_lod_for_view(c->L, V, c->P.screen_space_error), kept - 1);
   if (l >= kept) return fail(c, TVK_ERR_RANGE, "LOD " + std::to_string(l) + " is not in the brick pool");
   *lod = l;
   return TVK_OK;

@@ -33,6 +33,55 @@ SDF_ISO_2048 = dict(
     eye=(0.0, 0.0, 2.0), views=1,
 )
 
+LOD_64G = dict(
+    name="fbm_shells4096x4096x2048_u16_5lod_2dtf_lit_32req",
+    gen=_abi.GEN_FBM_SHELLS, seed=3, dtype=np.uint16, dims=(4096, 4096, 2048), brick=(258, 258, 258), overlap=1,
+    mode=_abi.RM_2DTRANS, lighting=True, max_lods=5, fov=45.0,
+)
+
+
+def lod_requests(n=32, seed=9, near=2.0, far=8.0):
+    """Requests of the large-volume workload: (eye distance, rotation, width,
+    height); half 3840x2160 and half 1920x1080, distances log-uniform."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        w, h = (3840, 2160) if k % 2 == 0 else (1920, 1080)
+        out.append((float(np.exp(rng.uniform(np.log(near), np.log(far)))), rotation_y(rng.uniform(0, 2 * np.pi)),
+                    w, h))
+    return out
+
+
+def setup_lod_batch(requests, cfg=LOD_64G, device=0, ert_eps=2.0 ** -11, dims=None):
+    """The large volume (all kept levels in the HBM pool) shared by one renderer
+    per request, each with its own camera distance and screen-space-error level
+    choice, in one RenderBatch."""
+    from .batch import RenderBatch
+    owner = HIPRaycaster(device)
+    owner.GenerateVolume(cfg["gen"], cfg["seed"], cfg["dtype"], dims or cfg["dims"], cfg["brick"], cfg["overlap"],
+                         max_lods=cfg["max_lods"], keep_linear=True)
+    tf = tf2d_gaussian_lobes().GetByteArray()
+    g = owner.MaxGradientMagnitude()
+    batch = RenderBatch(device)
+    rens = []
+    for (dist, rot, w, h) in requests:
+        r = HIPRaycaster(device)
+        r.ShareDataset(owner)
+        r.Set2DTrans(tf)
+        r.SetRendermode(cfg["mode"])
+        r.SetUseLighting(cfg["lighting"])
+        r.SetGradientScale(1.0 / g if g > 0 else 1.0)
+        r.SetLOD(-1)
+        r.SetEarlyRayTermination(ert_eps)
+        r.SetViewParameters(cfg["fov"], 0.01, 1000.0)
+        r.SetViewPosition((0.0, 0.0, dist))
+        r.Resize(w, h)
+        r.SetRotation(rot)
+        batch.Add(r, rot, w, h)
+        rens.append(r)
+    return batch, rens, owner
+
+
 # the three volumes of the batch: (generator, seed, edge)
 BATCH_VOLUMES = [(_abi.GEN_BLOBS, 10, 256), (_abi.GEN_ELLIPSOID_SDF, 11, 1024), (_abi.GEN_ELLIPSOID_SDF, 12, 2048)]
 
