@@ -60,19 +60,20 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
                        f"({W}x{hi - lo} rays, {ns_total} samples, {dt_total:.1f} s, {threads} threads)")
 
 
-def learned_leg(ren, views, cfg):
+def learned_leg(ren, views, cfg, train_views=48, seed=123):
     """Learned step vs fixed step on the same volume, TF and views, at the
-    0.25-voxel reference step (srm 2): segment samples from 48 random views at
-    480x270, training on the device, calibration on 3 other views, then the 16
-    views timed with and without the proposal and compared pixel by pixel."""
+    0.25-voxel reference step (srm 2): segment samples from `train_views`
+    random views at 480x270, training on the device, calibration on 3 other
+    views, then `views` timed with and without the proposal and compared pixel
+    by pixel."""
     from tuvok_amd.stepnet import calibrate, train_proposal
     from tuvok_amd.renderer import rotation_y
-    rng = np.random.default_rng(123)
+    rng = np.random.default_rng(seed)
     W, H = cfg["width"], cfg["height"]
     ren.SetSampleRateModifier(2.0)
     t0 = time.perf_counter()
     ren.Resize(480, 270)
-    net, info = train_proposal(ren, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 48)], epochs=20)
+    net, info = train_proposal(ren, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, train_views)], epochs=20)
     ren.Resize(W, H)
     train_s = time.perf_counter() - t0
     print(f"[bench] learned: {info} in {train_s:.1f} s", file=sys.stderr, flush=True)
@@ -128,7 +129,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-step leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-step legs")
+    ap.add_argument("--no-phantom", action="store_true", help="skip the learned-step leg on the 2048^3 phantom")
     ap.add_argument("--size", type=int, default=1024, help="volume edge (default: the 1024^3 workload)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -196,6 +198,21 @@ def main():
         vol = ren.DownloadVolume()
         thr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         cpu = cpu_baseline(ren, vol, views, thr)
+        del vol
+    ren.close()
+    phantom = None
+    if not args.no_learned and not args.no_phantom:
+        try:
+            pc = workloads.PHANTOM_2048
+            t0 = time.perf_counter()
+            pren = workloads.setup(pc, ert_eps=ERT_EPS)
+            print(f"[bench] phantom ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            phantom = learned_leg(pren, workloads.orbit(4), pc, train_views=32, seed=321)
+            phantom["workload"] = pc["name"]
+            pren.close()
+        except Exception as e:
+            phantom = dict(error=f"{type(e).__name__}: {e}")
+        print(f"[bench] learned (phantom): {phantom}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "Gsamples/s (fixed-step-equivalent) + frames/s @1080p",
@@ -228,6 +245,7 @@ def main():
                      "lane_util": prof.get("lane_util") if prof else None},
         "cpu_baseline": cpu,
         "learned_step": learned,
+        "learned_step_phantom": phantom,
     }
     print(json.dumps(out))
 

@@ -139,3 +139,19 @@ def tf2d_gaussian_lobes(width=256, height=256, lobes=None):
     tf.color[..., :3] = rgb.astype(np.float32)
     tf.color[..., 3] = np.clip(a, 0, 1).astype(np.float32)
     return tf
+
+
+def tf1d_phantom(size=256, lo=0.08):
+    """1D TF for the nested-ellipsoid phantom (intensities 0-4000 plus noise):
+    transparent background below `lo` of the range, then faint, rising opacity
+    over the soft classes and denser opacity over the bright ones; colour goes
+    from dark red through orange to near white."""
+    tf = TransferFunction1D(size)
+    t = np.arange(size, dtype=np.float64) / (size - 1)
+    u = np.clip((t - lo) / (1.0 - lo), 0.0, 1.0)
+    alpha = np.where(t < lo, 0.0, 0.01 + 0.05 * u + 0.25 * u ** 4)
+    tf.color[:, 0] = (0.5 + 0.5 * u).astype(np.float32)
+    tf.color[:, 1] = (0.15 + 0.8 * u).astype(np.float32)
+    tf.color[:, 2] = (0.1 + 0.8 * u ** 2).astype(np.float32)
+    tf.color[:, 3] = alpha.astype(np.float32)
+    return tf

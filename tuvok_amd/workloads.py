@@ -4,14 +4,16 @@ BLOBS_256 is the CPU-runnable 1D-TF case (256^3 uint8 blobs, 64^3 cores + 1
 voxel overlap, 512x512, one view).  FBM_SHELLS_1024 is the lit 2D-TF case at
 scale (1024^3 uint16 fBm + shells, 128^3 cores + 1 voxel overlap, 1920x1080,
 16 views on an orbit).  SDF_ISO_2048 is the isosurface case (2048^3 uint8
-signed distance of 500 ellipsoids, iso 128, 4 bisection steps).  batch48()
+signed distance of 500 ellipsoids, iso 128, 4 bisection steps).  PHANTOM_2048
+is the second volume of the learned-step case (2048^3 uint16 phantom of 40
+nested ellipsoids, intensities 0-4000, Gaussian noise sigma 20).  batch48()
 lists the 48 uneven requests over three uint8 volumes rendered as one batch.
 """
 import numpy as np
 
 from . import _abi
 from .renderer import HIPRaycaster, rotation_y
-from .transfer import tf1d_blue_white_red, tf1d_random, tf2d_gaussian_lobes
+from .transfer import tf1d_blue_white_red, tf1d_phantom, tf1d_random, tf2d_gaussian_lobes
 
 BLOBS_256 = dict(
     name="blobs256_u8_1dtf_512x512",
@@ -31,6 +33,12 @@ SDF_ISO_2048 = dict(
     gen=_abi.GEN_ELLIPSOID_SDF, seed=2, dtype=np.uint8, dims=(2048, 2048, 2048), brick=(130, 130, 130), overlap=1,
     mode=_abi.RM_ISOSURFACE, lighting=True, isovalue=128.0, refine_steps=4, width=1920, height=1080, fov=45.0,
     eye=(0.0, 0.0, 2.0), views=1,
+)
+PHANTOM_2048 = dict(
+    name="phantom2048_u16_1dtf_lit_1080p",
+    gen=_abi.GEN_PHANTOM, seed=4, dtype=np.uint16, dims=(2048, 2048, 2048), brick=(130, 130, 130), overlap=1,
+    mode=_abi.RM_1DTRANS, lighting=True, width=1920, height=1080, fov=45.0, eye=(0.0, 0.0, 2.0), views=16,
+    tf=tf1d_phantom,
 )
 
 LOD_64G = dict(
@@ -114,7 +122,7 @@ def setup(cfg, device=0, keep_linear=False, max_lods=1, ert_eps=0.0, dims=None, 
     ren.GenerateVolume(cfg["gen"], cfg["seed"], cfg["dtype"], d, cfg["brick"], cfg["overlap"],
                        max_lods=max_lods, keep_linear=keep_linear)
     if cfg["mode"] == _abi.RM_1DTRANS:
-        ren.Set1DTrans(tf1d_blue_white_red().GetByteArray())
+        ren.Set1DTrans(cfg.get("tf", tf1d_blue_white_red)().GetByteArray())
     elif cfg["mode"] == _abi.RM_2DTRANS:
         ren.Set2DTrans(tf2d_gaussian_lobes().GetByteArray())
         if keep_linear:
