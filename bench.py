@@ -73,7 +73,8 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     ren.SetSampleRateModifier(2.0)
     t0 = time.perf_counter()
     ren.Resize(480, 270)
-    net, info = train_proposal(ren, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, train_views)], epochs=20)
+    net, info = train_proposal(ren, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, train_views)], keep=0.02,
+                               epochs=20)
     ren.Resize(W, H)
     train_s = time.perf_counter() - t0
     print(f"[bench] learned: {info} in {train_s:.1f} s", file=sys.stderr, flush=True)
@@ -86,18 +87,27 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     for _ in range(3):
         ren.PaintViews(views, download=False)
         t_fixed.append(ren.last_stats.ms)
-    ren.SetStepProposal(net, **prop)
-    learned = ren.PaintViews(views)
-    t_learned = []
-    for _ in range(3):
-        ren.PaintViews(views, download=False)
-        t_learned.append(ren.last_stats.ms)
+    def timed(prop):
+        ren.SetStepProposal(None if prop is None else net, **(prop or {}))
+        img = ren.PaintViews(views)
+        ts = []
+        for _ in range(3):
+            ren.PaintViews(views, download=False)
+            ts.append(ren.last_stats.ms)
+        return img, float(np.median(ts))
+
+    # the chunked march with the exact skip only (multiplier 1, no network
+    # decision): what the method gives without the network
+    exact, ems = timed(dict(margin=1e9, skip_threshold=1.0, skip_bound=0.0))
+    learned, lms = timed(prop)
     ren.SetStepProposal(None)
     ren.SetSampleRateModifier(1.0)
     d = np.abs(learned - fixed)
-    fms, lms = float(np.median(t_fixed)), float(np.median(t_learned))
+    fms = float(np.median(t_fixed))
     return dict(reference_step_voxels=0.25, views=len(views), fixed_fps=round(len(views) / (fms / 1e3), 2),
                 learned_fps=round(len(views) / (lms / 1e3), 2), speedup=round(fms / lms, 3),
+                exact_skip_fps=round(len(views) / (ems / 1e3), 2),
+                exact_skip_max_err=float(np.abs(exact - fixed).max()),
                 max_err=float(d.max()), mean_err=float(d.mean()), max_err_x255=round(float(d.max()) * 255, 3),
                 proposal=prop, train_s=round(train_s, 1), train=info)
 

@@ -287,22 +287,40 @@ int tvk_stepnet_train(tvk_stepnet* s, int n, int epochs, int batch, float lr, fl
 
 /* Learned step on a context: with a network attached, tvk_render and
  * tvk_render_views (1D/2D TF modes) march every ray segment through a brick
- * with the proposed step multiplier m (step m * fRayStepsize, opacity
- * correction exponent m * fStepScale) or skip it; NULL detaches (fixed step). */
+ * in chunks of 32 fixed steps; per chunk the network proposes a step
+ * multiplier m (step m * fRayStepsize, opacity correction exponent
+ * m * fStepScale; a wavefront takes the smallest m of its rays) or a skip.
+ * The contribution bound of a chunk is T_before * (1-(1-alpha_max)^n), with
+ * alpha_max the largest TF alpha over the chunk's value range (from the
+ * dataset's cell grid) and n its fixed-step count.  NULL detaches (fixed step). */
 typedef struct {
   float margin;          /* multiplier = 2^clamp(floor(3*y0 + 0.5 - margin), 0, max_log2) */
   float skip_threshold;  /* skip when sigmoid(y1) > threshold (>= 1: never skip)          */
   int max_log2;          /* 0..3                                                          */
-  float fallback_bound;  /* fixed step where T_before * (1-(1-alpha_max)^n) exceeds this (<= 0: never) */
+  float fallback_bound;  /* fixed step where the contribution bound exceeds this (<= 0: never) */
+  float skip_bound;      /* skip, without the network, chunks whose bound is <= this (< 0: never;
+                            0 skips the chunks whose value range is fully transparent, which
+                            leaves the frame unchanged) */
+  int fixed_log2;        /* >= 0: no network (net may be NULL), every chunk that is not skipped
+                            takes multiplier 2^fixed_log2 (the chunked march without the
+                            network, for comparison); < 0: the network decides */
 } tvk_proposal_params;
 int tvk_set_step_proposal(tvk_ctx* ctx, tvk_stepnet* net, const tvk_proposal_params* p);
+/* The dataset's cell grid (built on first use): per TOC brick, cells of 8^3
+ * stored voxels, x fastest, ceil(n/8) per axis; each word is (max << 16) | min
+ * over the cell's stored voxels grown by 4 on every side (clipped to the
+ * brick).  Writes up to cap words and the total count (*n). */
+int tvk_cell_grid(tvk_ctx* ctx, uint32_t* out, uint64_t cap, uint64_t* n);
 /* Training samples from the fixed-step march: renders nviews views (rots[n*16])
- * and records a hash-selected fraction `keep` of the ray segments (at most cap),
- * each as 48 floats: 24 features, transmittance before the segment, fixed-step
- * sample count, the segment's premultiplied RGBA at multipliers 1, 2, 4, 8, 6 pad. */
+ * and records a hash-selected fraction `keep` of the chunks (at most cap) that
+ * are not fully transparent, each as 48 floats: 24 features, transmittance
+ * before the chunk, fixed-step sample count, the chunk's premultiplied RGBA at
+ * multipliers 1, 2, 4, 8, 6 pad.  *n: records written; *total (may be NULL):
+ * chunks selected by the hash, written or not. */
 int tvk_generate_segments(tvk_ctx* ctx, int nviews, const float* rots, uint32_t cap, float keep, uint32_t seed,
-                          float* out, uint32_t* n);
-/* The same proposal for every 1D/2D-TF request of a batch (NULL: fixed step). */
+                          float* out, uint32_t* n, uint32_t* total);
+/* The same proposal for every 1D/2D-TF request of a batch (NULL: fixed step;
+ * fixed_log2 must be < 0). */
 int tvk_batch_set_step_proposal(tvk_batch* b, tvk_stepnet* net, const tvk_proposal_params* p);
 
 #ifdef __cplusplus

@@ -17,7 +17,7 @@ EXPORTS = [
     "tvk_brick_md", "tvk_brick_minmax", "tvk_get_brick", "tvk_get_pool", "tvk_max_gradient",
     "tvk_set_tf1d", "tvk_set_tf2d", "tvk_set_view_matrices", "tvk_set_camera", "tvk_set_rotation",
     "tvk_get_view_matrices", "tvk_set_params", "tvk_get_params", "tvk_current_lod", "tvk_plan",
-    "tvk_render", "tvk_render_views", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_frame_device_ptr",
+    "tvk_render", "tvk_render_views", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_cell_grid", "tvk_frame_device_ptr",
     "tvk_event_record", "tvk_event_elapsed", "tvk_synchronize",
     "tvk_default_tuning", "tvk_batch_create", "tvk_batch_destroy", "tvk_batch_last_error", "tvk_batch_clear",
     "tvk_batch_add", "tvk_batch_set_tuning", "tvk_batch_get_tuning", "tvk_batch_set_count_samples",
@@ -58,7 +58,7 @@ class Tuning(C.Structure):
 
 class ProposalParams(C.Structure):
     _fields_ = [("margin", C.c_float), ("skip_threshold", C.c_float), ("max_log2", C.c_int),
-                ("fallback_bound", C.c_float)]
+                ("fallback_bound", C.c_float), ("skip_bound", C.c_float), ("fixed_log2", C.c_int)]
 
 
 class LeapStats(C.Structure):
@@ -116,6 +116,7 @@ def load(path=LIB_PATH):
         "tvk_render_views": (C.c_int, [vp, C.c_int, fp, fp, C.POINTER(Stats)]),
         "tvk_set_count_samples": (C.c_int, [vp, C.c_int]),
         "tvk_touched_bytes": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
+        "tvk_cell_grid": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_uint64, u64p]),
         "tvk_frame_device_ptr": (C.c_int, [vp, C.POINTER(vp)]),
         "tvk_event_record": (C.c_int, [vp, C.c_int]),
         "tvk_event_elapsed": (C.c_int, [vp, C.c_int, C.c_int, fp]),
@@ -151,7 +152,7 @@ def load(path=LIB_PATH):
         "tvk_leap_page_in": (C.c_int, [vp, C.c_uint64, u64p]),
         "tvk_leap_resume_state": (C.c_int, [vp, fp, C.c_size_t]),
         "tvk_generate_segments": (C.c_int, [vp, C.c_int, fp, C.c_uint32, C.c_float, C.c_uint32, fp,
-                                            C.POINTER(C.c_uint32)]),
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -16,6 +16,7 @@ std::vector<float4> unorm_table(const uint8_t* rgba8, size_t n) {
 void free_dataset(tvk_ctx* c) {
   c->pool_hold.reset();   // frees the pool once no context shares it
   c->lin_hold.reset();
+  c->cells.reset();
   c->d_pool = c->d_lin = nullptr;
   c->pool_bytes = c->lin_bytes = 0;
   c->has_ds = false;
@@ -34,6 +35,7 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
   c->pool_bytes = (size_t)L.pool_elems * bpv;
   HIPCHK(c, hipMalloc(&c->d_pool, c->pool_bytes));
   c->pool_hold = std::shared_ptr<void>(c->d_pool, [](void* p) { (void)hipFree(p); });
+  c->cells = std::make_shared<CellGrid>();
   // descriptors of every kept brick
   std::vector<GatherBrick> gb(L.toc.size());
   for (size_t i = 0; i < L.toc.size(); ++i) {
@@ -352,6 +354,7 @@ int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src) {
   dst->pool_bytes = src->pool_bytes;
   dst->d_lin = src->d_lin;
   dst->lin_hold = src->lin_hold;
+  dst->cells = src->cells;
   dst->lin_bytes = src->lin_bytes;
   dst->minmax = src->minmax;
   dst->range_min = src->range_min;
@@ -719,6 +722,21 @@ void build_tf_stats(tvk_ctx* c) {
   };
   sparse(a, T.amax);
   sparse(slope, T.smax);
+  // device layout: prefix sums, then the sparse tables with rows of n entries
+  T.levels = (int)T.amax.size();
+  T.flat.assign(tfst_size(n, T.levels), 0.0f);
+  for (int i = 0; i <= n; ++i) {
+    T.flat[(size_t)tfst_pa(n) + i] = (float)T.pa[(size_t)i];
+    T.flat[(size_t)tfst_pnz(n) + i] = (float)T.pnz[(size_t)i];
+  }
+  for (int i = 0; i < n; ++i) {
+    T.flat[(size_t)tfst_ptv(n) + i] = (float)T.ptv[(size_t)i];
+    T.flat[(size_t)tfst_prgb(n) + i] = (float)T.prgb[(size_t)i];
+  }
+  for (int l = 0; l < T.levels; ++l)
+    std::copy(T.amax[(size_t)l].begin(), T.amax[(size_t)l].end(), T.flat.begin() + tfst_amax(n, l));
+  for (int l = 0; l < (int)T.smax.size() && l < T.levels; ++l)
+    std::copy(T.smax[(size_t)l].begin(), T.smax[(size_t)l].end(), T.flat.begin() + tfst_smax(n, T.levels, l));
 }
 
 float range_max(const std::vector<std::vector<float>>& t, int lo, int hi) {   // inclusive, lo <= hi
@@ -727,9 +745,64 @@ float range_max(const std::vector<std::vector<float>>& t, int lo, int hi) {   //
   return std::max(t[(size_t)k][(size_t)lo], t[(size_t)k][(size_t)(hi - (1 << k) + 1)]);
 }
 
+int learn_prepare(tvk_ctx* c) {
+  if (c->tfst_dirty) {
+    build_tf_stats(c);
+    const std::vector<float>& F = c->tfst.flat;
+    if (!F.empty()) {
+      int rc = ensure(c, &c->d_tfst, &c->tfst_cap, F.size());
+      if (rc) return rc;
+      HIPCHK(c, hipMemcpyAsync(c->d_tfst, F.data(), F.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+  }
+  if (c->tfst.n <= 0) return fail(c, TVK_ERR_STATE, "the learned step needs a transfer function");
+  return ensure_cells(c);
+}
+
+int ensure_cells(tvk_ctx* c) {
+  if (!c->cells) return fail(c, TVK_ERR_STATE, "no dataset");
+  CellGrid& G = *c->cells;
+  if (G.d) return TVK_OK;
+  const Layout& L = c->L;
+  std::vector<GatherBrick> gb(L.toc.size());
+  G.base.assign(L.toc.size(), 0);
+  uint64_t total = 0;
+  uint32_t max_cells = 0;
+  for (size_t i = 0; i < L.toc.size(); ++i) {
+    for (int k = 0; k < 3; ++k) { gb[i].c[k] = (uint32_t)L.toc[i].coords[k]; gb[i].n[k] = L.toc[i].size[k]; }
+    gb[i].offset = L.toc[i].offset;
+    const uint32_t nc = cell_count(gb[i].n[0]) * cell_count(gb[i].n[1]) * cell_count(gb[i].n[2]);
+    G.base[i] = (uint32_t)total;
+    total += nc;
+    max_cells = std::max(max_cells, nc);
+  }
+  if (total >= (1ull << 32)) return fail(c, TVK_ERR_UNSUPPORTED, "cell grid exceeds 2^32 cells");
+  GatherBrick* d_gb = nullptr;
+  uint32_t* d_base = nullptr;
+  uint32_t* d_cells = nullptr;
+  hipError_t e = hipMalloc(&d_cells, std::max<uint64_t>(total, 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&d_gb, gb.size() * sizeof(GatherBrick));
+  if (e == hipSuccess) e = hipMalloc(&d_base, G.base.size() * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_gb, gb.data(), gb.size() * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_base, G.base.data(), G.base.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_cells(c->d_pool, L.bpv, d_gb, d_base, (uint32_t)gb.size(), max_cells, d_cells, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_gb) (void)hipFree(d_gb);
+  if (d_base) (void)hipFree(d_base);
+  if (e != hipSuccess) {
+    if (d_cells) (void)hipFree(d_cells);
+    return hipfail(c, "cell grid", e);
+  }
+  G.d = d_cells;
+  return TVK_OK;
+}
+
 void brick_tf_stats(tvk_ctx* c, FrameBrick& fb) {
-  if (c->tfst_dirty) build_tf_stats(c);
   const tvk_ctx::TfStats& T = c->tfst;
+  fb.cells = c->cells && !c->cells->base.empty() ? c->cells->base[fb.toc] : 0u;
   for (float& v : fb.tfs) v = 0.0f;
   if (T.n <= 0 || c->P.mode == TVK_RM_ISOSURFACE) return;
   // TF coordinate and entry of a stored value, as the march computes them
@@ -774,18 +847,27 @@ void fill_shade(const tvk_ctx* c, ShadeConst& S) {
   }
   S.refine_steps = c->P.refine_steps;
   S.clamp_edge = c->P.clamp_to_edge;
+  S.tfst = c->d_tfst;
+  S.tfst_n = c->tfst.n;
+  S.tfst_levels = c->tfst.levels;
+  S.cells = c->cells ? c->cells->d : nullptr;
+  S.overlap = (int)c->L.overlap;
 }
 
 // learn: 0 fixed step, 1 step proposal of c->net, 2 fixed step recording segment samples into *seg
 int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st, int learn = -1,
                 const SegOut* seg = nullptr) {
-  if (learn < 0) learn = (c->net && c->P.mode != TVK_RM_ISOSURFACE) ? 1 : 0;
+  if (learn < 0) learn = (c->learn_on && c->P.mode != TVK_RM_ISOSURFACE) ? 1 : 0;
   if (learn > 0 && c->P.mode == TVK_RM_ISOSURFACE)
     return fail(c, TVK_ERR_UNSUPPORTED, "step proposal is not available in isosurface mode");
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
   if (!c->has_view) return fail(c, TVK_ERR_STATE, "no view");
   if (nviews < 1) return fail(c, TVK_ERR_ARG, "nviews must be >= 1");
   HIPCHK(c, hipSetDevice(c->device));
+  if (learn > 0) {
+    int rc = learn_prepare(c);
+    if (rc) return rc;
+  }
   c->h_list.clear();
   c->h_views.assign((size_t)nviews, ViewConst{});
   std::vector<PlannedBrick> planned;
@@ -852,7 +934,8 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   a.nviews = nviews;
   if (learn == 1) {
     a.learn = c->learn;
-    if (!stepnet_device(c->net, &a.learn.net)) return fail(c, TVK_ERR_STATE, "no step network");
+    if (a.learn.fixed_k < 0 && !stepnet_device(c->net, &a.learn.net))
+      return fail(c, TVK_ERR_STATE, "no step network");
   }
   if (learn == 2 && seg) a.seg = *seg;
   a.lighting = c->P.lighting;
@@ -940,21 +1023,27 @@ int tvk_render_views(tvk_ctx* c, int n, const float* rots, float* host_out, tvk_
 int tvk_set_step_proposal(tvk_ctx* c, tvk_stepnet* net, const tvk_proposal_params* p) {
   if (!c) return TVK_ERR_ARG;
   if (net && !p) return fail(c, TVK_ERR_ARG, "null proposal parameters");
-  c->net = net;
+  c->net = nullptr;
   c->learn = LearnConst{};
-  if (net) {
+  c->learn_on = false;
+  if (net || (p && p->fixed_log2 >= 0)) {
     if (p->max_log2 < 0 || p->max_log2 > 3) return fail(c, TVK_ERR_ARG, "max_log2 must be 0..3");
+    if (p->fixed_log2 > p->max_log2) return fail(c, TVK_ERR_ARG, "fixed_log2 exceeds max_log2");
+    c->net = net;
+    c->learn_on = true;
+    c->learn.fixed_k = p->fixed_log2 >= 0 ? p->fixed_log2 : -1;
     c->learn.margin = p->margin;
     const double q = p->skip_threshold;
     c->learn.skip_logit = q >= 1.0 ? INFINITY : (q <= 0.0 ? -INFINITY : (float)std::log(q / (1.0 - q)));
     c->learn.max_log2 = p->max_log2;
     c->learn.fallback = p->fallback_bound > 0.0f ? p->fallback_bound : INFINITY;
+    c->learn.skip_bound = p->skip_bound;
   }
   return TVK_OK;
 }
 
 int tvk_generate_segments(tvk_ctx* c, int nviews, const float* rots, uint32_t cap, float keep, uint32_t seed,
-                          float* host_out, uint32_t* n) {
+                          float* host_out, uint32_t* n, uint32_t* total) {
   if (!c || !rots || !host_out || !n || nviews < 1 || cap == 0) return TVK_ERR_ARG;
   if (c->P.mode == TVK_RM_ISOSURFACE) return fail(c, TVK_ERR_UNSUPPORTED, "segments need a TF render mode");
   if (!(keep > 0.0f) || keep > 1.0f) return fail(c, TVK_ERR_ARG, "keep fraction must be in (0,1]");
@@ -981,6 +1070,7 @@ int tvk_generate_segments(tvk_ctx* c, int nviews, const float* rots, uint32_t ca
   if (!rc) {
     e = hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (total) *total = cnt;
     cnt = std::min(cnt, cap);
     if (e == hipSuccess && cnt)
       e = hipMemcpy(host_out, d_out, (size_t)cnt * sizeof(SegSample), hipMemcpyDeviceToHost);
@@ -991,6 +1081,25 @@ int tvk_generate_segments(tvk_ctx* c, int nviews, const float* rots, uint32_t ca
   if (d_cnt) (void)hipFree(d_cnt);
   *n = rc ? 0 : cnt;
   return rc;
+}
+
+int tvk_cell_grid(tvk_ctx* c, uint32_t* out, uint64_t cap, uint64_t* n) {
+  if (!c || !n) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int rc = ensure_cells(c);
+  if (rc) return rc;
+  const CellGrid& G = *c->cells;
+  uint64_t total = 0;
+  for (size_t i = 0; i < c->L.toc.size(); ++i) {
+    const uint32_t* sz = c->L.toc[i].size;
+    total += (uint64_t)cell_count(sz[0]) * cell_count(sz[1]) * cell_count(sz[2]);
+  }
+  *n = total;
+  if (out && cap) {
+    HIPCHK(c, hipMemcpy(out, G.d, std::min(cap, total) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  }
+  return TVK_OK;
 }
 
 int tvk_touched_bytes(tvk_ctx* c, uint64_t* bytes, uint64_t* bricks, uint64_t* unique_bytes,

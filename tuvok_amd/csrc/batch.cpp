@@ -151,8 +151,11 @@ int plan_batch(tvk_batch* b, uint32_t* bricks_total) {
       if (!rc) {
         plan_frame(c->L, c->minmax, c->range_max, V, pp, planned);
         make_frame_bricks(c->L, V, pp, planned, P[i].list);
-        if (b->net && c->P.mode != TVK_RM_ISOSURFACE)
-          for (FrameBrick& fb : P[i].list) brick_tf_stats(c, fb);
+        if (b->net && c->P.mode != TVK_RM_ISOSURFACE) {
+          rc = learn_prepare(c);
+          if (!rc)
+            for (FrameBrick& fb : P[i].list) brick_tf_stats(c, fb);
+        }
       }
     }
     if (rc) return bfail(b, rc, "request " + std::to_string(i) + ": " + c->err);
@@ -360,6 +363,7 @@ int tvk_batch_get_tuning(tvk_batch* b, tvk_tuning* t) {
 int tvk_batch_set_step_proposal(tvk_batch* b, tvk_stepnet* net, const tvk_proposal_params* p) {
   if (!b) return TVK_ERR_ARG;
   if (net && !p) return bfail(b, TVK_ERR_ARG, "null proposal parameters");
+  if (p && p->fixed_log2 >= 0) return bfail(b, TVK_ERR_UNSUPPORTED, "a batch takes the network (fixed_log2 < 0)");
   b->net = net;
   b->learn = LearnConst{};
   if (net) {
@@ -369,6 +373,7 @@ int tvk_batch_set_step_proposal(tvk_batch* b, tvk_stepnet* net, const tvk_propos
     b->learn.skip_logit = q >= 1.0 ? INFINITY : (q <= 0.0 ? -INFINITY : (float)std::log(q / (1.0 - q)));
     b->learn.max_log2 = p->max_log2;
     b->learn.fallback = p->fallback_bound > 0.0f ? p->fallback_bound : INFINITY;
+    b->learn.skip_bound = p->skip_bound;
   }
   return TVK_OK;
 }

@@ -34,6 +34,19 @@ hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bri
 }  // namespace tvk
 
 
+namespace tvk {
+hipError_t launch_cells(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t* d_base, uint32_t nb,
+                        uint32_t max_cells, uint32_t* out, hipStream_t st);
+// Cell grid of a dataset (learned mode): per brick, the (min, max) of its cells
+// (tvk_types.hpp), built on first use and shared by the contexts that share
+// the dataset.
+struct CellGrid {
+  uint32_t* d = nullptr;
+  std::vector<uint32_t> base;   // first cell of each TOC brick
+  ~CellGrid() { if (d) (void)hipFree(d); }
+};
+}  // namespace tvk
+
 // the opaque handle of the C ABI
 struct tvk_ctx {
   using Layout = tvk::Layout;
@@ -52,6 +65,7 @@ struct tvk_ctx {
   void* d_pool = nullptr;
   std::shared_ptr<void> pool_hold;   // owns d_pool; shared by contexts that share the dataset
   std::shared_ptr<void> lin_hold;    // owns d_lin
+  std::shared_ptr<tvk::CellGrid> cells;   // created with the pool, filled on first learned use
   size_t pool_bytes = 0;
   void* d_lin = nullptr;
   size_t lin_bytes = 0;
@@ -83,12 +97,17 @@ struct tvk_ctx {
   // step proposal (learned mode): the network and its decision thresholds
   struct tvk_stepnet* net = nullptr;
   tvk::LearnConst learn{};
+  bool learn_on = false;   // chunked march (network, or a fixed multiplier without one)
   // per-entry statistics of the active TF (brick features of the step proposal)
   struct TfStats {
     int n = 0;
     std::vector<double> pa, ptv, prgb, pnz;   // prefix sums of alpha, |dalpha|, |drgb|, alpha > 0
     std::vector<std::vector<float>> amax, smax;   // sparse tables: max alpha, max |dalpha|
+    int levels = 0;            // rows of the device tables
+    std::vector<float> flat;   // the device layout (tvk_types.hpp), kept while the upload runs
   } tfst;
+  float* d_tfst = nullptr;
+  size_t tfst_cap = 0;
   bool tfst_dirty = true;
   // ray-guided traversal (GridLeaper): residency, metadata, miss table, resume state
   std::vector<uint8_t> leap_resident;   // per TOC brick (empty: all resident)
@@ -161,8 +180,12 @@ int frame_lod(tvk_ctx* c, const ViewState& V, int* lod);
 int plan_params(tvk_ctx* c, int lod, PlanParams& pp);
 // GLRenderer::CalculateScaling
 float tf_scale_of(const tvk_ctx* c);
-// TF statistics over a brick's value range (FrameBrick::tfs)
+// TF statistics over a brick's value range (FrameBrick::tfs) and its cell grid
 void brick_tf_stats(tvk_ctx* c, FrameBrick& fb);
+// learned mode: device TF statistics of the active TF and the dataset's cell grid
+int learn_prepare(tvk_ctx* c);
+// the dataset's cell grid (built on first use)
+int ensure_cells(tvk_ctx* c);
 // the step network as the kernels read it (false when s is null)
 bool stepnet_device(const struct tvk_stepnet* s, StepNetDev* out);
 // shading constants of the context's dataset, TF and parameters

@@ -13,6 +13,7 @@
 // Bit-exactness against the procedural restatement is checked by the tests.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
 #include "../tvk_types.hpp"
 
 namespace tvk {
@@ -133,6 +134,37 @@ __global__ __launch_bounds__(256) void brick_minmax(const T* __restrict__ pool, 
     out[2 * blockIdx.x] = mn;
     out[2 * blockIdx.x + 1] = mx;
   }
+}
+
+// Cell grid of every brick: one wave per cell reduces the stored voxels of the
+// cell grown by CELL_DIL on every side (clipped to the brick) to (max << 16) | min.
+template <typename T>
+__global__ __launch_bounds__(256) void cell_minmax(const T* __restrict__ pool, const GatherBrick* __restrict__ bricks,
+                                                   const uint32_t* __restrict__ base, uint32_t* __restrict__ out) {
+  const GatherBrick b = bricks[blockIdx.y];
+  const uint32_t ncx = cell_count(b.n[0]), ncy = cell_count(b.n[1]), ncz = cell_count(b.n[2]);
+  const uint32_t cell = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (cell >= ncx * ncy * ncz) return;   // whole waves leave
+  const int lane = threadIdx.x & 63;
+  const uint32_t cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
+  const int x0 = max((int)(cx << CELL_LW) - CELL_DIL, 0), x1 = min((int)((cx + 1) << CELL_LW) + CELL_DIL, (int)b.n[0]);
+  const int y0 = max((int)(cy << CELL_LW) - CELL_DIL, 0), y1 = min((int)((cy + 1) << CELL_LW) + CELL_DIL, (int)b.n[1]);
+  const int z0 = max((int)(cz << CELL_LW) - CELL_DIL, 0), z1 = min((int)((cz + 1) << CELL_LW) + CELL_DIL, (int)b.n[2]);
+  const int rx = x1 - x0, ry = y1 - y0, cnt = rx * ry * (z1 - z0);
+  const int lx = lx_of<T>();
+  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2);
+  uint32_t mn = 0xffffu, mx = 0u;
+  for (int i = lane; i < cnt; i += 64) {
+    const int x = x0 + i % rx, t = i / rx, y = y0 + t % ry, z = z0 + t / ry;
+    const uint32_t v = pool[b.offset + swz((uint32_t)x, (uint32_t)y, (uint32_t)z, nbx, nby, lx)];
+    mn = min(mn, v);
+    mx = max(mx, v);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, 64));
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+  }
+  if (lane == 0) out[base[blockIdx.y] + cell] = (mx << 16) | mn;
 }
 
 // MaxGradientMagnitude over interior voxels: |(f(x-1)-f(x+1))/(2 norm), ...|
@@ -335,6 +367,21 @@ hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks,
     else
       hipLaunchKernelGGL(brick_minmax<uint16_t>, dim3(cnt), dim3(256), 0, st, (const uint16_t*)pool, d_bricks + first,
                          out + 2 * first);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cells(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t* d_base, uint32_t nb,
+                        uint32_t max_cells, uint32_t* out, hipStream_t st) {
+  const uint32_t gx = (max_cells + 3) / 4;
+  for (uint32_t first = 0; first < nb; first += 65535u) {
+    const uint32_t cnt = std::min(nb - first, 65535u);
+    if (bpv == 1)
+      hipLaunchKernelGGL(cell_minmax<uint8_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint8_t*)pool, d_bricks + first,
+                         d_base + first, out);
+    else
+      hipLaunchKernelGGL(cell_minmax<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint16_t*)pool,
+                         d_bricks + first, d_base + first, out);
   }
   return hipGetLastError();
 }

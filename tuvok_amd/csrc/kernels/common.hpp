@@ -34,8 +34,14 @@ __device__ __forceinline__ float fp16_round(float x) {
 
 // value of a stored voxel as the unorm texture returns it
 template <typename T> struct Unorm;
-template <> struct Unorm<uint8_t> { static __device__ __forceinline__ float inv() { return 1.0f / 255.0f; } };
-template <> struct Unorm<uint16_t> { static __device__ __forceinline__ float inv() { return 1.0f / 65535.0f; } };
+template <> struct Unorm<uint8_t> {
+  static __device__ __forceinline__ float inv() { return 1.0f / 255.0f; }
+  static __device__ __forceinline__ uint32_t max_value() { return 255u; }
+};
+template <> struct Unorm<uint16_t> {
+  static __device__ __forceinline__ float inv() { return 1.0f / 65535.0f; }
+  static __device__ __forceinline__ uint32_t max_value() { return 65535u; }
+};
 
 // Element offset parts of the blocked brick layout (see tvk_types.hpp): the
 // offset of voxel (x,y,z) is ox(x) + oy(y) + oz(z).

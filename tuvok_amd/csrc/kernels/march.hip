@@ -400,134 +400,218 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         continue;
       }
 
-      // the fixed-step march of this segment with the step multiplied by mult
-      // (mult 1: GLRaycaster's step; the opacity correction exponent scales with it)
-      auto segment = [&](float mult, float& c0, float& c1, float& c2, float& c3) {
-      const float rs = rstep * mult;
-      const int steps_m = (int)(L / rs) + 1;
-      const V3 inc_m = (exitTex - entryTex) / (L / rs);
-      const float ssc = sscale * mult;
-      c0 = 0.f; c1 = 0.f; c2 = 0.f; c3 = 0.f;
-      V3 pos = entryTex;
-      V3 pos_s = entryTex;   // the position shade() samples
-      V3 cur = entry;
-      const V3 rayInc = dir / L * rs;
-      // TF, gradient and shading of the sample at pos (the cross sampler's two forms)
-      auto shade = [&](auto fast_c, float& s0, float& s1, float& s2, float& s3) {
-        CrossSampler<T, decltype(fast_c)::value> cs;
-        const float val = cs.value(bd, nx, ny, nz, edge, pos_s);
-        if (MODE == 0) {
-          const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
-          const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
-          const float4 tv = tf1[ti];
-          s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
-          if (LIT && s3 != 0.0f) {   // a transparent sample composites to nothing: skip its shading
-            const V3 gg = cs.gradient(vstep);
-            const V3 nn = normal_eye(gg, VC->nm, dsc);
-            const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
-            s0 = lc.x; s1 = lc.y; s2 = lc.z;
-          }
-          const float tt = 1.0f - s3;
-          s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
-        } else {
-          const float sx = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
-          const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
-          if (((S.tf2_cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
-            // every row of this value column is transparent: alpha 0 whatever the gradient
-            s0 = s1 = s2 = s3 = 0.0f;
-          } else {
-            const V3 gg = cs.gradient(vstep);
-            const float gm = length(gg);
-            const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
-            const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
-            const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];   // byte/255, precomputed
+      // March nsamp samples from (pos, cur) with the step multiplied by 2^k
+      // (k = 0: GLRaycaster's step; the opacity-correction exponent scales with
+      // it), under-compositing into (c0..c3).  Returns false once the segment
+      // ends: 'a >= 0.99', or early termination of the ray whose transmittance
+      // before the brick is tray.
+      const V3 rayInc = dir / L * rstep;
+      auto run = [&](int k, int nsamp, V3& pos, V3& cur, float& c0, float& c1, float& c2, float& c3,
+                     float tray) -> bool {
+        const float mlt = (float)(1 << k);
+        const float ssc = k == 0 ? sscale : sscale * mlt;
+        const V3 inc = k == 0 ? incTex : incTex * mlt;
+        const V3 rinc = k == 0 ? rayInc : rayInc * mlt;
+        V3 pos_s = pos;   // the position shade() samples
+        // TF, gradient and shading of the sample at pos_s (the cross sampler's two forms)
+        auto shade = [&](auto fast_c, float& s0, float& s1, float& s2, float& s3) {
+          CrossSampler<T, decltype(fast_c)::value> cs;
+          const float val = cs.value(bd, nx, ny, nz, edge, pos_s);
+          if (MODE == 0) {
+            const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
+            const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
+            const float4 tv = tf1[ti];
             s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
-            if (LIT) {
+            if (LIT && s3 != 0.0f) {   // a transparent sample composites to nothing: skip its shading
+              const V3 gg = cs.gradient(vstep);
               const V3 nn = normal_eye(gg, VC->nm, dsc);
               const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
-              const float tt = 1.0f - s3;
-              s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
-              s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
+              s0 = lc.x; s1 = lc.y; s2 = lc.z;
+            }
+            const float tt = 1.0f - s3;
+            s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
+          } else {
+            const float sx = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
+            const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
+            if (((S.tf2_cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
+              // every row of this value column is transparent: alpha 0 whatever the gradient
+              s0 = s1 = s2 = s3 = 0.0f;
             } else {
-              const float tt = 1.0f - s3;
-              s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
+              const V3 gg = cs.gradient(vstep);
+              const float gm = length(gg);
+              const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
+              const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
+              const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];   // byte/255, precomputed
+              s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
+              if (LIT) {
+                const V3 nn = normal_eye(gg, VC->nm, dsc);
+                const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
+                const float tt = 1.0f - s3;
+                s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
+                s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
+              } else {
+                const float tt = 1.0f - s3;
+                s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
+              }
             }
           }
-        }
-      };
-      for (int i = 0; i < steps_m; ++i) {
-        ++ns;
-        float s0, s1, s2, s3;
-        // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
-        bool inner = true;
-        {
-          const float pc[3] = {pos.x, pos.y, pos.z};
-          const int nn3[3] = {nx, ny, nz};
+        };
+        for (int i = 0; i < nsamp; ++i) {
+          ++ns;
+          float s0, s1, s2, s3;
+          // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
+          bool inner = true;
+          {
+            const float pc[3] = {pos.x, pos.y, pos.z};
+            const int nn3[3] = {nx, ny, nz};
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
-            const int ia = (int)floorf(u);
-            inner = inner && ia >= 1 && ia + 2 < nn3[a];
+            for (int a = 0; a < 3; ++a) {
+              const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
+              const int ia = (int)floorf(u);
+              inner = inner && ia >= 1 && ia + 2 < nn3[a];
+            }
           }
+          pos_s = pos;
+          if (__ballot(!inner) == 0) shade(std::true_type{}, s0, s1, s2, s3);
+          else shade(std::false_type{}, s0, s1, s2, s3);
+          // UnderCompositing (Compositing.glsl:33-38)
+          const float wgt = (1.0f - c3) * s3;
+          c0 += s0 * wgt;
+          c1 += s1 * wgt;
+          c2 += s2 * wgt;
+          c3 += s3 * (1.0f - c3);
+          if (LIT) cur = cur + rinc;
+          if (c3 >= 0.99f) return false;
+          if (eps > 0.0f && tray * (1.0f - c3) < eps) return false;
+          pos = pos + inc;
         }
-        pos_s = pos;
-        if (__ballot(!inner) == 0) shade(std::true_type{}, s0, s1, s2, s3);
-        else shade(std::false_type{}, s0, s1, s2, s3);
-        // UnderCompositing (Compositing.glsl:33-38)
-        const float wgt = (1.0f - c3) * s3;
-        c0 += s0 * wgt;
-        c1 += s1 * wgt;
-        c2 += s2 * wgt;
-        c3 += s3 * (1.0f - c3);
-        if (LIT) cur = cur + rayInc;
-        if (c3 >= 0.99f) break;
-        if (eps > 0.0f && (1.0f - dst3) * (1.0f - c3) < eps) break;
-        pos = pos + inc_m;
-      }
+        return true;
       };
 
-      float mult = 1.0f;
-      bool skip = false;
-      if (LEARN > 0) {
-        // step proposal features (all lanes: the network runs on the whole wave)
-        float f[SN_IN];
-        const float T0 = 1.0f - dst3;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) f[q < 2 ? q : q + 1] = fb->tfs[q];
-        f[2] = fb->tfs[1] - fb->tfs[0];
-        f[9] = LIT ? 1.0f : 0.0f;
-        f[10] = MODE == 1 ? 1.0f : 0.0f;
+      float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+      if constexpr (LEARN == 0) {
+        V3 pos = entryTex, cur = entry;
+        run(0, steps, pos, cur, c0, c1, c2, c3, 1.0f - dst3);
+      } else {
+        // Learned step: the segment is marched in chunks of SN_CHUNK fixed
+        // steps.  Per chunk every lane bounds its values with the brick's cell
+        // grid (probes every SN_PROBE steps), forms the features, and the
+        // network proposes a multiplier and a skip flag; the wave marches the
+        // chunk with the smallest multiplier any of its marching lanes asks
+        // for.  A chunk whose contribution bound (transmittance before it x
+        // the most opacity its value range allows) is at most skip_bound is
+        // skipped without the network.  LEARN 2 marches the fixed step and
+        // records chunks with their RGBA at multipliers 1, 2, 4 and 8.
+        const float tray = 1.0f - dst3;
+        V3 pos = entryTex, cur = entry;
+        bool live = active;
         V3 dv = mk(incTex.x * (float)nx, incTex.y * (float)ny, incTex.z * (float)nz);
-        const float dl = length(dv);
+        const float dl = length(dv);   // the fixed step in stored voxels
         dv = dl > 0.0f ? dv * (1.0f / dl) : mk(0, 0, 0);
-        f[11] = fabsf(dv.x); f[12] = fabsf(dv.y); f[13] = fabsf(dv.z);
-        f[14] = (float)steps * (1.0f / 512.0f);
-        f[15] = T0;
-        f[16] = -log2f(fmaxf(T0, 1e-6f)) * (1.0f / 20.0f);
-        f[17] = sscale;
-        const float obound = 1.0f - powf(1.0f - fminf(fb->tfs[3], 1.0f), sscale * (float)steps);
-        f[18] = obound;
-        f[19] = T0 * obound;
-        f[20] = -entry.z * 0.25f;
-        f[21] = (float)nx * (1.0f / 256.0f);
-        f[22] = (fb->tfs[1] - fb->tfs[0]) * 16.0f / (float)max(steps, 1);
-        f[23] = 1.0f;
-        if (!active) {
+        // A sample within 3 voxels (per axis) of a probe interpolates voxels
+        // within 4 of the probe's voxel, all inside the probe cell's dilated
+        // range: probes at most SN_PROBE steps apart then bound every value of
+        // the chunk when SN_PROBE steps span at most 6 voxels.  Without overlap
+        // (and GL_CLAMP) the border colour 0 can enter the interpolation.
+        const bool strict = (float)SN_PROBE * dl <= 6.0f;
+        const bool border0 = !edge && S.overlap == 0;
+        const int ncx = (int)cell_count((uint32_t)nx), ncy = (int)cell_count((uint32_t)ny);
+        const uint32_t* __restrict__ cg = S.cells + fb->cells;
+        const int tn = S.tfst_n;
+        const float* __restrict__ ts = S.tfst;
+        const float vmaxs = (float)Unorm<T>::max_value();
+        for (int i0 = 0;; i0 += SN_CHUNK) {
+          const bool has = live && i0 < steps;
+          if (__ballot(has) == 0) break;
+          const int nf = has ? min(SN_CHUNK, steps - i0) : 1;
+          const float T0 = tray * (1.0f - c3);
+          // value range of the chunk (stored values) from the cell grid
+          uint32_t vlo = 0xffffu, vhi = 0u;
 #pragma unroll
-          for (int q = 0; q < SN_IN; ++q) f[q] = 0.0f;
-        }
-        if (LEARN == 1) {
-          float y0, y1;
-          sn_eval_wave(LC->net, f, sn_lds, y0, y1);
-          const int k = max(0, min(LC->max_log2, (int)floorf(3.0f * y0 + 0.5f - LC->margin)));
-          mult = (float)(1 << k);
-          skip = y1 > LC->skip_logit;
-          if (f[19] > LC->fallback) { mult = 1.0f; skip = false; }   // bound exceeded: fixed step
-        } else {
-          if (active) {
-            // record this segment (hash-selected) with its RGBA at multipliers 1, 2, 4, 8
+          for (int q = 0; q <= SN_CHUNK; q += SN_PROBE) {
+            const V3 p = pos + incTex * (float)min(q, nf - 1);
+            const float pc[3] = {p.x, p.y, p.z};
+            const int nn3[3] = {nx, ny, nz};
+            int ci[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const float u = fminf(fmaxf(pc[a], 0.0f), 1.0f) * (float)nn3[a] - 0.5f;
+              ci[a] = min(max((int)floorf(u), 0), nn3[a] - 1) >> CELL_LW;
+            }
+            const uint32_t w = cg[((uint32_t)ci[2] * (uint32_t)ncy + (uint32_t)ci[1]) * (uint32_t)ncx + (uint32_t)ci[0]];
+            vlo = min(vlo, w & 0xffffu);
+            vhi = max(vhi, w >> 16);
+          }
+          if (border0) vlo = 0u;
+          // TF entries of the range (one stored value of slack on either side)
+          auto entry_of = [&](float v) {
+            const float cc = fminf(fmaxf(v * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
+            return min((int)floorf(cc * (float)tn), tn - 1);
+          };
+          const float vl = fmaxf((float)vlo - 1.0f, 0.0f), vh = fminf((float)vhi + 1.0f, vmaxs);
+          const int lo = entry_of(vl), hi = max(entry_of(vh), lo);
+          const float cnt = (float)(hi - lo + 1);
+          const int kk = 31 - __clz(hi - lo + 1);
+          const float amax = fmaxf(ts[tfst_amax(tn, kk) + lo], ts[tfst_amax(tn, kk) + hi - (1 << kk) + 1]);
+          float smax = 0.0f;
+          if (hi > lo) {
+            const int k2 = 31 - __clz(hi - lo);
+            const int r = tfst_smax(tn, S.tfst_levels, k2);
+            smax = fmaxf(ts[r + lo], ts[r + hi - (1 << k2)]);
+          }
+          const float obound = 1.0f - powf(1.0f - fminf(amax, 1.0f), sscale * (float)nf);
+          float f[SN_IN];
+          f[0] = fminf(fmaxf(vl * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
+          f[1] = fminf(fmaxf(vh * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
+          f[2] = f[1] - f[0];
+          f[3] = (ts[tfst_pa(tn) + hi + 1] - ts[tfst_pa(tn) + lo]) / cnt;
+          f[4] = amax;
+          f[5] = ts[tfst_ptv(tn) + hi] - ts[tfst_ptv(tn) + lo];
+          f[6] = ts[tfst_prgb(tn) + hi] - ts[tfst_prgb(tn) + lo];
+          f[7] = smax;
+          f[8] = (ts[tfst_pnz(tn) + hi + 1] - ts[tfst_pnz(tn) + lo]) / cnt;
+          f[9] = LIT ? 1.0f : 0.0f;
+          f[10] = MODE == 1 ? 1.0f : 0.0f;
+          f[11] = fabsf(dv.x); f[12] = fabsf(dv.y); f[13] = fabsf(dv.z);
+          f[14] = (float)nf * (1.0f / (float)SN_CHUNK);
+          f[15] = T0;
+          f[16] = -log2f(fmaxf(T0, 1e-6f)) * (1.0f / 20.0f);
+          f[17] = sscale;
+          f[18] = obound;
+          f[19] = T0 * obound;
+          f[20] = -entry.z * 0.25f;
+          f[21] = dl;
+          f[22] = fb->tfs[3];
+          f[23] = 1.0f;
+          bool skip = has && strict && f[19] <= LC->skip_bound;
+          int k = 0;
+          if constexpr (LEARN == 1) {
+            const bool ask = has && !skip;
+            if (LC->fixed_k >= 0) {
+              k = LC->fixed_k;
+            } else if (__ballot(ask) != 0) {
+              if (!ask) {
+#pragma unroll
+                for (int q = 0; q < SN_IN; ++q) f[q] = 0.0f;
+              }
+              float y0, y1;
+              sn_eval_wave(LC->net, f, sn_lds, y0, y1);
+              if (ask) {
+                k = max(0, min(LC->max_log2, (int)floorf(3.0f * y0 + 0.5f - LC->margin)));
+                skip = y1 > LC->skip_logit;
+                if (T0 * obound > LC->fallback) { k = 0; skip = false; }   // bound exceeded: fixed step
+              }
+            }
+            // one multiplier for the wave: the smallest one a marching lane asks for
+            const bool marching = has && !skip;
+            int kw = LC->max_log2;
+            for (int q = LC->max_log2 - 1; q >= 0; --q)
+              if (__ballot(marching && k <= q) != 0) kw = q;
+            k = kw;
+          } else if (has && !skip) {
+            // record this chunk (hash-selected) with its RGBA at multipliers 1, 2, 4, 8
             uint32_t h = (uint32_t)(px * 73856093) ^ (uint32_t)(py * 19349663) ^ (fb->toc * 83492791u) ^
-                         (view_id * 2654435761u) ^ SO->seed;
+                         (view_id * 2654435761u) ^ ((uint32_t)i0 * 2246822519u) ^ SO->seed;
             h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
             if (h < SO->keep) {
               const unsigned int slot = atomicAdd(SO->count, 1u);
@@ -536,24 +620,30 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
 #pragma unroll
                 for (int q = 0; q < SN_IN; ++q) o->f[q] = f[q];
                 o->t_before = T0;
-                o->steps = (float)steps;
+                o->steps = (float)nf;
 #pragma unroll 1
                 for (int mi = 0; mi < 4; ++mi) {
-                  float a0, a1, a2, a3;
-                  const float dsave = dst3;
-                  dst3 = 0.0f;   // no early termination inside the recorded marches
-                  segment((float)(1 << mi), a0, a1, a2, a3);
-                  dst3 = dsave;
+                  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+                  V3 pp = pos, cc = cur;
+                  run(mi, (nf + (1 << mi) - 1) >> mi, pp, cc, a0, a1, a2, a3, 1.0f);
                   o->c[mi][0] = a0; o->c[mi][1] = a1; o->c[mi][2] = a2; o->c[mi][3] = a3;
                 }
               }
             }
           }
+          if (has) {
+            if (skip) {
+              for (int i = 0; i < nf; ++i) {   // the fixed march's positions, bit for bit
+                if (LIT) cur = cur + rayInc;
+                pos = pos + incTex;
+              }
+            } else {
+              live = run(k, (nf + (1 << k) - 1) >> k, pos, cur, c0, c1, c2, c3, tray);
+            }
+          }
         }
-        if (!active || skip) continue;
+        if (!active) continue;
       }
-      float c0, c1, c2, c3;
-      segment(mult, c0, c1, c2, c3);
       // blend (ONE_MINUS_DST_ALPHA, ONE) into the RGBA32F target
       const float t = 1.0f - dst3;
       dst0 = c0 * t + dst0;
@@ -599,7 +689,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
 
 // Multi-view launch: one 256-thread workgroup = 16x16 pixels = 4 waves of 8x8.
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
-__global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
+__device__ __forceinline__ void march_group(const MarchArgs& A) {
   extern __shared__ float4 s_tf[];  // 1D TF (tf_n entries, MODE 0 only)
   __shared__ __align__(16) uint16_t s_sn[LEARN > 0 ? 4 * SN_WAVE_LDS : 8];   // step network scratch, one slice per wave
 
@@ -625,6 +715,23 @@ __global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
   march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
                                          A.counter, COUNT ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
                                          &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v);
+}
+
+template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
+__global__ __launch_bounds__(256) void march_kernel(MarchArgs A) {
+  march_group<MODE, LIT, T, COUNT, LEARN>(A);
+}
+
+// The learned march (LEARN 1): the network evaluation raises the register
+// count; a cap of TVK_LEARN_WAVES waves per SIMD trades a few spilled
+// registers for occupancy.
+#ifndef TVK_LEARN_WAVES
+#define TVK_LEARN_WAVES 3
+#endif
+template <int MODE, bool LIT, typename T, bool COUNT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TVK_LEARN_WAVES))) void march_kernel_learn(
+    MarchArgs A) {
+  march_group<MODE, LIT, T, COUNT, 1>(A);
 }
 
 // Persistent batch launch: every wave pulls tiles from the per-XCD queues until
@@ -676,9 +783,9 @@ static void launch_one(const MarchArgs& a, dim3 grid, size_t lds, hipStream_t st
     hipLaunchKernelGGL((march_kernel<M, L, T, false, (M != 2 ? 2 : 0)>), grid, dim3(256), lds, st, a);
   else if (learn == 1 && M != 2) {
     if (a.counter)
-      hipLaunchKernelGGL((march_kernel<M, L, T, true, (M != 2 ? 1 : 0)>), grid, dim3(256), lds, st, a);
+      hipLaunchKernelGGL((march_kernel_learn<M, L, T, true>), grid, dim3(256), lds, st, a);
     else
-      hipLaunchKernelGGL((march_kernel<M, L, T, false, (M != 2 ? 1 : 0)>), grid, dim3(256), lds, st, a);
+      hipLaunchKernelGGL((march_kernel_learn<M, L, T, false>), grid, dim3(256), lds, st, a);
   } else if (a.counter)
     hipLaunchKernelGGL((march_kernel<M, L, T, true, 0>), grid, dim3(256), lds, st, a);
   else

@@ -83,53 +83,75 @@ __device__ inline void sn_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS scratch per wave for sn_eval_wave: two buffers of 32 x 64 bf16
-constexpr int SN_WAVE_LDS = 2 * 32 * SN_HID;
+// LDS scratch per wave for sn_eval_wave: X [64][32] and H1 [32][64] bf16
+constexpr int SN_WAVE_LDS = 64 * SN_INP + 32 * SN_HID;
 
-// Evaluates the network for the 64 rays of a wave (lane = ray) in two halves
-// of 32 rays.  f: this lane's features.  lds: SN_WAVE_LDS uint16 per wave.
+// Evaluates the network for the 64 rays of a wave (lane = ray).  f: this
+// lane's features (dead once stored).  lds: SN_WAVE_LDS uint16 per wave.
+// Layers 1 and 2 are MFMA products over 32 rays at a time (hidden unit x ray
+// tiles); layer 3 (64 -> 2) is folded into the layer-2 epilogue: every lane
+// dots its 16 bf16-rounded hidden units of a tile with the bf16 W3 columns,
+// and the two lanes that hold the same ray add their partial sums.  Loops are
+// kept rolled so that the evaluation adds few registers to the caller's.
 __device__ inline void sn_eval_wave(const StepNetDev& N, const float* f, uint16_t* lds, float& y0, float& y1) {
-  const int lane = threadIdx.x & 63;
-  uint16_t* buf0 = lds;                 // X [32][32], then H2 [32][64]
-  uint16_t* buf1 = lds + 32 * SN_HID;   // H1 [32][64]
+  const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+  uint16_t* xs = lds;                   // X [64][32]: ray, feature
+  uint16_t* h1 = lds + 64 * SN_INP;     // H1 [32][64]: ray of the half, hidden unit
+  {
+    sn_u16x8 v[SN_INP / 8];
+#pragma unroll
+    for (int k = 0; k < SN_INP; ++k) v[k >> 3][k & 7] = k < SN_IN ? bf16_bits(f[k]) : (uint16_t)0;
+#pragma unroll
+    for (int q = 0; q < SN_INP / 8; ++q) *reinterpret_cast<sn_u16x8*>(xs + lane * SN_INP + 8 * q) = v[q];
+  }
+  sn_wave_sync();
   float o0 = 0.0f, o1 = 0.0f;
+#pragma unroll 1
   for (int half = 0; half < 2; ++half) {
-    if ((lane >> 5) == half) {
-      uint16_t* x = buf0 + (lane & 31) * SN_INP;
-#pragma unroll
-      for (int k = 0; k < SN_INP; ++k) x[k] = k < SN_IN ? bf16_bits(f[k]) : (uint16_t)0;
-    }
-    sn_wave_sync();
-#pragma unroll
+#pragma unroll 1
     for (int mt = 0; mt < 2; ++mt) {
-      const sn_f32x16 acc = sn_tile_nt<SN_INP>(N.w1t, SN_INP, buf0, SN_INP, 32 * mt, 0, lane);
+      const sn_f32x16 acc = sn_tile_nt<SN_INP>(N.w1t, SN_INP, xs + 32 * half * SN_INP, SN_INP, 32 * mt, 0, lane);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int o = 32 * mt + sn_row(i, lane);
-        buf1[(lane & 31) * SN_HID + o] = bf16_bits(fmaxf(acc[i] + N.b1[o], 0.0f));
+      for (int q = 0; q < 4; ++q) {
+        const int o = 32 * mt + 8 * q + 4 * h;   // rows o .. o+3 of element 4q .. 4q+3
+        const float4 b = *reinterpret_cast<const float4*>(N.b1 + o);
+        const float bb[4] = {b.x, b.y, b.z, b.w};
+        uint16_t hv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) hv[j] = bf16_bits(fmaxf(acc[4 * q + j] + bb[j], 0.0f));
+        uint2 w;
+        w.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
+        w.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
+        *reinterpret_cast<uint2*>(h1 + r * SN_HID + o) = w;
       }
     }
     sn_wave_sync();
-#pragma unroll
+    float p0 = 0.0f, p1 = 0.0f;
+#pragma unroll 1
     for (int mt = 0; mt < 2; ++mt) {
-      const sn_f32x16 acc = sn_tile_nt<SN_HID>(N.w2t, SN_HID, buf1, SN_HID, 32 * mt, 0, lane);
+      const sn_f32x16 acc = sn_tile_nt<SN_HID>(N.w2t, SN_HID, h1, SN_HID, 32 * mt, 0, lane);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int o = 32 * mt + sn_row(i, lane);
-        buf0[(lane & 31) * SN_HID + o] = bf16_bits(fmaxf(acc[i] + N.b2[o], 0.0f));
+      for (int q = 0; q < 4; ++q) {
+        const int o = 32 * mt + 8 * q + 4 * h;
+        const float4 b = *reinterpret_cast<const float4*>(N.b2 + o);
+        const uint2 wa = *reinterpret_cast<const uint2*>(N.w3t + o);            // W3[o..o+3][0]
+        const uint2 wb = *reinterpret_cast<const uint2*>(N.w3t + SN_HID + o);   // W3[o..o+3][1]
+        const float bb[4] = {b.x, b.y, b.z, b.w};
+        const uint32_t a2[4] = {wa.x & 0xffffu, wa.x >> 16, wa.y & 0xffffu, wa.y >> 16};
+        const uint32_t b2[4] = {wb.x & 0xffffu, wb.x >> 16, wb.y & 0xffffu, wb.y >> 16};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float v = bf16_value(bf16_bits(fmaxf(acc[4 * q + j] + bb[j], 0.0f)));
+          p0 += v * bf16_value((uint16_t)a2[j]);
+          p1 += v * bf16_value((uint16_t)b2[j]);
+        }
       }
     }
-    sn_wave_sync();
-    const sn_f32x16 acc = sn_tile_nt<SN_HID>(N.w3t, SN_HID, buf0, SN_HID, 0, 0, lane);
-    // rows 0 and 1 (outputs) of column r sit in elements 0 and 1 of lanes 0..31
-    float a0 = acc[0] + N.b3[0], a1 = acc[1] + N.b3[1];
-    if (half == 0) { o0 = a0; o1 = a1; }
-    else {
-      a0 = __shfl_xor(a0, 32, 64);
-      a1 = __shfl_xor(a1, 32, 64);
-      if (lane >= 32) { o0 = a0; o1 = a1; }
-    }
-    sn_wave_sync();
+    // lanes l and l ^ 32 hold the two halves of the hidden units of ray r
+    p0 += __shfl_xor(p0, 32, 64);
+    p1 += __shfl_xor(p1, 32, 64);
+    if (h == half) { o0 = p0 + N.b3[0]; o1 = p1 + N.b3[1]; }
+    sn_wave_sync();   // H1 is rewritten by the next half
   }
   y0 = o0;
   y1 = o1;

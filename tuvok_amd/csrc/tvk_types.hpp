@@ -38,7 +38,7 @@ struct FrameBrick {
   uint64_t offset;      // element offset of the brick in the pool
   int16_t rect[4];      // conservative pixel rectangle x0,y0,x1,y1 (inclusive)
   uint32_t toc;         // TOC index of the brick
-  uint32_t pad;
+  uint32_t cells;       // first entry of the brick's cell grid (learned mode)
   // transfer-function statistics over the brick's value range (step proposal
   // features): vmin, vmax (TF coordinates), alpha mean, alpha max, alpha total
   // variation, colour total variation, largest alpha step, fraction of the
@@ -81,7 +81,31 @@ struct ShadeConst {
   float tf_scale, grad_scale, iso_n, ert_eps;
   float dscale[3], amb[3], dif[3], spc[3], ldir[3], iso_col[3];
   int refine_steps, clamp_edge;
+  // learned mode: per-entry TF statistics (layout of tf_stats_layout) and the
+  // dataset's cell grid (cell_grid_dims)
+  const float* tfst;
+  const uint32_t* cells;
+  int tfst_n, tfst_levels;
+  int overlap;                  // brick overlap of the dataset
 };
+
+// Cell grid of a brick: cells of CELL_W^3 stored voxels; the (min, max) of a
+// cell (low and high 16 bits) span the stored voxels within CELL_DIL of it, so
+// a probe's cell bounds every value interpolated within 2 voxels of the probe.
+constexpr int CELL_LW = 3, CELL_W = 1 << CELL_LW, CELL_DIL = 4;
+TVK_HD inline uint32_t cell_count(uint32_t n) { return (n + CELL_W - 1) >> CELL_LW; }
+
+// Flat TF statistics over n entries (floats): prefix sums of alpha [n+1], of
+// (alpha > 0) [n+1], of |d alpha| [n], of mean |d rgb| [n]; then `levels`
+// rows of n of the sparse max table of alpha and `levels` rows of the sparse
+// max table of |d alpha|.
+TVK_HD inline int tfst_pa(int) { return 0; }
+TVK_HD inline int tfst_pnz(int n) { return n + 1; }
+TVK_HD inline int tfst_ptv(int n) { return 2 * n + 2; }
+TVK_HD inline int tfst_prgb(int n) { return 3 * n + 2; }
+TVK_HD inline int tfst_amax(int n, int l) { return 4 * n + 2 + l * n; }
+TVK_HD inline int tfst_smax(int n, int levels, int l) { return 4 * n + 2 + (levels + l) * n; }
+TVK_HD inline size_t tfst_size(int n, int levels) { return (size_t)4 * n + 2 + (size_t)2 * levels * n; }
 
 // Step proposal (learned mode): network and decision thresholds.
 struct LearnConst {
@@ -89,15 +113,25 @@ struct LearnConst {
   float margin;        // subtracted from 3*y0 before rounding to a log2 multiplier
   float skip_logit;    // skip a segment when y1 exceeds this logit
   int max_log2;        // largest log2 multiplier (multiplier 1..8)
-  float fallback;      // fixed step when the segment's contribution bound exceeds this
+  float fallback;      // fixed step when the chunk's contribution bound exceeds this
+  float skip_bound;    // skip a chunk whose contribution bound is at most this (< 0: never)
+  int fixed_k = -1;    // >= 0: no network, every marched chunk takes multiplier 2^fixed_k
 };
 
-// One training sample of the step proposal: a ray's segment through one brick.
+// The learned march decides per chunk of SN_CHUNK fixed steps; the cell grid
+// is probed every SN_PROBE steps of a chunk.
+#ifndef TVK_SN_CHUNK
+#define TVK_SN_CHUNK 32
+#endif
+constexpr int SN_CHUNK = TVK_SN_CHUNK, SN_PROBE = 8;
+
+// One training sample of the step proposal: a chunk of a ray's segment
+// through one brick (at most SN_CHUNK fixed steps).
 struct SegSample {
   float f[24];         // features (the network input)
-  float t_before;      // transmittance of the ray before the segment
-  float steps;         // fixed-step sample count of the segment
-  float c[4][4];       // segment RGBA (premultiplied) at multipliers 1, 2, 4, 8
+  float t_before;      // transmittance of the ray before the chunk
+  float steps;         // fixed-step sample count of the chunk
+  float c[4][4];       // chunk RGBA (premultiplied) at multipliers 1, 2, 4, 8
   float pad[6];
 };
 static_assert(sizeof(SegSample) == 192, "SegSample layout");

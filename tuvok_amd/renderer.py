@@ -346,26 +346,46 @@ class HIPRaycaster:
         return img, stats
 
     # -- step proposal ------------------------------------------------------------
-    def SetStepProposal(self, net, margin=0.0, skip_threshold=1.0, max_log2=3, fallback_bound=0.0):
-        """Attach a StepNet (tuvok_amd.stepnet) to march with proposed step
-        multipliers; net=None restores the fixed step."""
-        if net is None:
+    def SetStepProposal(self, net, margin=0.0, skip_threshold=1.0, max_log2=3, fallback_bound=0.0, skip_bound=0.0,
+                        fixed_log2=-1):
+        """Attach a StepNet (tuvok_amd.stepnet) to march in chunks with proposed
+        step multipliers; net=None restores the fixed step, unless fixed_log2 >= 0
+        (the chunked march without a network: multiplier 2^fixed_log2)."""
+        if net is None and fixed_log2 < 0:
             self._chk(self._L.tvk_set_step_proposal(self._h, None, None))
             self._net = None
             return
-        p = _abi.ProposalParams(float(margin), float(skip_threshold), int(max_log2), float(fallback_bound))
-        self._chk(self._L.tvk_set_step_proposal(self._h, net._h, C.byref(p)))
+        p = _abi.ProposalParams(float(margin), float(skip_threshold), int(max_log2), float(fallback_bound),
+                                float(skip_bound), int(fixed_log2))
+        self._chk(self._L.tvk_set_step_proposal(self._h, net._h if net is not None else None, C.byref(p)))
         self._net = net   # keep it alive while attached
 
-    def GenerateSegments(self, rotations, cap, keep=1.0, seed=0):
-        """Segment samples of the fixed-step march over the given views (SegSample
-        records: features, transmittance before, step count, RGBA at x1/x2/x4/x8)."""
+    def GenerateSegments(self, rotations, cap, keep=1.0, seed=0, fit=False):
+        """Chunk samples of the fixed-step march over the given views (records:
+        features, transmittance before, step count, RGBA at x1/x2/x4/x8).  With
+        fit=True and more selected chunks than cap, the views are marched again
+        with the keep fraction that selects about cap of them (an unbiased
+        sample instead of the first cap)."""
         rots = np.ascontiguousarray(np.stack([np.asarray(r, np.float32).reshape(4, 4) for r in rotations]))
         out = np.zeros((int(cap), 48), np.float32)
-        n = C.c_uint32()
+        n, total = C.c_uint32(), C.c_uint32()
         self._chk(self._L.tvk_generate_segments(self._h, rots.shape[0], _fp(rots), int(cap), float(keep), int(seed),
-                                                _fp(out), C.byref(n)))
+                                                _fp(out), C.byref(n), C.byref(total)))
+        if fit and total.value > cap:
+            keep = float(keep) * 0.97 * cap / total.value
+            self._chk(self._L.tvk_generate_segments(self._h, rots.shape[0], _fp(rots), int(cap), keep, int(seed),
+                                                    _fp(out), C.byref(n), C.byref(total)))
+        self.last_segments_total = int(total.value)
         return out[:n.value]
+
+    def CellGrid(self):
+        """The dataset's cell grid (uint32 words (max << 16) | min), TOC brick
+        after TOC brick, each x-fastest with ceil(n/8) cells per axis."""
+        n = C.c_uint64()
+        self._chk(self._L.tvk_cell_grid(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint32)
+        self._chk(self._L.tvk_cell_grid(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size, C.byref(n)))
+        return out
 
     def TouchedBytes(self, unique=False):
         """Stored bytes and count of the bricks the last counted render sampled:

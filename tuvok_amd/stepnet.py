@@ -1,12 +1,14 @@
 """Learned step proposal: the network, its training data and its training.
 
 StepNet wraps the device network (libtvkhip.so: BF16 MFMA inference and
-training kernels).  Training data are segment samples produced by the
-fixed-step march itself (HIPRaycaster.GenerateSegments): for every recorded
-ray segment through a brick, the network input, the ray's transmittance
-before the segment and the segment's RGBA marched at 1, 2, 4 and 8 times the
-fine step.  labels() turns them into targets: the largest multiplier whose
-error against the fine march stays within tau, and whether the whole segment
+training kernels).  The learned march decides per chunk of 32 fixed steps of
+a ray's segment through a brick.  Training data are chunk samples produced by
+the fixed-step march itself (HIPRaycaster.GenerateSegments): for every
+recorded chunk that is not fully transparent, the network input, the ray's
+transmittance before the chunk and the chunk's RGBA marched at 1, 2, 4 and 8
+times the fine step.  labels() turns them into targets: the largest
+multiplier whose error against the fine march stays within
+tau + rel * (the chunk's contribution), and whether the whole chunk
 contributes at most tau_skip.
 """
 import ctypes as C
@@ -84,27 +86,30 @@ class StepNet:
 
 
 def split_samples(rec):
-    """SegSample records (n x 48) -> features, transmittance before, steps, RGBA[n, 4 multipliers, 4]."""
+    """Chunk records (n x 48) -> features, transmittance before, steps, RGBA[n, 4 multipliers, 4]."""
     rec = np.asarray(rec, np.float32)
     return rec[:, :24], rec[:, 24], rec[:, 25], rec[:, 26:42].reshape(-1, 4, 4)
 
 
 def segment_errors(rec):
     """Error of each multiplier against the fine march, as the pixel sees it:
-    transmittance before the segment x the largest channel difference."""
+    transmittance before the chunk x the largest channel difference."""
     _, tb, _, c = split_samples(rec)
     d = np.abs(c - c[:, :1, :]).max(axis=2)          # n x 4
     return tb[:, None] * d
 
 
-def labels(rec, tau, tau_skip, weight_pos=1.0):
-    """Targets (n x 3): t0 = log2(largest safe multiplier)/3, t1 = skip, weight."""
+def labels(rec, tau, tau_skip, weight_pos=1.0, rel=0.0):
+    """Targets (n x 3): t0 = log2(largest safe multiplier)/3, t1 = skip, weight.
+    A multiplier is safe when it and every smaller one stay within
+    tau + rel * (transmittance before x the chunk's alpha) of the fine march."""
     _, tb, _, c = split_samples(rec)
     e = segment_errors(rec)
+    bound = tau + rel * tb * c[:, 0, 3]
     k = np.zeros(len(rec), np.int64)
     ok = np.ones(len(rec), bool)
     for m in (1, 2, 3):
-        ok &= e[:, m] <= tau
+        ok &= e[:, m] <= bound
         k = np.where(ok, m, k)
     contrib = tb * c[:, 0, :].max(axis=1)
     skip = (contrib <= tau_skip).astype(np.float32)
@@ -119,24 +124,24 @@ def decide(y, margin, skip_threshold, max_log2=3):
     return k, p > skip_threshold
 
 
-def train_proposal(ren, rotations, cap=2_000_000, keep=1.0, tau=0.25 / 255, tau_skip=None, epochs=20, batch=4096,
-                   lr=2e-3, seed=0, net=None, device=0):
-    """Segment samples of the renderer's fixed-step march over the given views,
+def train_proposal(ren, rotations, cap=2_000_000, keep=1.0, tau=0.25 / 255, tau_skip=None, rel=0.0, epochs=20,
+                   batch=4096, lr=2e-3, seed=0, net=None, device=0):
+    """Chunk samples of the renderer's fixed-step march over the given views,
     their labels, and a network trained on them with the device kernels."""
-    rec = ren.GenerateSegments(rotations, cap, keep, seed)
+    rec = ren.GenerateSegments(rotations, cap, keep, seed, fit=True)
     if len(rec) == 0:
-        raise RuntimeError("no segment samples (empty views?)")
-    tgt = labels(rec, tau, tau / 2 if tau_skip is None else tau_skip)
+        raise RuntimeError("no chunk samples (empty views?)")
+    tgt = labels(rec, tau, tau / 2 if tau_skip is None else tau_skip, rel=rel)
     net = net or StepNet(device, seed=seed)
     net.set_data(split_samples(rec)[0], tgt)
     losses = net.train(epochs=epochs, batch=batch, lr=lr, seed=seed)
-    info = dict(segments=int(len(rec)), mean_safe_log2=float((tgt[:, 0] * 3).mean()),
+    info = dict(chunks=int(len(rec)), mean_safe_log2=float((tgt[:, 0] * 3).mean()),
                 skip_fraction=float(tgt[:, 1].mean()), loss_first=float(losses[0]), loss_last=float(losses[-1]))
     return net, info
 
 
 def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0),
-              skips=(0.99, 0.9), max_log2=3):
+              skips=(0.999, 0.99, 1.0), max_log2=3, skip_bound=0.0):
     """The most aggressive proposal setting (least device time) whose frames stay
     within `bound` of the fixed-step frames on the given views.  Returns
     (setting or None, [(setting, max error, ms)])."""
@@ -145,7 +150,7 @@ def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.25, 0.5, 0.7
     tried, best = [], None
     for m in margins:
         for s in skips:
-            prop = dict(margin=m, skip_threshold=s, max_log2=max_log2)
+            prop = dict(margin=m, skip_threshold=s, max_log2=max_log2, skip_bound=skip_bound)
             ren.SetStepProposal(net, **prop)
             imgs = ren.PaintViews(rotations)
             err = float(np.abs(imgs - refs).max())
