@@ -53,7 +53,7 @@ struct BrickTex {
     }
     size_t i = (size_t)x + (size_t)n[0] * ((size_t)y + (size_t)n[1] * (size_t)z);
     unsigned v = bpv == 1 ? data[i] : ((const uint16_t*)data)[i];
-    return (float)v * inv_max;
+    return (float)v;   // stored integer; sample() applies the unorm scale once
   }
   // texture3D(texVolume, p).x with GL_LINEAR
   float sample(const V3& p) const {
@@ -71,13 +71,14 @@ struct BrickTex {
     float c010 = texel(i0[0], i0[1] + 1, i0[2]), c110 = texel(i0[0] + 1, i0[1] + 1, i0[2]);
     float c001 = texel(i0[0], i0[1], i0[2] + 1), c101 = texel(i0[0] + 1, i0[1], i0[2] + 1);
     float c011 = texel(i0[0], i0[1] + 1, i0[2] + 1), c111 = texel(i0[0] + 1, i0[1] + 1, i0[2] + 1);
-    float c00 = c000 * (1.0f - f[0]) + c100 * f[0];
-    float c10 = c010 * (1.0f - f[0]) + c110 * f[0];
-    float c01 = c001 * (1.0f - f[0]) + c101 * f[0];
-    float c11 = c011 * (1.0f - f[0]) + c111 * f[0];
-    float c0 = c00 * (1.0f - f[1]) + c10 * f[1];
-    float c1 = c01 * (1.0f - f[1]) + c11 * f[1];
-    return c0 * (1.0f - f[2]) + c1 * f[2];
+    // GL leaves the filter arithmetic to the implementation: x, y, z lerps,
+    // each a + f (b - a) as one fused multiply-add, of the stored integers,
+    // scaled to unorm once (the device sampler's form)
+    auto lerp = [](float a, float b, float t) { return std::fma(t, b - a, a); };
+    float c00 = lerp(c000, c100, f[0]), c10 = lerp(c010, c110, f[0]);
+    float c01 = lerp(c001, c101, f[0]), c11 = lerp(c011, c111, f[0]);
+    float c0 = lerp(c00, c10, f[1]), c1 = lerp(c01, c11, f[1]);
+    return lerp(c0, c1, f[2]) * inv_max;
   }
   // Volume3D.glsl:43-53
   V3 gradient(const V3& c, const V3& h) const {

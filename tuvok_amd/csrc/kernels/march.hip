@@ -30,6 +30,18 @@
 
 namespace tvk {
 
+// Trilinear interpolation of eight texels: x lerps, then y, then z, each
+// lerp a + f (b - a) as one fused multiply-add.  GL leaves the filter's
+// arithmetic to the implementation; the oracle uses this same form.
+__device__ __forceinline__ float lerp_fma(float a, float b, float f) { return fmaf(f, b - a, a); }
+__device__ __forceinline__ float tri_lerp(float c000, float c100, float c010, float c110, float c001, float c101,
+                                          float c011, float c111, float f0, float f1, float f2) {
+  const float c00 = lerp_fma(c000, c100, f0), c10 = lerp_fma(c010, c110, f0);
+  const float c01 = lerp_fma(c001, c101, f0), c11 = lerp_fma(c011, c111, f0);
+  const float c0 = lerp_fma(c00, c10, f1), c1 = lerp_fma(c01, c11, f1);
+  return lerp_fma(c0, c1, f2);
+}
+
 // texture3D with GL_LINEAR and GL_CLAMP (border colour 0) or GL_CLAMP_TO_EDGE
 template <typename T>
 __device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p) {
@@ -57,7 +69,8 @@ __device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, i
   const uint32_t zA = oz<T>(za, nbxy), zB = oz<T>(zb, nbxy);
   // the clamped addresses are always inside the brick: load unconditionally and
   // select the border colour afterwards (no per-tap branch)
-  auto tap = [&](bool ok, uint32_t off) { const float v = (float)d[off] * inv; return ok ? v : 0.0f; };
+  // taps as stored integers (the unorm scale is applied once to the result)
+  auto tap = [&](bool ok, uint32_t off) { const float v = (float)d[off]; return v * (ok ? 1.0f : 0.0f); };
   float c000 = tap(vxa && vya && vza, zA + yA + xA);
   float c100 = tap(vxb && vya && vza, zA + yA + xB);
   float c010 = tap(vxa && vyb && vza, zA + yB + xA);
@@ -66,13 +79,7 @@ __device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, i
   float c101 = tap(vxb && vya && vzb, zB + yA + xB);
   float c011 = tap(vxa && vyb && vzb, zB + yB + xA);
   float c111 = tap(vxb && vyb && vzb, zB + yB + xB);
-  float c00 = c000 * (1.0f - f[0]) + c100 * f[0];
-  float c10 = c010 * (1.0f - f[0]) + c110 * f[0];
-  float c01 = c001 * (1.0f - f[0]) + c101 * f[0];
-  float c11 = c011 * (1.0f - f[0]) + c111 * f[0];
-  float c0 = c00 * (1.0f - f[1]) + c10 * f[1];
-  float c1 = c01 * (1.0f - f[1]) + c11 * f[1];
-  return c0 * (1.0f - f[2]) + c1 * f[2];
+  return tri_lerp(c000, c100, c010, c110, c001, c101, c011, c111, f[0], f[1], f[2]) * inv;
 }
 
 // Volume3D.glsl:43-53 (6-tap central difference at +-1 brick voxel, halved)
@@ -94,13 +101,11 @@ __device__ __forceinline__ V3 gradient(const T* __restrict__ d, int nx, int ny, 
 // edge), otherwise the generic six-sample gradient runs.  value() first loads
 // the centre cube only, so samples whose TF entry is fully transparent never
 // pay for the gradient.
+// sample_brick's interpolation of stored integers, scaled to unorm once
+template <typename T>
 __device__ __forceinline__ float tri(float c000, float c100, float c010, float c110, float c001, float c101, float c011,
                                      float c111, float f0, float f1, float f2) {
-  // sample_brick's order: x lerps, then y, then z
-  const float c00 = c000 * (1.0f - f0) + c100 * f0, c10 = c010 * (1.0f - f0) + c110 * f0;
-  const float c01 = c001 * (1.0f - f0) + c101 * f0, c11 = c011 * (1.0f - f0) + c111 * f0;
-  const float c0 = c00 * (1.0f - f1) + c10 * f1, c1 = c01 * (1.0f - f1) + c11 * f1;
-  return c0 * (1.0f - f2) + c1 * f2;
+  return tri_lerp(c000, c100, c010, c110, c001, c101, c011, c111, f0, f1, f2) * Unorm<T>::inv();
 }
 
 // FAST: every tap of the 4x4x4 neighbourhood lies inside the brick for every
@@ -115,15 +120,18 @@ struct CrossSampler {
   int ic[3];
   float fc[3];
   float c[2][2][2];
-  // element offset parts and border validity of indices ic-1 .. ic+2 per axis
+  // byte offset parts and border validity of indices ic-1 .. ic+2 per axis
   uint32_t ofx[4], ofy[4], ofz[4];
   bool okx[4], oky[4], okz[4];
 
   __device__ __forceinline__ float V(int i, int j, int k) const {   // i,j,k in 0..3 -> index ic-1+i ...
     // offsets are clamped into the brick: unconditional load, then the border select
-    const float v = (float)d[ofz[k] + ofy[j] + ofx[i]] * Unorm<T>::inv();
+    // byte offsets below 2^32: a uniform base plus a 32-bit lane offset
+    const float v = (float)*reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(d) + (ofz[k] + ofy[j] + ofx[i]));
     if (FAST) return v;
-    return (okx[i] && oky[j] && okz[k]) ? v : 0.0f;
+    // border colour 0 as a product with 0/1 (v is finite and >= 0), so that the
+    // load stays unconditional and all taps are in flight at once
+    return v * ((okx[i] && oky[j] && okz[k]) ? 1.0f : 0.0f);
   }
 
   __device__ __forceinline__ float value(const T* __restrict__ dd, int nx, int ny, int nz, bool e, V3 pos) {
@@ -140,16 +148,16 @@ struct CrossSampler {
     for (int k = 0; k < 4; ++k) {
       const int x = ic[0] - 1 + k, y = ic[1] - 1 + k, z = ic[2] - 1 + k;
       if (FAST) {
-        ofx[k] = ox<T>(x);
-        ofy[k] = oy<T>(y, nbx);
-        ofz[k] = oz<T>(z, nbxy);
+        ofx[k] = ox<T>(x) * (uint32_t)sizeof(T);
+        ofy[k] = oy<T>(y, nbx) * (uint32_t)sizeof(T);
+        ofz[k] = oz<T>(z, nbxy) * (uint32_t)sizeof(T);
       } else {
         okx[k] = edge || (x >= 0 && x < nx);
         oky[k] = edge || (y >= 0 && y < ny);
         okz[k] = edge || (z >= 0 && z < nz);
-        ofx[k] = ox<T>(min(max(x, 0), nx - 1));
-        ofy[k] = oy<T>(min(max(y, 0), ny - 1), nbx);
-        ofz[k] = oz<T>(min(max(z, 0), nz - 1), nbxy);
+        ofx[k] = ox<T>(min(max(x, 0), nx - 1)) * (uint32_t)sizeof(T);
+        ofy[k] = oy<T>(min(max(y, 0), ny - 1), nbx) * (uint32_t)sizeof(T);
+        ofz[k] = oz<T>(min(max(z, 0), nz - 1), nbxy) * (uint32_t)sizeof(T);
       }
     }
 #pragma unroll
@@ -158,7 +166,7 @@ struct CrossSampler {
       for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int k = 0; k < 2; ++k) c[i][j][k] = V(1 + i, 1 + j, 1 + k);
-    return tri(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], fc[0],
+    return tri<T>(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], fc[0],
                fc[1], fc[2]);
   }
 
@@ -200,27 +208,31 @@ struct CrossSampler {
         yl[j][k] = V(1 + j, 0, 1 + k); yr[j][k] = V(1 + j, 3, 1 + k);
         zl[j][k] = V(1 + j, 1 + k, 0); zr[j][k] = V(1 + j, 1 + k, 3);
       }
-    const float fxm = tri(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
+    const float fxm = tri<T>(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
                           fm[0], fc[1], fc[2]);
-    const float fxp = tri(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
+    const float fxp = tri<T>(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
                           fq[0], fc[1], fc[2]);
-    const float fym = tri(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
+    const float fym = tri<T>(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
                           fc[0], fm[1], fc[2]);
-    const float fyp = tri(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
+    const float fyp = tri<T>(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
                           fc[0], fq[1], fc[2]);
-    const float fzm = tri(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
+    const float fzm = tri<T>(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
                           fc[0], fc[1], fm[2]);
-    const float fzp = tri(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
+    const float fzp = tri<T>(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
                           fc[0], fc[1], fq[2]);
     return mk(fxm - fxp, fym - fyp, fzm - fzp) / 2.0f;
   }
 };
 
 
+// normalize() with the hardware reciprocal square root (about 1 ulp; used
+// for shading directions only, never for positions, step counts or TF indices)
+__device__ __forceinline__ V3 normalize_rsq(V3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
+
 // lighting.glsl:33-50 (eye at the origin)
 __device__ __forceinline__ V3 lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {
-  V3 view = normalize(mk(0, 0, 0) - pos);
-  V3 R = normalize(reflect(view, n));
+  V3 view = normalize_rsq(mk(0, 0, 0) - pos);
+  V3 R = normalize_rsq(reflect(view, n));
   float dd = fmaxf(fabsf(dot(n, L)), 0.0f);
   float ss = pow8(fmaxf(dot(R, L), 0.0f));
   V3 c = amb + dif * dd + spc * ss;
@@ -230,8 +242,8 @@ __device__ __forceinline__ V3 lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 
 // Compose-FS.glsl Lighting()
 __device__ __forceinline__ V3 iso_lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {
   n.z = fabsf(n.z);
-  V3 view = normalize(mk(0, 0, 0) - pos);
-  V3 R = normalize(reflect(view, n));
+  V3 view = normalize_rsq(mk(0, 0, 0) - pos);
+  V3 R = normalize_rsq(reflect(view, n));
   float dd = fmaxf(fabsf(dot(n, L * -1.0f)), 0.0f);
   float ss = pow8(fmaxf(dot(R, L), 0.0f));
   V3 c = amb + dif * dd + spc * ss;
@@ -260,8 +272,8 @@ __device__ __forceinline__ V3 normal_eye(V3 g, const float* nm, V3 ds) {
   V3 s = g * ds;
   V3 n = mk(nm[0] * s.x + nm[1] * s.y + nm[2] * s.z, nm[3] * s.x + nm[4] * s.y + nm[5] * s.z,
             nm[6] * s.x + nm[7] * s.y + nm[8] * s.z);
-  float l = length(n);
-  if (l > 0.0f) { const float r = 1.0f / l; n = n * r; }
+  const float l2 = dot(n, n);
+  if (l2 > 0.0f) n = n * __builtin_amdgcn_rsqf(l2);
   return n;
 }
 
