@@ -79,6 +79,8 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     train_s = time.perf_counter() - t0
     print(f"[bench] learned: {info} in {train_s:.1f} s", file=sys.stderr, flush=True)
     prop, tried = calibrate(ren, net, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 3)])
+    for (p, e, ms) in sorted(tried, key=lambda t: t[2])[:6]:
+        print(f"[bench] calibration: {p} max err {e * 255:.3f}/255 {ms:.2f} ms", file=sys.stderr, flush=True)
     if prop is None:
         return dict(error="no proposal setting met 0.9/255 on the calibration views", train=info)
     ren.SetStepProposal(None)

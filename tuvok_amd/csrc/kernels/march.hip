@@ -598,10 +598,13 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           bool skip = has && strict && f[19] <= LC->skip_bound;
           int k = 0;
           if constexpr (LEARN == 1) {
-            const bool ask = has && !skip;
+            // a lane whose bound exceeds the fallback marches the fixed step and
+            // so sets the wave's multiplier to 1: then the network is not asked
+            const bool forced = has && !skip && T0 * obound > LC->fallback;
+            const bool ask = has && !skip && !forced;
             if (LC->fixed_k >= 0) {
               k = LC->fixed_k;
-            } else if (__ballot(ask) != 0) {
+            } else if (__ballot(forced) == 0 && __ballot(ask) != 0) {
               if (!ask) {
 #pragma unroll
                 for (int q = 0; q < SN_IN; ++q) f[q] = 0.0f;
@@ -611,7 +614,6 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
               if (ask) {
                 k = max(0, min(LC->max_log2, (int)floorf(3.0f * y0 + 0.5f - LC->margin)));
                 skip = y1 > LC->skip_logit;
-                if (T0 * obound > LC->fallback) { k = 0; skip = false; }   // bound exceeded: fixed step
               }
             }
             // one multiplier for the wave: the smallest one a marching lane asks for
