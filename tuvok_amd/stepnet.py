@@ -141,7 +141,8 @@ def train_proposal(ren, rotations, cap=2_000_000, keep=1.0, tau=0.25 / 255, tau_
 
 
 def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.5, 1.0, 1.5, 2.0, 3.0),
-              skips=(0.999, 1.0), fallbacks=(0.0, 0.02, 0.005, 0.001), max_log2=3, skip_bound=0.0):
+              skips=(0.999, 1.0), fallbacks=(0.0, 0.05, 0.02, 0.01, 0.005, 0.002, 0.001), max_log2=3,
+              skip_bound=0.0):
     """The most aggressive proposal setting (least device time) whose frames stay
     within `bound` of the fixed-step frames on the given views (fallback 0: no
     fallback).  Returns (setting or None, [(setting, max error, ms)])."""
