@@ -40,11 +40,13 @@ struct Leaper {
   std::vector<uint64_t> ilayout;   // GetBrickLayout, 3 per level
   std::vector<uint64_t> lodoff;    // m_vLoDOffsetTable
   std::vector<uint32_t> meta;      // per brick ID
-  M4 emm, proj, inv_proj;
+  M4 emm, proj, inv_proj, mte, mvi;
   int W, H;
   float lod_factor, l0err, srm;
   V3 mlight, meye, dscale, amb, dif, spc;
   float tf_scale, grad_scale;
+  float iso_n;
+  V3 iso_col, ldir;
 
   uint64_t id(uint64_t x, uint64_t y, uint64_t z, int l) const {
     const uint64_t* b = &ilayout[3 * (size_t)l];
@@ -99,7 +101,12 @@ bool setup(Leaper& L, const orc_ds* ds, const orc_view* v, const orc_params* p, 
   if (off != t.toc.size()) return false;   // the pool's brick grid must be the octree's
   // visibility (GLVolumePool.cpp:961-1100): scalar range test, child-empty propagation
   double fmin = 0, fmax = 0;
-  tf_alpha_limits_scale(t, p, fmin, fmax);
+  if (p->mode == 2) {
+    // ContainsData for RM_ISOSURFACE (GLVolumePool.cpp:983-985): min <= isovalue <= max
+    fmin = fmax = (double)p->isovalue;
+  } else {
+    tf_alpha_limits_scale(t, p, fmin, fmax);
+  }
   std::vector<uint8_t> empty(t.toc.size(), 0);
   for (size_t i = 0; i < t.toc.size(); ++i)
     empty[i] = !(fmax >= t.toc[i].minmax[0] && fmin <= t.toc[i].minmax[1]);
@@ -136,6 +143,9 @@ bool setup(Leaper& L, const orc_ds* ds, const orc_view* v, const orc_params* p, 
   const M4 mv = mul(mul(from_arr(v->rot), from_arr(v->trans)), from_arr(v->view));
   L.emm = mul(mul(mul(inverse(mv), translation(-0.0f, -0.0f, -0.0f)), scaling(1.0f / ext.x, 1.0f / ext.y, 1.0f / ext.z)),
               translation(0.5f, 0.5f, 0.5f));
+  // isosurface: mModelToEye and mModelViewIT (GLGridLeaper.cpp:720-723)
+  L.mte = inverse(L.emm);
+  L.mvi = inverse(mv);
   L.proj = from_arr(v->proj);
   L.inv_proj = inverse(L.proj);
   L.W = (int)v->width;
@@ -155,6 +165,9 @@ bool setup(Leaper& L, const orc_ds* ds, const orc_view* v, const orc_params* p, 
   L.mlight = V3(ml.x / mll, ml.y / mll, ml.z / mll);   // FLOATVECTOR3::normalized
   L.tf_scale = p->tf_scale;
   L.grad_scale = p->grad_scale;
+  L.iso_n = (float)((double)p->isovalue / (double)(1 << p->bit_width));   // GetNormalizedIsovalue
+  L.iso_col = V3(p->iso_color[0], p->iso_color[1], p->iso_color[2]);
+  L.ldir = V3(p->light_dir[0], p->light_dir[1], p->light_dir[2]);
   return true;
 }
 
@@ -209,6 +222,17 @@ V3 lighting(const V3& eye, const V3& pos, const V3& n, const V3& amb, const V3& 
   V3 view = normalize_rcp(eye - pos);
   V3 R = normalize_rcp(reflect(view, n));
   float d = std::max(std::fabs(dot(n, L)), 0.0f);
+  float s = pow8(std::max(dot(R, L), 0.0f));
+  V3 c = amb + diff * d + spec * s;
+  return V3(clamp01(c.x), clamp01(c.y), clamp01(c.z));
+}
+
+// Compose-FS.glsl Lighting(): eye-space position and normal, eye at the origin
+V3 compose_lighting(const V3& pos, V3 n, const V3& amb, const V3& diff, const V3& spec, const V3& L) {
+  n.z = std::fabs(n.z);
+  V3 view = normalize_rcp(V3(0, 0, 0) - pos);
+  V3 R = normalize_rcp(reflect(view, n));
+  float d = std::max(std::fabs(dot(n, L * -1.0f)), 0.0f);
   float s = pow8(std::max(dot(R, L), 0.0f));
   V3 c = amb + diff * d + spec * s;
   return V3(clamp01(c.x), clamp01(c.y), clamp01(c.z));
@@ -277,8 +301,9 @@ struct Ray {
 void pass_pixel(const Leaper& L, int px, int py, const Ray& r, float out[4], float resume[8],
                 uint8_t* missing, uint64_t& ns) {
   const Octree& t = *L.t;
+  const orc_params* p = L.p;
   float acc[4] = {r.col[0], r.col[1], r.col[2], r.col[3]};
-  for (int k = 0; k < 4; ++k) { resume[4 + k] = acc[k]; out[k] = acc[k]; }
+  for (int k = 0; k < 4; ++k) { resume[4 + k] = acc[k]; out[k] = p->mode == 2 ? 0.0f : acc[k]; }
   resume[0] = r.pos.x; resume[1] = r.pos.y; resume[2] = r.pos.z; resume[3] = r.depth;
   if (!r.live) { resume[3] = 1000.0f; return; }
   // exit point and depth from the box's back face along the pixel ray
@@ -395,6 +420,32 @@ void pass_pixel(const Leaper& L, int px, int py, const Ray& r, float out[4], flo
         V3 b = pin;
         for (int i = 0; i < steps; ++i) {
           ++ns;
+          if (p->mode == 2) {
+            // GLGridLeaper-iso.glsl: first sample at or above the isovalue, then
+            // RefineIsosurface (GLGridLeaper-Method-iso.glsl) in brick voxels
+            if (tx.sample(b) >= L.iso_n) {
+              V3 rd = step / 2.0f;
+              V3 bb = b - rd;
+              for (int k = 0; k < p->refine_steps; ++k) {
+                rd = rd / 2.0f;
+                if (tx.sample(bb) >= L.iso_n) bb = bb - rd; else bb = bb + rd;
+              }
+              // rayHitPos = mModelToEye * hit; rayHitNormal = mModelViewIT * ComputeNormal
+              const V3 he = xform_point((bb - tr) / sc, L.mte);
+              V3 g = tx.gradient(bb) * L.dscale;
+              const float l = length(g);
+              if (l > 0.0f) g = V3(g.x / l, g.y / l, g.z / l);
+              const float* MI = L.mvi.a;
+              const V3 ne(g.x * MI[0] + g.y * MI[1] + g.z * MI[2], g.x * MI[4] + g.y * MI[5] + g.z * MI[6],
+                          g.x * MI[8] + g.y * MI[9] + g.z * MI[10]);
+              const V3 c = compose_lighting(he, ne, L.amb, L.dif * L.iso_col, L.spc, L.ldir);   // Compose-FS.glsl
+              out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = 1.0f;
+              if (optimal) resume[3] = 501.0f;   // TerminateRay: 499 + rayHitPos.a (1 + white)
+              return;
+            }
+            b = b + step;
+            continue;
+          }
           float c[4];
           color_at(L, tx, b, (b - tr) / sc, c);
           c[3] = 1.0f - (oc == 1.0f ? 1.0f - c[3] : std::pow(1.0f - c[3], oc));   // OpacityCorrectColor
@@ -419,7 +470,7 @@ void pass_pixel(const Leaper& L, int px, int py, const Ray& r, float out[4], flo
       if (tt > 0.9999f) break;
     }
   }
-  for (int k = 0; k < 4; ++k) out[k] = acc[k];
+  for (int k = 0; k < 4; ++k) out[k] = p->mode == 2 ? 0.0f : acc[k];
   terminate(optimal);
 }
 

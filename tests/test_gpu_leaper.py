@@ -8,8 +8,8 @@ import pytest
 
 import oracle_ffi as O
 from helpers import err_stats, oracle_camera_from, oracle_params_from
-from tuvok_amd import (GEN_BLOBS, GEN_FBM_SHELLS, RM_1DTRANS, RM_2DTRANS, HIPRaycaster, rotation_y,
-                       tf1d_blue_white_red, tf2d_gaussian_lobes)
+from tuvok_amd import (GEN_BLOBS, GEN_ELLIPSOID_SDF, GEN_FBM_SHELLS, RM_1DTRANS, RM_2DTRANS, RM_ISOSURFACE,
+                       HIPRaycaster, rotation_y, tf1d_blue_white_red, tf2d_gaussian_lobes)
 
 pytestmark = pytest.mark.gpu
 
@@ -22,6 +22,12 @@ def setup(kind="blobs", eye=(0.1, -0.2, 2.0), w=96, h=80):
         r.Set1DTrans(tf)
         r.SetRendermode(RM_1DTRANS)
         p = dict(tf1d=tf)
+    elif kind == "sdf":
+        r.GenerateVolume(GEN_ELLIPSOID_SDF, 2, np.uint8, (64, 64, 64), (18, 18, 18), 1, max_lods=0, keep_linear=True)
+        r.SetRendermode(RM_ISOSURFACE)
+        r.SetIsoValue(128.0)
+        r.SetRefineSteps(4)
+        p = dict()
     else:
         r.GenerateVolume(GEN_FBM_SHELLS, 1, np.uint16, (64, 64, 64), (18, 18, 18), 1, max_lods=0, keep_linear=True)
         tf = tf2d_gaussian_lobes().GetByteArray()
@@ -111,4 +117,34 @@ def test_distant_view_uses_coarser_levels():
     img3, st3 = r.LeapRender(first=True)
     print(f"  samples far {far_samples} near {st3.samples}", flush=True)
     assert far_samples < st3.samples
+    r.close()
+
+
+def test_isosurface_passes_match_oracle():
+    """GLGridLeaper-iso.glsl: first sample at or above the isovalue, refinement,
+    hit position and normal in eye space, Compose-FS shading; bricks empty unless
+    min <= iso <= max.  All resident, then paging from the coarse levels."""
+    r, ds, p, eye = setup("sdf", eye=(0.1, 0.1, 1.8))
+    for rot in (None, rotation_y(0.7)):
+        if rot is not None:
+            r.SetRotation(rot)
+        img, res, miss = check_pass(r, ds, p, eye, True, None, None, rot)
+        hits = img[..., 3] > 0
+        assert miss.sum() == 0 and hits.sum() > 50
+        assert (img[..., 3][hits] == 1).all()
+        assert set(np.unique(res[..., 3])) <= {501.0, 1000.0}
+        assert ((res[..., 3] == 501) == hits).all()
+    n, nl = ds.brick_count, ds.lod_count
+    coarse = np.array([ds.brick_info(i)["coords"][3] >= nl - 2 for i in range(n)], np.uint8)
+    r.LeapSetResident(coarse)
+    resident, resume, first = coarse.copy(), None, True
+    for k in range(12):
+        img, res, miss = check_pass(r, ds, p, eye, first, resident, resume, rotation_y(0.7))
+        if miss.sum() == 0:
+            break
+        r.LeapPageIn()
+        resident = np.maximum(resident, miss)
+        resume, first = res, False
+    assert k > 0 and miss.sum() == 0
+    r.LeapSetResident(None)
     r.close()

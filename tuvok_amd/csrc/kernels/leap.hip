@@ -77,6 +77,17 @@ __device__ __forceinline__ V3 lighting_eye(V3 eye, V3 pos, V3 n, V3 amb, V3 dif,
   return mk(clamp01(c.x), clamp01(c.y), clamp01(c.z));
 }
 
+// Compose-FS.glsl Lighting(): eye-space hit position and normal, eye at the origin
+__device__ __forceinline__ V3 compose_lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {
+  n.z = fabsf(n.z);
+  const V3 view = normalize(mk(0, 0, 0) - pos);
+  const V3 R = normalize(reflect(view, n));
+  const float dd = fmaxf(fabsf(dot(n, L * -1.0f)), 0.0f);
+  const float ss = pow8(fmaxf(dot(R, L), 0.0f));
+  const V3 c = amb + dif * dd + spc * ss;
+  return mk(clamp01(c.x), clamp01(c.y), clamp01(c.z));
+}
+
 __device__ __forceinline__ V3 xf_point(V3 p, const float* M) {
   return mk(p.x * M[0] + p.y * M[4] + p.z * M[8] + 1.0f * M[12], p.x * M[1] + p.y * M[5] + p.z * M[9] + 1.0f * M[13],
             p.x * M[2] + p.y * M[6] + p.z * M[10] + 1.0f * M[14]);
@@ -157,6 +168,11 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
   float4 rcol = make_float4(acc[0], acc[1], acc[2], acc[3]);
   uint32_t ns = 0;
   bool optimal = true;
+  // isosurface hit (MODE 2): brick voxel position, brick transform and data
+  bool have_hit = false;
+  V3 hit_b = mk(0, 0, 0), hit_tr = mk(0, 0, 0), hit_sc = mk(1, 1, 1);
+  const T* hit_d = nullptr;
+  int hit_n3[3] = {1, 1, 1};
   if (live) {
     const V3 pe = d * tfar;
     const V3 exitN = xf_point(pe, A.emm);
@@ -249,6 +265,25 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
           for (int i = 0; i < steps; ++i) {
             ++ns;
             const float data = sample_vox<T>(bd, nx, ny, nz, b);
+            if constexpr (MODE == 2) {
+              // GLGridLeaper-iso.glsl: first sample at or above the isovalue,
+              // then RefineIsosurface (GLGridLeaper-Method-iso.glsl) in brick voxels
+              if (data >= S.iso_n) {
+                V3 rd = step / 2.0f;
+                V3 bb = b - rd;
+                for (int k = 0; k < S.refine_steps; ++k) {
+                  rd = rd / 2.0f;
+                  if (sample_vox<T>(bd, nx, ny, nz, bb) >= S.iso_n) bb = bb - rd; else bb = bb + rd;
+                }
+                // the normal is taken once the brick loop is over
+                hit_b = bb; hit_tr = tr; hit_sc = sc; hit_d = bd; hit_n3[0] = nx; hit_n3[1] = ny; hit_n3[2] = nz;
+                have_hit = true;
+                done = true;
+                break;
+              }
+              b = b + step;
+              continue;
+            }
             float c[4];
             if (MODE == 0) {
               const float cc = fminf(fmaxf(data * S.tf_scale, 0.0f), 1.0f);
@@ -298,13 +333,35 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
     }
     // TerminateRay
     if (optimal) {
-      rpos.w = 1000.0f;
+      // isosurface: 499 + rayHitPos.a, i.e. 501 for a (white) hit, 1000 without one
+      rpos.w = (MODE == 2 && have_hit) ? 501.0f : 1000.0f;
       rcol = make_float4(acc[0], acc[1], acc[2], acc[3]);
     }
   } else {
     rpos.w = 1000.0f;
   }
-  reinterpret_cast<float4*>(A.out)[pix] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  if constexpr (MODE == 2) {
+    // rayHitPos = mModelToEye * hit, rayHitNormal = mModelViewIT * ComputeNormal
+    // (GLGridLeaper-GradientTools.glsl), then Compose-FS.glsl shading
+    float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (have_hit) {
+      const V3 he = xf_point((hit_b - hit_tr) / hit_sc, A.mte);
+      V3 g = gradient_vox<T>(hit_d, hit_n3[0], hit_n3[1], hit_n3[2], hit_b) * mk(S.dscale[0], S.dscale[1], S.dscale[2]);
+      const float l = length(g);
+      if (l > 0.0f) g = mk(g.x / l, g.y / l, g.z / l);
+      const float* N = A.nit;
+      const V3 ne = mk(g.x * N[0] + g.y * N[3] + g.z * N[6], g.x * N[1] + g.y * N[4] + g.z * N[7],
+                       g.x * N[2] + g.y * N[5] + g.z * N[8]);
+      const V3 isoc = mk(S.iso_col[0], S.iso_col[1], S.iso_col[2]);
+      const V3 c = compose_lighting(he, ne, mk(S.amb[0], S.amb[1], S.amb[2]),
+                                    mk(S.dif[0], S.dif[1], S.dif[2]) * isoc, mk(S.spc[0], S.spc[1], S.spc[2]),
+                                    mk(S.ldir[0], S.ldir[1], S.ldir[2]));
+      res = make_float4(c.x, c.y, c.z, 1.0f);
+    }
+    reinterpret_cast<float4*>(A.out)[pix] = res;
+  } else {
+    reinterpret_cast<float4*>(A.out)[pix] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  }
   A.resume[2 * pix] = rpos;
   A.resume[2 * pix + 1] = rcol;
   if (A.counter && ns) atomicAdd(A.counter, (unsigned long long)ns);
@@ -329,6 +386,8 @@ hipError_t launch_leap(const LeapArgs& a, int bpv, hipStream_t st) {
   } else if (a.mode == 1) {
     if (lit) { if (u16) launch_leap_one<1, true, uint16_t>(a, blocks, st); else launch_leap_one<1, true, uint8_t>(a, blocks, st); }
     else { if (u16) launch_leap_one<1, false, uint16_t>(a, blocks, st); else launch_leap_one<1, false, uint8_t>(a, blocks, st); }
+  } else if (a.mode == 2) {
+    if (u16) launch_leap_one<2, false, uint16_t>(a, blocks, st); else launch_leap_one<2, false, uint8_t>(a, blocks, st);
   } else {
     return hipErrorInvalidValue;
   }

@@ -65,16 +65,22 @@ int make_grid(tvk_ctx* c, Grid& G) {
 // brick metadata: GLGridLeaper::RecomputeBrickVisibility + GLVolumePool visibility
 int make_meta(tvk_ctx* c, const Grid& G, std::vector<uint32_t>& meta) {
   const Layout& L = c->L;
-  const bool two = c->P.mode == TVK_RM_2DTRANS;
-  if (c->P.mode == TVK_RM_ISOSURFACE) return fail(c, TVK_ERR_UNSUPPORTED, "traversal modes are 1D and 2D TF");
-  const int n = two ? c->tf2d_w : c->tf1d_n;
-  if (n < 2) return fail(c, TVK_ERR_STATE, "no transfer function");
-  const double fMaxValue = c->range_min > c->range_max ? (double)n : c->range_max;
-  const double resc = fMaxValue / double(n - 1);
-  uint64_t lim[4] = {0, 0, 0, 0};
-  if (two) tf2d_limits(c->tf2d.data(), c->tf2d_w, c->tf2d_h, lim);
-  else tf1d_limits(c->tf1d.data(), c->tf1d_n, lim);
-  const double fmin = double(lim[0]) * resc, fmax = double(lim[1]) * resc;
+  const bool two = c->P.mode == TVK_RM_2DTRANS, iso = c->P.mode == TVK_RM_ISOSURFACE;
+  double fmin = 0.0, fmax = 0.0;
+  if (iso) {
+    // ContainsData for RM_ISOSURFACE (GLVolumePool.cpp:983-985): min <= isovalue <= max
+    fmin = fmax = (double)c->P.isovalue;
+  } else {
+    const int n = two ? c->tf2d_w : c->tf1d_n;
+    if (n < 2) return fail(c, TVK_ERR_STATE, "no transfer function");
+    const double fMaxValue = c->range_min > c->range_max ? (double)n : c->range_max;
+    const double resc = fMaxValue / double(n - 1);
+    uint64_t lim[4] = {0, 0, 0, 0};
+    if (two) tf2d_limits(c->tf2d.data(), c->tf2d_w, c->tf2d_h, lim);
+    else tf1d_limits(c->tf1d.data(), c->tf1d_n, lim);
+    fmin = double(lim[0]) * resc;
+    fmax = double(lim[1]) * resc;
+  }
   meta.assign(L.toc.size(), BI_MISSING);
   for (int l = 0; l < G.nlod; ++l)
     for (uint64_t z = 0; z < G.ilayout[l][2]; ++z)
@@ -156,7 +162,12 @@ int tvk_leap_render(tvk_ctx* c, int first, float* host_out, tvk_leap_stats* st) 
   const Mat4 emm = inverse(V.modelview()) * translation(-0.0f, -0.0f, -0.0f) *
                    scaling(1.0f / ext.x, 1.0f / ext.y, 1.0f / ext.z) * translation(0.5f, 0.5f, 0.5f);
   const Mat4 ip = inverse(V.proj);
-  for (int i = 0; i < 16; ++i) { a.emm[i] = emm.a[i]; a.inv_proj[i] = ip.a[i]; }
+  // mModelToEye = emm^-1 and mModelViewIT = (modelview^-1)^T as row-vector maps
+  // (GLGridLeaper.cpp:720-723)
+  const Mat4 mte = inverse(emm), mvi = inverse(V.modelview());
+  for (int i = 0; i < 16; ++i) { a.emm[i] = emm.a[i]; a.inv_proj[i] = ip.a[i]; a.mte[i] = mte.a[i]; }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a.nit[3 * i + j] = mvi.a[4 * j + i];   // n'_j = sum_i n_i (M^-1)^T[i][j]
   for (int l = 0; l < G.nlod; ++l) {
     for (int k = 0; k < 3; ++k) { a.layout[l][k] = G.layout[l][k]; a.ilayout[l][k] = G.ilayout[l][k]; }
     a.lodoff[l] = G.lodoff[l];

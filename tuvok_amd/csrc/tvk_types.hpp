@@ -197,6 +197,8 @@ struct LeapArgs {
   float* out;                  // RGBA32F accumulated colour
   unsigned long long* counter; // samples (or nullptr)
   float emm[16];               // eye -> normalised volume coordinates
+  float mte[16];               // normalised volume coordinates -> eye (isosurface hit)
+  float nit[9];                // inverse transpose of the modelview, row vectors (isosurface normal)
   float inv_proj[16];
   float layout[LEAP_MAX_LOD][3];     // float brick layout per level
   uint32_t ilayout[LEAP_MAX_LOD][3]; // integer brick layout per level
