@@ -191,6 +191,13 @@ int tvk_touched_bytes(tvk_ctx* ctx, uint64_t* bytes, uint64_t* bricks, uint64_t*
                       uint64_t* unique_bricks);
 /* Device pointer of the last rendered frame(s) (for zero-copy consumers). */
 int tvk_frame_device_ptr(tvk_ctx* ctx, void** dptr);
+/* The last rendered frame(s) as the window shows them: drawn over the
+ * background (a vertical ramp from bg_top to bg_bottom; equal colours: a flat
+ * clear) with (SRC_ALPHA, ONE_MINUS_SRC_ALPHA) into 8 bits per channel, GL row
+ * order.  Replaces GLRenderer::CopyImageToDisplayBuffer + ClearColorBuffer /
+ * DrawBackGradient (GLRenderer.cpp:489-510, :1539-1579, :1640-1680).
+ * out: views x H x W x 4 bytes (RGBA). */
+int tvk_display(tvk_ctx* ctx, const float bg_top[3], const float bg_bottom[3], uint8_t* out, size_t bytes);
 
 /* ---- timing on the context stream ------------------------------------------- */
 int tvk_event_record(tvk_ctx* ctx, int slot);          /* slot in [0, 6); 6 and 7 bracket the last launch */

@@ -253,3 +253,29 @@ def test_full_size_bricking_properties(ren):
                 assert ren.MaxMinForKey(toc) == (float(want.min()), float(want.max()))
                 off += b ** 3
     assert off == pool.size
+
+
+def test_display_over_background_closed_form():
+    """CopyImageToDisplayBuffer over ClearColorBuffer / DrawBackGradient: the
+    premultiplied frame blended with (SRC_ALPHA, ONE_MINUS_SRC_ALPHA) over a
+    vertical ramp into 8 bits, recomputed in float32 with numpy."""
+    from tuvok_amd import workloads
+    r = workloads.setup(workloads.BLOBS_256, width=96, height=64)
+    rots = [rotation_y(0.2), rotation_y(1.3)]
+    frames = r.PaintViews(rots)
+    top = np.array([0.2, 0.3, 0.9], np.float32)
+    bot = np.array([0.9, 0.6, 0.1], np.float32)
+    got = r.Display(top, bot)
+    H, W = frames.shape[1:3]
+    t = ((np.arange(H, dtype=np.float32) + np.float32(0.5)) / np.float32(H))[None, :, None, None]
+    bg = bot + t * (top - bot)
+    a = frames[..., 3:4]
+    rgb = frames[..., :3] * a + bg * (np.float32(1) - a)
+    q = lambda x: np.floor(np.clip(x, 0, 1).astype(np.float32) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+    want = np.concatenate([q(rgb), q(a * a)], axis=-1)
+    assert got.shape == want.shape and np.array_equal(got, want)
+    assert (frames[..., 3] > 0.1).any() and (frames[..., 3] == 0).any()
+    # one colour: a flat clear
+    flat = r.Display((0.5, 0.5, 0.5))
+    assert np.array_equal(flat[..., :3][frames[..., 3] == 0], np.full(((frames[..., 3] == 0).sum(), 3), 128, np.uint8))
+    r.close()

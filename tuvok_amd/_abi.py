@@ -17,7 +17,7 @@ EXPORTS = [
     "tvk_brick_md", "tvk_brick_minmax", "tvk_get_brick", "tvk_get_pool", "tvk_max_gradient",
     "tvk_set_tf1d", "tvk_set_tf2d", "tvk_set_view_matrices", "tvk_set_camera", "tvk_set_rotation",
     "tvk_get_view_matrices", "tvk_set_params", "tvk_get_params", "tvk_current_lod", "tvk_plan",
-    "tvk_render", "tvk_render_views", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_cell_grid", "tvk_frame_device_ptr",
+    "tvk_render", "tvk_render_views", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_cell_grid", "tvk_frame_device_ptr", "tvk_display",
     "tvk_event_record", "tvk_event_elapsed", "tvk_synchronize",
     "tvk_default_tuning", "tvk_batch_create", "tvk_batch_destroy", "tvk_batch_last_error", "tvk_batch_clear",
     "tvk_batch_add", "tvk_batch_set_tuning", "tvk_batch_get_tuning", "tvk_batch_set_count_samples",
@@ -117,6 +117,7 @@ def load(path=LIB_PATH):
         "tvk_set_count_samples": (C.c_int, [vp, C.c_int]),
         "tvk_touched_bytes": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
         "tvk_cell_grid": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_uint64, u64p]),
+        "tvk_display": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_size_t]),
         "tvk_frame_device_ptr": (C.c_int, [vp, C.POINTER(vp)]),
         "tvk_event_record": (C.c_int, [vp, C.c_int]),
         "tvk_event_elapsed": (C.c_int, [vp, C.c_int, C.c_int, fp]),

@@ -24,6 +24,7 @@ SOURCES = [
     ("kernels/generate.hip", []),
     ("kernels/stepnet.hip", []),
     ("kernels/leap.hip", ["-ffp-contract=off"]),
+    ("kernels/display.hip", ["-ffp-contract=off"]),
     ("host/host.cpp", ["-ffp-contract=off"]),
     ("api.cpp", ["-ffp-contract=off"]),
     ("batch.cpp", ["-ffp-contract=off"]),

@@ -1126,6 +1126,23 @@ int tvk_touched_bytes(tvk_ctx* c, uint64_t* bytes, uint64_t* bricks, uint64_t* u
   return TVK_OK;
 }
 
+int tvk_display(tvk_ctx* c, const float bg_top[3], const float bg_bottom[3], uint8_t* out, size_t bytes) {
+  if (!c || !bg_top || !bg_bottom || !out) return TVK_ERR_ARG;
+  if (c->last_views < 1 || !c->d_out) return fail(c, TVK_ERR_STATE, "no rendered frame");
+  const size_t px = (size_t)c->last_w * c->last_h * (size_t)c->last_views;
+  if (bytes != px * 4) return fail(c, TVK_ERR_ARG, "output size mismatch");
+  HIPCHK(c, hipSetDevice(c->device));
+  uint8_t* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, bytes));
+  hipError_t e = launch_display(c->d_out, c->last_w, c->last_h, (uint32_t)c->last_views, bg_top, bg_bottom, d,
+                                c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hipfail(c, "display", e);
+  return TVK_OK;
+}
+
 int tvk_frame_device_ptr(tvk_ctx* c, void** p) {
   if (!c || !p) return TVK_ERR_ARG;
   *p = c->d_out;

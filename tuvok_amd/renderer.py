@@ -291,6 +291,7 @@ class HIPRaycaster:
     def Paint(self, download=True):
         out = np.empty((self._h_px, self._w, 4), np.float32) if download else None
         self._chk(self._L.tvk_render(self._h, _fp(out) if download else None, C.byref(self.last_stats)))
+        self._last_views = 1
         return out
 
     def PaintViews(self, rotations, download=True):
@@ -299,6 +300,7 @@ class HIPRaycaster:
         out = np.empty((n, self._h_px, self._w, 4), np.float32) if download else None
         self._chk(self._L.tvk_render_views(self._h, n, _fp(rots), _fp(out) if download else None,
                                            C.byref(self.last_stats)))
+        self._last_views = n
         return out
 
     # -- ray-guided traversal (GLGridLeaper) ----------------------------------------
@@ -315,6 +317,7 @@ class HIPRaycaster:
         out = np.empty((self._h_px, self._w, 4), np.float32)
         st = _abi.LeapStats()
         self._chk(self._L.tvk_leap_render(self._h, int(bool(first)), _fp(out), C.byref(st)))
+        self._last_views = 1
         return out, st
 
     def LeapMissing(self):
@@ -377,6 +380,18 @@ class HIPRaycaster:
                                                     _fp(out), C.byref(n), C.byref(total)))
         self.last_segments_total = int(total.value)
         return out[:n.value]
+
+    def Display(self, bg_top=(0.0, 0.0, 0.0), bg_bottom=None):
+        """The last rendered frame(s) as the window shows them: over the
+        background ramp (flat when bg_bottom is None), 8 bits per channel,
+        GL row order; (views, H, W, 4) uint8, or (H, W, 4) for one view."""
+        top = np.ascontiguousarray(bg_top, np.float32)
+        bot = np.ascontiguousarray(bg_top if bg_bottom is None else bg_bottom, np.float32)
+        nv = max(getattr(self, "_last_views", 1), 1)
+        out = np.zeros((nv, self._h_px, self._w, 4), np.uint8)
+        self._chk(self._L.tvk_display(self._h, _fp(top), _fp(bot), out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                      out.nbytes))
+        return out[0] if nv == 1 else out
 
     def CellGrid(self):
         """The dataset's cell grid (uint32 words (max << 16) | min), TOC brick
