@@ -104,6 +104,13 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
       if (e != hipSuccess) { rc = hipfail(c, "fill_overlap", e); break; }
     }
   }
+  if (rc == TVK_OK) {
+    // rows repeat the next block's first voxels (see tvk_types.hpp)
+    uint32_t maxv = 0;
+    for (const TocBrick& t : L.toc) maxv = std::max<uint32_t>(maxv, (uint32_t)t.count);
+    hipError_t e = launch_extend(c->d_pool, bpv, d_gb, (uint32_t)gb.size(), maxv, c->stream);
+    if (e != hipSuccess) rc = hipfail(c, "extend", e);
+  }
   uint32_t* d_mm = nullptr;
   if (rc == TVK_OK) {
     hipError_t e = hipMalloc(&d_mm, 2 * gb.size() * sizeof(uint32_t));

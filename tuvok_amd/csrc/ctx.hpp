@@ -35,6 +35,8 @@ hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bri
 
 
 namespace tvk {
+hipError_t launch_extend(void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,
+                         hipStream_t st);
 hipError_t launch_display(const float* frames, uint32_t W, uint32_t H, uint32_t nviews, const float top[3],
                           const float bottom[3], uint8_t* out, hipStream_t st);
 hipError_t launch_cells(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t* d_base, uint32_t nb,

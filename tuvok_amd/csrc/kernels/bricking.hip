@@ -20,15 +20,34 @@ namespace tvk {
 
 template <typename T> __device__ __forceinline__ int lx_of() { return sizeof(T) == 1 ? 3 : 2; }
 
-// decode a storage index of the blocked brick layout into voxel coordinates
+// decode a storage index of the blocked brick layout into voxel coordinates;
+// the second half of a row decodes to the next block's voxels (callers clamp
+// coordinates to the brick, which gives the padding its repeated last voxel)
 __device__ __forceinline__ void unswz(uint64_t i, uint32_t nbx, uint32_t nby, int lx, uint32_t& x, uint32_t& y,
                                       uint32_t& z) {
-  const uint64_t blk = i >> (lx + 4);
-  const uint32_t r = (uint32_t)(i & ((1u << (lx + 4)) - 1u));
-  x = (uint32_t)(blk % nbx) << lx | (r & ((1u << lx) - 1u));
+  const uint64_t blk = i >> (lx + 5);
+  const uint32_t r = (uint32_t)(i & ((1u << (lx + 5)) - 1u));
+  x = ((uint32_t)(blk % nbx) << lx) + (r & ((2u << lx) - 1u));
   const uint64_t q = blk / nbx;
-  y = (uint32_t)(q % nby) << 2 | ((r >> lx) & 3u);
-  z = (uint32_t)(q / nby) << 2 | (r >> (lx + 2));
+  y = (uint32_t)(q % nby) << 2 | ((r >> (lx + 1)) & 3u);
+  z = (uint32_t)(q / nby) << 2 | (r >> (lx + 3));
+}
+
+// Second half of every row := the voxels it repeats (the next block's row, or
+// the clamped last voxel), after the overlap fill changed some of them.
+template <typename T>
+__global__ __launch_bounds__(256) void extend_rows(T* __restrict__ pool, const GatherBrick* __restrict__ bricks) {
+  const GatherBrick b = bricks[blockIdx.y];
+  const int lx = lx_of<T>();
+  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2);
+  const uint64_t half = padded_count(b.n, lx) >> 1;   // one half-row of every row
+  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < half; j += (uint64_t)gridDim.x * 256) {
+    const uint64_t i = ((j >> lx) << (lx + 1)) + (1u << lx) + (j & ((1u << lx) - 1u));   // an extension slot
+    uint32_t x, y, z;
+    unswz(i, nbx, nby, lx, x, y, z);
+    x = min(x, b.n[0] - 1); y = min(y, b.n[1] - 1); z = min(z, b.n[2] - 1);
+    pool[b.offset + i] = pool[b.offset + swz(x, y, z, nbx, nby, lx)];
+  }
 }
 
 // grid: x = chunks of the brick's storage, y = brick.  Threads walk the brick in
@@ -367,6 +386,21 @@ hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks,
     else
       hipLaunchKernelGGL(brick_minmax<uint16_t>, dim3(cnt), dim3(256), 0, st, (const uint16_t*)pool, d_bricks + first,
                          out + 2 * first);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_extend(void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,
+                         hipStream_t st) {
+  uint32_t gx = (uint32_t)((max_vox / 2 + 255) / 256);
+  if (gx > 1024) gx = 1024;
+  if (gx == 0) gx = 1;
+  for (uint32_t first = 0; first < nb; first += 65535u) {
+    const uint32_t cnt = std::min(nb - first, 65535u);
+    if (bpv == 1)
+      hipLaunchKernelGGL(extend_rows<uint8_t>, dim3(gx, cnt), dim3(256), 0, st, (uint8_t*)pool, d_bricks + first);
+    else
+      hipLaunchKernelGGL(extend_rows<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (uint16_t*)pool, d_bricks + first);
   }
   return hipGetLastError();
 }

@@ -44,22 +44,28 @@ template <> struct Unorm<uint16_t> {
 };
 
 // Element offset parts of the blocked brick layout (see tvk_types.hpp): the
-// offset of voxel (x,y,z) is ox(x) + oy(y) + oz(z).
+// offset of voxel (x,y,z) is ox(x) + oy(y) + oz(z); rows hold 2 << LX elements.
 template <typename T> struct Blk { static constexpr int LX = sizeof(T) == 1 ? 3 : 2; };
 template <typename T>
 __device__ __forceinline__ uint32_t ox(int x) {
   constexpr int L = Blk<T>::LX;
-  return ((uint32_t)(x >> L) << (L + 4)) + (uint32_t)(x & ((1 << L) - 1));
+  return ((uint32_t)(x >> L) << (L + 5)) + (uint32_t)(x & ((1 << L) - 1));
+}
+// start of the row of the block that holds voxel x (its own voxels from there)
+template <typename T>
+__device__ __forceinline__ uint32_t ox_row(int x) {
+  constexpr int L = Blk<T>::LX;
+  return (uint32_t)(x >> L) << (L + 5);
 }
 template <typename T>
 __device__ __forceinline__ uint32_t oy(int y, uint32_t nbx) {
   constexpr int L = Blk<T>::LX;
-  return (((uint32_t)(y >> 2) * nbx) << (L + 4)) + ((uint32_t)(y & 3) << L);
+  return (((uint32_t)(y >> 2) * nbx) << (L + 5)) + ((uint32_t)(y & 3) << (L + 1));
 }
 template <typename T>
 __device__ __forceinline__ uint32_t oz(int z, uint32_t nbxy) {
   constexpr int L = Blk<T>::LX;
-  return (((uint32_t)(z >> 2) * nbxy) << (L + 4)) + ((uint32_t)(z & 3) << (L + 2));
+  return (((uint32_t)(z >> 2) * nbxy) << (L + 5)) + ((uint32_t)(z & 3) << (L + 3));
 }
 
 __device__ __forceinline__ V3 reflect(V3 I, V3 N) { return I - N * (2.0f * dot(N, I)); }

@@ -225,6 +225,147 @@ struct CrossSampler {
 };
 
 
+// Voxels of one 16-byte row of the blocked layout (see tvk_types.hpp): the row
+// of the block that holds voxel x starts at ox_row(x); element e of the row is
+// voxel (x & ~(W-1)) + e for e < 2W.  row4 returns elements s..s+3, row2
+// elements s..s+1, as stored integers.
+template <typename T>
+__device__ __forceinline__ void row4(uint4 d, int s, uint32_t o[4]) {
+  if (sizeof(T) == 2) {
+    const bool q = (s >> 1) != 0;
+    const uint32_t sh = (uint32_t)(s & 1) * 2u;
+    const uint32_t d0 = q ? d.y : d.x, d1 = q ? d.z : d.y, d2 = q ? d.w : d.z;
+    const uint32_t lo = __builtin_amdgcn_alignbyte(d1, d0, sh), hi = __builtin_amdgcn_alignbyte(d2, d1, sh);
+    o[0] = lo & 0xffffu; o[1] = lo >> 16; o[2] = hi & 0xffffu; o[3] = hi >> 16;
+  } else {
+    const bool q = (s >> 2) != 0;
+    const uint32_t w = __builtin_amdgcn_alignbyte(q ? d.z : d.y, q ? d.y : d.x, (uint32_t)(s & 3));
+    o[0] = w & 0xffu; o[1] = (w >> 8) & 0xffu; o[2] = (w >> 16) & 0xffu; o[3] = w >> 24;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void row2(uint4 d, int s, uint32_t o[2]) {
+  if (sizeof(T) == 2) {
+    const bool q = (s >> 1) != 0;
+    const uint32_t w = __builtin_amdgcn_alignbyte(q ? d.z : d.y, q ? d.y : d.x, (uint32_t)(s & 1) * 2u);
+    o[0] = w & 0xffffu; o[1] = w >> 16;
+  } else {
+    const bool q = (s >> 2) != 0;
+    const uint32_t w = __builtin_amdgcn_alignbyte(q ? d.z : d.y, q ? d.y : d.x, (uint32_t)(s & 3));
+    o[0] = w & 0xffu; o[1] = (w >> 8) & 0xffu;
+  }
+}
+
+// Interior form (every lane's 4x4x4 neighbourhood inside the brick): the four
+// central x-rows (x0-1 .. x0+2 at y0..y0+1, z0..z0+1) are one 16-byte load
+// each and give the centre cube and the x arms; each y or z arm (x0, x0+1) is
+// one more load.  12 loads of 16 bytes instead of 32 two-byte gathers; the
+// voxels, weights and float operations are those of the general form.
+template <typename T>
+struct CrossSampler<T, true> {
+  const T* __restrict__ d;
+  int n[3];
+  bool edge;
+  V3 p;
+  int ic[3];
+  float fc[3];
+  float c[2][2][2];
+  float xl[2][2], xr[2][2];   // x arms at (y0+j, z0+k)
+  uint32_t nbx, nbxy;
+
+  __device__ __forceinline__ uint4 row(int x, int y, int z) const {
+    const uint32_t off = (ox_row<T>(x) + oy<T>(y, nbx) + oz<T>(z, nbxy)) * (uint32_t)sizeof(T);
+    return *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(d) + off);
+  }
+
+  __device__ __forceinline__ float value(const T* __restrict__ dd, int nx, int ny, int nz, bool e, V3 pos) {
+    d = dd; n[0] = nx; n[1] = ny; n[2] = nz; edge = e; p = pos;
+    const float pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float s = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = s * (float)n[a] - 0.5f, fl = floorf(u);
+      ic[a] = (int)fl;
+      fc[a] = u - fl;
+    }
+    nbx = nblocks((uint32_t)nx, Blk<T>::LX);
+    nbxy = nbx * nblocks((uint32_t)ny, 2);
+    const int xs = ic[0] - 1, sx = xs & ((1 << Blk<T>::LX) - 1);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        uint32_t v[4];
+        row4<T>(row(xs, ic[1] + j, ic[2] + k), sx, v);
+        xl[j][k] = (float)v[0];
+        c[0][j][k] = (float)v[1];
+        c[1][j][k] = (float)v[2];
+        xr[j][k] = (float)v[3];
+      }
+    return tri<T>(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1],
+                  fc[0], fc[1], fc[2]);
+  }
+
+  __device__ __forceinline__ V3 gradient(V3 h) const {
+    const float pm[3] = {p.x + -h.x, p.y + -h.y, p.z + -h.z};
+    const float pp[3] = {p.x + h.x, p.y + h.y, p.z + h.z};
+    int im[3], iq[3];
+    float fm[3], fq[3];
+    bool shared = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float s = fminf(fmaxf(pm[a], 0.0f), 1.0f), u = s * (float)n[a] - 0.5f, fl = floorf(u);
+      im[a] = (int)fl; fm[a] = u - fl;
+      s = fminf(fmaxf(pp[a], 0.0f), 1.0f); u = s * (float)n[a] - 0.5f; fl = floorf(u);
+      iq[a] = (int)fl; fq[a] = u - fl;
+      shared = shared && im[a] == ic[a] - 1 && iq[a] == ic[a] + 1;
+    }
+    if (!shared) {
+      // rare (a coordinate rounds across a voxel boundary): the six taps one at a time
+      float t[6];
+#pragma unroll 1
+      for (int k = 0; k < 6; ++k) {
+        const int a = k >> 1;
+        const float off = (k & 1) ? (a == 1 ? h.y : (a == 0 ? -h.x : -h.z)) : (a == 1 ? -h.y : (a == 0 ? h.x : h.z));
+        V3 q = p;
+        if (a == 0) q = p + mk(off, 0, 0); else if (a == 1) q = p + mk(0, off, 0); else q = p + mk(0, 0, off);
+        t[k] = sample_brick<T>(d, n[0], n[1], n[2], edge, q);
+      }
+      return mk(t[1] - t[0], t[2] - t[3], t[5] - t[4]) / 2.0f;
+    }
+    const int sx = ic[0] & ((1 << Blk<T>::LX) - 1);
+    float yl[2][2], yr[2][2], zl[2][2], zr[2][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      uint32_t a[2], b[2];
+      row2<T>(row(ic[0], ic[1] - 1, ic[2] + k), sx, a);
+      row2<T>(row(ic[0], ic[1] + 2, ic[2] + k), sx, b);
+      yl[0][k] = (float)a[0]; yl[1][k] = (float)a[1];
+      yr[0][k] = (float)b[0]; yr[1][k] = (float)b[1];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      uint32_t a[2], b[2];
+      row2<T>(row(ic[0], ic[1] + j, ic[2] - 1), sx, a);
+      row2<T>(row(ic[0], ic[1] + j, ic[2] + 2), sx, b);
+      zl[0][j] = (float)a[0]; zl[1][j] = (float)a[1];
+      zr[0][j] = (float)b[0]; zr[1][j] = (float)b[1];
+    }
+    const float fxm = tri<T>(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
+                             fm[0], fc[1], fc[2]);
+    const float fxp = tri<T>(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
+                             fq[0], fc[1], fc[2]);
+    const float fym = tri<T>(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
+                             fc[0], fm[1], fc[2]);
+    const float fyp = tri<T>(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
+                             fc[0], fq[1], fc[2]);
+    const float fzm = tri<T>(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
+                             fc[0], fc[1], fm[2]);
+    const float fzp = tri<T>(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
+                             fc[0], fc[1], fq[2]);
+    return mk(fxm - fxp, fym - fyp, fzm - fzp) / 2.0f;
+  }
+};
+
 // normalize() with the hardware reciprocal square root (about 1 ulp; used
 // for shading directions only, never for positions, step counts or TF indices)
 __device__ __forceinline__ V3 normalize_rsq(V3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }

@@ -6,22 +6,27 @@
 
 namespace tvk {
 
-// In HBM a brick is stored as 128-byte blocks of 8x4x4 (uint8) or 4x4x4
-// (uint16) voxels, blocks in x-fastest order, each axis padded to whole
-// blocks with copies of the last voxel.  A ray bundle marching through a brick
-// then touches compact cache lines instead of one line per (y,z) row.  The
-// reference layout (x-fastest, unpadded) is what the download calls return.
+// In HBM a brick is stored as 256-byte blocks of 8x4x4 (uint8) or 4x4x4
+// (uint16) voxels, blocks in x-fastest order, each axis padded to whole blocks
+// with copies of the last voxel.  Every x-row of a block holds its own voxels
+// followed by the first voxels of the next block's row (the last block of a
+// row repeats the brick's last voxel): any 4 consecutive voxels in x then lie
+// in one 16-byte row, so the march reads a gradient cross with one load per
+// row.  A ray bundle marching through a brick touches compact cache lines
+// instead of one line per (y,z) row.  The reference layout (x-fastest,
+// unpadded) is what the download calls return.
 #ifndef TVK_HD
 #define TVK_HD __host__ __device__
 #endif
 TVK_HD inline int block_lx(int bpv) { return bpv == 1 ? 3 : 2; }   // log2 of the block width in x
 TVK_HD inline uint32_t nblocks(uint32_t n, int l) { return (n + (1u << l) - 1u) >> l; }
+// storage index of voxel (x,y,z): rows of 2 << lx elements, 4 rows per z slice
 TVK_HD inline uint64_t swz(uint32_t x, uint32_t y, uint32_t z, uint32_t nbx, uint32_t nby, int lx) {
-  return ((((uint64_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> lx)) << (lx + 4)) + ((uint64_t)(z & 3u) << (lx + 2)) +
-         ((uint64_t)(y & 3u) << lx) + (x & ((1u << lx) - 1u));
+  return ((((uint64_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> lx)) << (lx + 5)) + ((uint64_t)(z & 3u) << (lx + 3)) +
+         ((uint64_t)(y & 3u) << (lx + 1)) + (x & ((1u << lx) - 1u));
 }
 TVK_HD inline uint64_t padded_count(const uint32_t n[3], int lx) {
-  return (uint64_t)nblocks(n[0], lx) * nblocks(n[1], 2) * nblocks(n[2], 2) << (lx + 4);
+  return (uint64_t)nblocks(n[0], lx) * nblocks(n[1], 2) * nblocks(n[2], 2) << (lx + 5);
 }
 
 // One entry of a frame's visible, non-empty brick list, in compositing order.
