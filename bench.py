@@ -62,10 +62,10 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
 
 def learned_leg(ren, views, cfg, train_views=48, seed=123):
     """Learned step vs fixed step on the same volume, TF and views, at the
-    0.25-voxel reference step (srm 2): segment samples from `train_views`
-    random views at 480x270, training on the device, calibration on 3 other
-    views, then `views` timed with and without the proposal and compared pixel
-    by pixel."""
+    0.25-voxel reference step (srm 2): chunk samples from `train_views` random
+    views at 480x270, training on the device, calibration on 5 other views
+    (bound 0.8/255, the fixed march a candidate), then `views` timed with and
+    without the proposal and compared pixel by pixel."""
     from tuvok_amd.stepnet import calibrate, train_proposal
     from tuvok_amd.renderer import rotation_y
     rng = np.random.default_rng(seed)
@@ -78,11 +78,9 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     ren.Resize(W, H)
     train_s = time.perf_counter() - t0
     print(f"[bench] learned: {info} in {train_s:.1f} s", file=sys.stderr, flush=True)
-    prop, tried = calibrate(ren, net, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 3)])
+    prop, tried = calibrate(ren, net, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 5)], bound=0.8 / 255)
     for (p, e, ms) in sorted(tried, key=lambda t: t[2])[:6]:
         print(f"[bench] calibration: {p} max err {e * 255:.3f}/255 {ms:.2f} ms", file=sys.stderr, flush=True)
-    if prop is None:
-        return dict(error="no proposal setting met 0.9/255 on the calibration views", train=info)
     ren.SetStepProposal(None)
     fixed = ren.PaintViews(views)
     t_fixed = []
@@ -101,7 +99,7 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     # the chunked march with the exact skip only (multiplier 1, no network
     # decision): what the method gives without the network
     exact, ems = timed(dict(margin=1e9, skip_threshold=1.0, skip_bound=0.0))
-    learned, lms = timed(prop)
+    learned, lms = timed(prop)   # prop None: the calibration kept the fixed march
     ren.SetStepProposal(None)
     ren.SetSampleRateModifier(1.0)
     d = np.abs(learned - fixed)
@@ -111,7 +109,8 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
                 exact_skip_fps=round(len(views) / (ems / 1e3), 2),
                 exact_skip_max_err=float(np.abs(exact - fixed).max()),
                 max_err=float(d.max()), mean_err=float(d.mean()), max_err_x255=round(float(d.max()) * 255, 3),
-                proposal=prop, train_s=round(train_s, 1), train=info)
+                proposal=prop if prop is not None else "fixed march (no proposal setting faster within the bound)",
+                train_s=round(train_s, 1), train=info)
 
 
 PROFILE = os.path.join(ROOT, "profiles", "march_fbm1024_lit2dtf_1080p_x16.md")

@@ -145,10 +145,13 @@ def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.5, 1.0, 1.5,
               skip_bound=0.0):
     """The most aggressive proposal setting (least device time) whose frames stay
     within `bound` of the fixed-step frames on the given views (fallback 0: no
-    fallback).  Returns (setting or None, [(setting, max error, ms)])."""
+    fallback).  The fixed march itself is a candidate: when no setting is faster
+    within the bound the result is None (march the fixed step).  Returns
+    (setting or None, [(setting, max error, ms)])."""
     ren.SetStepProposal(None)
     refs = ren.PaintViews(rotations)
-    tried, best = [], None
+    ren.PaintViews(rotations, download=False)
+    tried, best = [(None, 0.0, float(ren.last_stats.ms))], (None, float(ren.last_stats.ms))
     for fb in fallbacks:
         for m in margins:
             for s in skips:
@@ -161,4 +164,4 @@ def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.5, 1.0, 1.5,
                 if err <= bound and (best is None or ms < best[1]):
                     best = (prop, ms)
     ren.SetStepProposal(None)
-    return (best[0] if best else None), tried
+    return best[0], tried
