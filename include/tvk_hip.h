@@ -135,6 +135,10 @@ int tvk_generate_volume(tvk_ctx* ctx, int kind, uint64_t seed, int dtype, const 
  * keeps its own TF, parameters and camera.  GPUMemMan shares one dataset's
  * textures between renderers the same way (GPUMemMan.cpp:846-996). */
 int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src);
+/* Re-bricks the dataset from its kept linear volume with another brick size,
+ * overlap and level count (ExtendedOctreeConverter::Convert with new
+ * parameters, on the device); used by the brick-size autotuner. */
+int tvk_rebrick(tvk_ctx* ctx, const uint32_t brick[3], uint32_t overlap, int max_lods);
 int tvk_download_volume(tvk_ctx* ctx, void* host_out, size_t bytes);
 int tvk_dataset_info(tvk_ctx* ctx, tvk_dataset_desc* out);
 int tvk_lod_info(tvk_ctx* ctx, int lod, uint64_t size[3], uint64_t bricks[3], uint64_t* offset,

@@ -13,7 +13,7 @@ GEN_BLOBS, GEN_FBM_SHELLS, GEN_ELLIPSOID_SDF, GEN_PHANTOM = 1, 2, 3, 4
 # every symbol include/tvk_hip.h declares
 EXPORTS = [
     "tvk_create", "tvk_destroy", "tvk_last_error", "tvk_default_params", "tvk_load_volume",
-    "tvk_generate_volume", "tvk_share_dataset", "tvk_download_volume", "tvk_dataset_info", "tvk_lod_info", "tvk_brick_info",
+    "tvk_generate_volume", "tvk_share_dataset", "tvk_rebrick", "tvk_download_volume", "tvk_dataset_info", "tvk_lod_info", "tvk_brick_info",
     "tvk_brick_md", "tvk_brick_minmax", "tvk_get_brick", "tvk_get_pool", "tvk_max_gradient",
     "tvk_set_tf1d", "tvk_set_tf2d", "tvk_set_view_matrices", "tvk_set_camera", "tvk_set_rotation",
     "tvk_get_view_matrices", "tvk_set_params", "tvk_get_params", "tvk_current_lod", "tvk_plan",
@@ -117,6 +117,7 @@ def load(path=LIB_PATH):
         "tvk_set_count_samples": (C.c_int, [vp, C.c_int]),
         "tvk_touched_bytes": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
         "tvk_cell_grid": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_uint64, u64p]),
+        "tvk_rebrick": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_uint32, C.c_int]),
         "tvk_display": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_size_t]),
         "tvk_frame_device_ptr": (C.c_int, [vp, C.POINTER(vp)]),
         "tvk_event_record": (C.c_int, [vp, C.c_int]),

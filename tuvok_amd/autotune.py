@@ -1,0 +1,29 @@
+"""Brick-size autotuner: re-bricks a renderer's dataset on the device with each
+candidate stored brick size and keeps the fastest for the given views.
+
+The tile shape and grid size of batches are tuned by RenderBatch.Autotune
+(tvk_batch_autotune); the step schedule of the learned march by
+stepnet.calibrate.  Brick size is a property of the dataset, so it is tuned
+here, on the renderer that owns the dataset (its linear volume must be kept).
+"""
+import numpy as np
+
+DEFAULT_BRICKS = (34, 66, 130, 258)   # stored sizes: cores 32, 64, 128, 256 + 1-voxel overlap
+
+
+def tune_brick_size(ren, rotations, candidates=DEFAULT_BRICKS, overlap=1, max_lods=1, reps=3):
+    """Renders `rotations` with every candidate brick size (cubic, stored size
+    incl. overlap) and re-bricks with the fastest.  Returns (best size,
+    {size: median ms})."""
+    times = {}
+    for b in candidates:
+        ren.Rebrick((b, b, b), overlap, max_lods)
+        ren.PaintViews(rotations, download=False)   # warm-up (plans, uploads)
+        ms = []
+        for _ in range(reps):
+            ren.PaintViews(rotations, download=False)
+            ms.append(ren.last_stats.ms)
+        times[b] = float(np.median(ms))
+    best = min(times, key=times.get)
+    ren.Rebrick((best, best, best), overlap, max_lods)
+    return best, times

@@ -348,6 +348,26 @@ int tvk_generate_volume(tvk_ctx* c, int kind, uint64_t seed, int dtype, const ui
   return rc;
 }
 
+int tvk_rebrick(tvk_ctx* c, const uint32_t brick[3], uint32_t overlap, int max_lods) {
+  if (!c || !brick) return TVK_ERR_ARG;
+  if (!c->has_ds || !c->d_lin) return fail(c, TVK_ERR_STATE, "rebricking needs the linear volume (keep_linear)");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the linear volume outlives the old pool (contexts sharing the old pool keep it)
+  std::shared_ptr<void> lin = c->lin_hold;
+  void* d_lin = c->d_lin;
+  const size_t lin_bytes = c->lin_bytes;
+  const uint64_t dims[3] = {c->L.dims[0], c->L.dims[1], c->L.dims[2]};
+  int rc = begin_dataset(c, c->L.bpv, dims, brick, overlap, c->clamp, max_lods);
+  c->lin_hold = lin;
+  c->d_lin = d_lin;
+  c->lin_bytes = lin_bytes;
+  c->leap_resident.clear();
+  if (!rc) rc = build_pool(c, c->d_lin);
+  if (rc) free_dataset(c);
+  return rc;
+}
+
 int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src) {
   if (!dst || !src || dst == src) return TVK_ERR_ARG;
   if (!src->has_ds) return fail(dst, TVK_ERR_STATE, "source context has no dataset");

@@ -94,6 +94,11 @@ class HIPRaycaster:
                                               b.ctypes.data_as(C.POINTER(C.c_uint32)), overlap, int(clamp_to_edge),
                                               max_lods, int(keep_linear)))
 
+    def Rebrick(self, brick, overlap=1, max_lods=0):
+        """Re-brick the dataset from its kept linear volume (keep_linear=True)."""
+        b = np.array(brick, np.uint32)
+        self._chk(self._L.tvk_rebrick(self._h, b.ctypes.data_as(C.POINTER(C.c_uint32)), int(overlap), int(max_lods)))
+
     def ShareDataset(self, other):
         """Render `other`'s dataset (its HBM brick pool is shared, not copied)."""
         self._chk(self._L.tvk_share_dataset(self._h, other._h))
