@@ -63,8 +63,8 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
 def learned_leg(ren, views, cfg, train_views=48, seed=123):
     """Learned step vs fixed step on the same volume, TF and views, at the
     0.25-voxel reference step (srm 2): chunk samples from `train_views` random
-    views at 480x270, training on the device, calibration on 5 other views
-    (bound 0.8/255, the fixed march a candidate), then `views` timed with and
+    views at 480x270, training on the device, calibration on 10 other views
+    (bound 0.75/255, the fixed march a candidate), then `views` timed with and
     without the proposal and compared pixel by pixel."""
     from tuvok_amd.stepnet import calibrate, train_proposal
     from tuvok_amd.renderer import rotation_y
@@ -78,7 +78,7 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     ren.Resize(W, H)
     train_s = time.perf_counter() - t0
     print(f"[bench] learned: {info} in {train_s:.1f} s", file=sys.stderr, flush=True)
-    prop, tried = calibrate(ren, net, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 5)], bound=0.8 / 255)
+    prop, tried = calibrate(ren, net, [rotation_y(a) for a in rng.uniform(0, 2 * np.pi, 10)], bound=0.75 / 255)
     for (p, e, ms) in sorted(tried, key=lambda t: t[2])[:6]:
         print(f"[bench] calibration: {p} max err {e * 255:.3f}/255 {ms:.2f} ms", file=sys.stderr, flush=True)
     ren.SetStepProposal(None)
@@ -87,8 +87,8 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     for _ in range(3):
         ren.PaintViews(views, download=False)
         t_fixed.append(ren.last_stats.ms)
-    def timed(prop):
-        ren.SetStepProposal(None if prop is None else net, **(prop or {}))
+    def timed(prop, use_net=True):
+        ren.SetStepProposal(net if (prop is not None and use_net) else None, **(prop or {}))
         img = ren.PaintViews(views)
         ts = []
         for _ in range(3):
@@ -97,8 +97,8 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
         return img, float(np.median(ts))
 
     # the chunked march with the exact skip only (multiplier 1, no network
-    # decision): what the method gives without the network
-    exact, ems = timed(dict(margin=1e9, skip_threshold=1.0, skip_bound=0.0))
+    # evaluated): what the method gives without the network
+    exact, ems = timed(dict(fixed_log2=0, skip_bound=0.0), use_net=False)
     learned, lms = timed(prop)   # prop None: the calibration kept the fixed march
     ren.SetStepProposal(None)
     ren.SetSampleRateModifier(1.0)

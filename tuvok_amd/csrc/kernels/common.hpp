@@ -51,12 +51,6 @@ __device__ __forceinline__ uint32_t ox(int x) {
   constexpr int L = Blk<T>::LX;
   return ((uint32_t)(x >> L) << (L + 5)) + (uint32_t)(x & ((1 << L) - 1));
 }
-// start of the row of the block that holds voxel x (its own voxels from there)
-template <typename T>
-__device__ __forceinline__ uint32_t ox_row(int x) {
-  constexpr int L = Blk<T>::LX;
-  return (uint32_t)(x >> L) << (L + 5);
-}
 template <typename T>
 __device__ __forceinline__ uint32_t oy(int y, uint32_t nbx) {
   constexpr int L = Blk<T>::LX;

@@ -225,42 +225,52 @@ struct CrossSampler {
 };
 
 
-// Voxels of one 16-byte row of the blocked layout (see tvk_types.hpp): the row
-// of the block that holds voxel x starts at ox_row(x); element e of the row is
-// voxel (x & ~(W-1)) + e for e < 2W.  row4 returns elements s..s+3, row2
-// elements s..s+1, as stored integers.
+// Runs of voxels along x of the blocked layout.  A row holds its block's W
+// voxels followed by the next block's first W (16 bytes), so voxels s .. s+m-1
+// of a row (s < W, s + m <= 2W) are contiguous.
+//   run4: voxels s .. s+3 of the row at byte offset r, from the dword-aligned
+//         12 (uint16) or 8 (uint8) bytes that start at the dword holding voxel s;
+//   run2: voxels x .. x+1 from byte offset r of voxel x itself, one 4-byte
+//         (uint16) or 2-byte (uint8) under-aligned load: the amdhsa code
+//         object runs with unaligned global access enabled, and the two voxels
+//         never leave their 16-byte row.
+// Measured on the lit 2D-TF 1024^3 uint16 workload (16 views, 1080p): all
+// twelve loads as 16-byte rows 74.9 ms; these forms 71.0 ms; under-aligned
+// 8-byte loads for the four central runs as well 76.9 ms.  Values are stored
+// integers.
+typedef uint32_t tvk_u32x3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+typedef uint32_t tvk_u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef uint32_t tvk_u32_a1 __attribute__((aligned(1)));
+typedef uint16_t tvk_u16_a1 __attribute__((aligned(1)));
 template <typename T>
-__device__ __forceinline__ void row4(uint4 d, int s, uint32_t o[4]) {
+__device__ __forceinline__ void run4(const uint8_t* __restrict__ b, uint32_t r, int s, uint32_t o[4]) {
   if (sizeof(T) == 2) {
-    const bool q = (s >> 1) != 0;
+    const tvk_u32x3_a4 w = *reinterpret_cast<const tvk_u32x3_a4*>(b + (r + 4u * (uint32_t)(s >> 1)));
     const uint32_t sh = (uint32_t)(s & 1) * 2u;
-    const uint32_t d0 = q ? d.y : d.x, d1 = q ? d.z : d.y, d2 = q ? d.w : d.z;
-    const uint32_t lo = __builtin_amdgcn_alignbyte(d1, d0, sh), hi = __builtin_amdgcn_alignbyte(d2, d1, sh);
+    const uint32_t lo = __builtin_amdgcn_alignbyte(w.y, w.x, sh), hi = __builtin_amdgcn_alignbyte(w.z, w.y, sh);
     o[0] = lo & 0xffffu; o[1] = lo >> 16; o[2] = hi & 0xffffu; o[3] = hi >> 16;
   } else {
-    const bool q = (s >> 2) != 0;
-    const uint32_t w = __builtin_amdgcn_alignbyte(q ? d.z : d.y, q ? d.y : d.x, (uint32_t)(s & 3));
-    o[0] = w & 0xffu; o[1] = (w >> 8) & 0xffu; o[2] = (w >> 16) & 0xffu; o[3] = w >> 24;
+    const tvk_u32x2_a4 w = *reinterpret_cast<const tvk_u32x2_a4*>(b + (r + 4u * (uint32_t)(s >> 2)));
+    const uint32_t v = __builtin_amdgcn_alignbyte(w.y, w.x, (uint32_t)(s & 3));
+    o[0] = v & 0xffu; o[1] = (v >> 8) & 0xffu; o[2] = (v >> 16) & 0xffu; o[3] = v >> 24;
   }
 }
 template <typename T>
-__device__ __forceinline__ void row2(uint4 d, int s, uint32_t o[2]) {
+__device__ __forceinline__ void run2(const uint8_t* __restrict__ b, uint32_t r, uint32_t o[2]) {
   if (sizeof(T) == 2) {
-    const bool q = (s >> 1) != 0;
-    const uint32_t w = __builtin_amdgcn_alignbyte(q ? d.z : d.y, q ? d.y : d.x, (uint32_t)(s & 1) * 2u);
+    const uint32_t w = *reinterpret_cast<const tvk_u32_a1*>(b + r);
     o[0] = w & 0xffffu; o[1] = w >> 16;
   } else {
-    const bool q = (s >> 2) != 0;
-    const uint32_t w = __builtin_amdgcn_alignbyte(q ? d.z : d.y, q ? d.y : d.x, (uint32_t)(s & 3));
-    o[0] = w & 0xffu; o[1] = (w >> 8) & 0xffu;
+    const uint32_t w = *reinterpret_cast<const tvk_u16_a1*>(b + r);
+    o[0] = w & 0xffu; o[1] = w >> 8;
   }
 }
 
 // Interior form (every lane's 4x4x4 neighbourhood inside the brick): the four
-// central x-rows (x0-1 .. x0+2 at y0..y0+1, z0..z0+1) are one 16-byte load
-// each and give the centre cube and the x arms; each y or z arm (x0, x0+1) is
-// one more load.  12 loads of 16 bytes instead of 32 two-byte gathers; the
-// voxels, weights and float operations are those of the general form.
+// central x-runs (x0-1 .. x0+2 at y0..y0+1, z0..z0+1) are one load each and
+// give the centre cube and the x arms; each y or z arm (x0, x0+1) is one more
+// load.  12 loads instead of 32 gathers; the voxels, weights and float
+// operations are those of the general form.
 template <typename T>
 struct CrossSampler<T, true> {
   const T* __restrict__ d;
@@ -273,10 +283,14 @@ struct CrossSampler<T, true> {
   float xl[2][2], xr[2][2];   // x arms at (y0+j, z0+k)
   uint32_t nbx, nbxy;
 
-  __device__ __forceinline__ uint4 row(int x, int y, int z) const {
-    const uint32_t off = (ox_row<T>(x) + oy<T>(y, nbx) + oz<T>(z, nbxy)) * (uint32_t)sizeof(T);
-    return *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(d) + off);
+  // byte offsets of voxel (x, y, z) and of the start of its row
+  __device__ __forceinline__ uint32_t vox(int x, int y, int z) const {
+    return (ox<T>(x) + oy<T>(y, nbx) + oz<T>(z, nbxy)) * (uint32_t)sizeof(T);
   }
+  __device__ __forceinline__ uint32_t rowo(int x, int y, int z) const {
+    return (ox<T>(x & ~((1 << Blk<T>::LX) - 1)) + oy<T>(y, nbx) + oz<T>(z, nbxy)) * (uint32_t)sizeof(T);
+  }
+  __device__ __forceinline__ const uint8_t* base() const { return reinterpret_cast<const uint8_t*>(d); }
 
   __device__ __forceinline__ float value(const T* __restrict__ dd, int nx, int ny, int nz, bool e, V3 pos) {
     d = dd; n[0] = nx; n[1] = ny; n[2] = nz; edge = e; p = pos;
@@ -295,7 +309,7 @@ struct CrossSampler<T, true> {
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         uint32_t v[4];
-        row4<T>(row(xs, ic[1] + j, ic[2] + k), sx, v);
+        run4<T>(base(), rowo(xs, ic[1] + j, ic[2] + k), sx, v);
         xl[j][k] = (float)v[0];
         c[0][j][k] = (float)v[1];
         c[1][j][k] = (float)v[2];
@@ -332,21 +346,20 @@ struct CrossSampler<T, true> {
       }
       return mk(t[1] - t[0], t[2] - t[3], t[5] - t[4]) / 2.0f;
     }
-    const int sx = ic[0] & ((1 << Blk<T>::LX) - 1);
     float yl[2][2], yr[2][2], zl[2][2], zr[2][2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       uint32_t a[2], b[2];
-      row2<T>(row(ic[0], ic[1] - 1, ic[2] + k), sx, a);
-      row2<T>(row(ic[0], ic[1] + 2, ic[2] + k), sx, b);
+      run2<T>(base(), vox(ic[0], ic[1] - 1, ic[2] + k), a);
+      run2<T>(base(), vox(ic[0], ic[1] + 2, ic[2] + k), b);
       yl[0][k] = (float)a[0]; yl[1][k] = (float)a[1];
       yr[0][k] = (float)b[0]; yr[1][k] = (float)b[1];
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       uint32_t a[2], b[2];
-      row2<T>(row(ic[0], ic[1] + j, ic[2] - 1), sx, a);
-      row2<T>(row(ic[0], ic[1] + j, ic[2] + 2), sx, b);
+      run2<T>(base(), vox(ic[0], ic[1] + j, ic[2] - 1), a);
+      run2<T>(base(), vox(ic[0], ic[1] + j, ic[2] + 2), b);
       zl[0][j] = (float)a[0]; zl[1][j] = (float)a[1];
       zr[0][j] = (float)b[0]; zr[1][j] = (float)b[1];
     }
@@ -563,8 +576,6 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
                      float tray) -> bool {
         const float mlt = (float)(1 << k);
         const float ssc = k == 0 ? sscale : sscale * mlt;
-        const V3 inc = k == 0 ? incTex : incTex * mlt;
-        const V3 rinc = k == 0 ? rayInc : rayInc * mlt;
         V3 pos_s = pos;   // the position shade() samples
         // TF, gradient and shading of the sample at pos_s (the cross sampler's two forms)
         auto shade = [&](auto fast_c, float& s0, float& s1, float& s2, float& s3) {
@@ -633,10 +644,12 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           c1 += s1 * wgt;
           c2 += s2 * wgt;
           c3 += s3 * (1.0f - c3);
-          if (LIT) cur = cur + rinc;
+          // x * 2^k is exact, so the fused form equals pos + (incTex * 2^k)
+          // and adds no registers for the scaled increments
+          if (LIT) cur = mk(fmaf(rayInc.x, mlt, cur.x), fmaf(rayInc.y, mlt, cur.y), fmaf(rayInc.z, mlt, cur.z));
           if (c3 >= 0.99f) return false;
           if (eps > 0.0f && tray * (1.0f - c3) < eps) return false;
-          pos = pos + inc;
+          pos = mk(fmaf(incTex.x, mlt, pos.x), fmaf(incTex.y, mlt, pos.y), fmaf(incTex.z, mlt, pos.z));
         }
         return true;
       };
@@ -668,6 +681,29 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         // (and GL_CLAMP) the border colour 0 can enter the interpolation.
         const bool strict = (float)SN_PROBE * dl <= 6.0f;
         const bool border0 = !edge && S.overlap == 0;
+        // the network inputs that stay fixed over the segment, put(index, value)
+        auto seg_features = [&](auto&& put) {
+          put(9, LIT ? 1.0f : 0.0f);
+          put(10, MODE == 1 ? 1.0f : 0.0f);
+          put(11, fabsf(dv.x));
+          put(12, fabsf(dv.y));
+          put(13, fabsf(dv.z));
+          put(17, sscale);
+          put(20, -entry.z * 0.25f);
+          put(21, dl);
+          put(22, fb->tfs[3]);
+          put(23, 1.0f);
+        };
+        // This lane's X row of the network in LDS (bf16; 24..31 zero): the
+        // segment's constant inputs are stored once, the others per chunk.  A
+        // row only enters its own ray's outputs, so rows of lanes that do not
+        // ask are left as they are.
+        uint16_t* const xrow = sn_lds + lane * SN_INP;
+        if constexpr (LEARN == 1) {
+          seg_features([&](int q, float v) { xrow[q] = bf16_bits(v); });
+#pragma unroll
+          for (int q = SN_IN; q < SN_INP; q += 2) *reinterpret_cast<uint32_t*>(xrow + q) = 0u;
+        }
         const int ncx = (int)cell_count((uint32_t)nx), ncy = (int)cell_count((uint32_t)ny);
         const uint32_t* __restrict__ cg = S.cells + fb->cells;
         const int tn = S.tfst_n;
@@ -713,30 +749,26 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             smax = fmaxf(ts[r + lo], ts[r + hi - (1 << k2)]);
           }
           const float obound = 1.0f - powf(1.0f - fminf(amax, 1.0f), sscale * (float)nf);
-          float f[SN_IN];
-          f[0] = fminf(fmaxf(vl * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
-          f[1] = fminf(fmaxf(vh * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
-          f[2] = f[1] - f[0];
-          f[3] = (ts[tfst_pa(tn) + hi + 1] - ts[tfst_pa(tn) + lo]) / cnt;
-          f[4] = amax;
-          f[5] = ts[tfst_ptv(tn) + hi] - ts[tfst_ptv(tn) + lo];
-          f[6] = ts[tfst_prgb(tn) + hi] - ts[tfst_prgb(tn) + lo];
-          f[7] = smax;
-          f[8] = (ts[tfst_pnz(tn) + hi + 1] - ts[tfst_pnz(tn) + lo]) / cnt;
-          f[9] = LIT ? 1.0f : 0.0f;
-          f[10] = MODE == 1 ? 1.0f : 0.0f;
-          f[11] = fabsf(dv.x); f[12] = fabsf(dv.y); f[13] = fabsf(dv.z);
-          f[14] = (float)nf * (1.0f / (float)SN_CHUNK);
-          f[15] = T0;
-          f[16] = -log2f(fmaxf(T0, 1e-6f)) * (1.0f / 20.0f);
-          f[17] = sscale;
-          f[18] = obound;
-          f[19] = T0 * obound;
-          f[20] = -entry.z * 0.25f;
-          f[21] = dl;
-          f[22] = fb->tfs[3];
-          f[23] = 1.0f;
-          bool skip = has && strict && f[19] <= LC->skip_bound;
+          // the network inputs that change per chunk, put(index, value)
+          auto chunk_features = [&](auto&& put) {
+            const float x0 = fminf(fmaxf(vl * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
+            const float x1 = fminf(fmaxf(vh * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
+            put(0, x0);
+            put(1, x1);
+            put(2, x1 - x0);
+            put(3, (ts[tfst_pa(tn) + hi + 1] - ts[tfst_pa(tn) + lo]) / cnt);
+            put(4, amax);
+            put(5, ts[tfst_ptv(tn) + hi] - ts[tfst_ptv(tn) + lo]);
+            put(6, ts[tfst_prgb(tn) + hi] - ts[tfst_prgb(tn) + lo]);
+            put(7, smax);
+            put(8, (ts[tfst_pnz(tn) + hi + 1] - ts[tfst_pnz(tn) + lo]) / cnt);
+            put(14, (float)nf * (1.0f / (float)SN_CHUNK));
+            put(15, T0);
+            put(16, -log2f(fmaxf(T0, 1e-6f)) * (1.0f / 20.0f));
+            put(18, obound);
+            put(19, T0 * obound);
+          };
+          bool skip = has && strict && T0 * obound <= LC->skip_bound;
           int k = 0;
           if constexpr (LEARN == 1) {
             // a lane whose bound exceeds the fallback marches the fixed step and
@@ -746,12 +778,9 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             if (LC->fixed_k >= 0) {
               k = LC->fixed_k;
             } else if (__ballot(forced) == 0 && __ballot(ask) != 0) {
-              if (!ask) {
-#pragma unroll
-                for (int q = 0; q < SN_IN; ++q) f[q] = 0.0f;
-              }
+              chunk_features([&](int q, float v) { xrow[q] = bf16_bits(v); });
               float y0, y1;
-              sn_eval_wave(LC->net, f, sn_lds, y0, y1);
+              sn_eval_wave_x(LC->net, sn_lds, y0, y1);
               if (ask) {
                 k = max(0, min(LC->max_log2, (int)floorf(3.0f * y0 + 0.5f - LC->margin)));
                 skip = y1 > LC->skip_logit;
@@ -772,8 +801,8 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
               const unsigned int slot = atomicAdd(SO->count, 1u);
               if (slot < SO->cap) {
                 SegSample* o = SO->out + slot;
-#pragma unroll
-                for (int q = 0; q < SN_IN; ++q) o->f[q] = f[q];
+                seg_features([&](int q, float v) { o->f[q] = v; });
+                chunk_features([&](int q, float v) { o->f[q] = v; });
                 o->t_before = T0;
                 o->steps = (float)nf;
 #pragma unroll 1

@@ -86,24 +86,18 @@ __device__ inline void sn_wave_sync() {
 // LDS scratch per wave for sn_eval_wave: X [64][32] and H1 [32][64] bf16
 constexpr int SN_WAVE_LDS = 64 * SN_INP + 32 * SN_HID;
 
-// Evaluates the network for the 64 rays of a wave (lane = ray).  f: this
-// lane's features (dead once stored).  lds: SN_WAVE_LDS uint16 per wave.
-// Layers 1 and 2 are MFMA products over 32 rays at a time (hidden unit x ray
-// tiles); layer 3 (64 -> 2) is folded into the layer-2 epilogue: every lane
-// dots its 16 bf16-rounded hidden units of a tile with the bf16 W3 columns,
-// and the two lanes that hold the same ray add their partial sums.  Loops are
-// kept rolled so that the evaluation adds few registers to the caller's.
-__device__ inline void sn_eval_wave(const StepNetDev& N, const float* f, uint16_t* lds, float& y0, float& y1) {
+// Evaluates the network for the 64 rays of a wave (lane = ray) whose inputs
+// are already in the X rows of lds (bf16, row = lane, features 24..31 zero).
+// lds: SN_WAVE_LDS uint16 per wave.  Layers 1 and 2 are MFMA products over 32
+// rays at a time (hidden unit x ray tiles); layer 3 (64 -> 2) is folded into
+// the layer-2 epilogue: every lane dots its 16 bf16-rounded hidden units of a
+// tile with the bf16 W3 columns, and the two lanes that hold the same ray add
+// their partial sums.  Loops are kept rolled so that the evaluation adds few
+// registers to the caller's.
+__device__ inline void sn_eval_wave_x(const StepNetDev& N, uint16_t* lds, float& y0, float& y1) {
   const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
   uint16_t* xs = lds;                   // X [64][32]: ray, feature
   uint16_t* h1 = lds + 64 * SN_INP;     // H1 [32][64]: ray of the half, hidden unit
-  {
-    sn_u16x8 v[SN_INP / 8];
-#pragma unroll
-    for (int k = 0; k < SN_INP; ++k) v[k >> 3][k & 7] = k < SN_IN ? bf16_bits(f[k]) : (uint16_t)0;
-#pragma unroll
-    for (int q = 0; q < SN_INP / 8; ++q) *reinterpret_cast<sn_u16x8*>(xs + lane * SN_INP + 8 * q) = v[q];
-  }
   sn_wave_sync();
   float o0 = 0.0f, o1 = 0.0f;
 #pragma unroll 1
@@ -155,6 +149,17 @@ __device__ inline void sn_eval_wave(const StepNetDev& N, const float* f, uint16_
   }
   y0 = o0;
   y1 = o1;
+}
+
+// The same from this lane's features f (fp32, rounded to bf16 here).
+__device__ inline void sn_eval_wave(const StepNetDev& N, const float* f, uint16_t* lds, float& y0, float& y1) {
+  const int lane = threadIdx.x & 63;
+  sn_u16x8 v[SN_INP / 8];
+#pragma unroll
+  for (int k = 0; k < SN_INP; ++k) v[k >> 3][k & 7] = k < SN_IN ? bf16_bits(f[k]) : (uint16_t)0;
+#pragma unroll
+  for (int q = 0; q < SN_INP / 8; ++q) *reinterpret_cast<sn_u16x8*>(lds + lane * SN_INP + 8 * q) = v[q];
+  sn_eval_wave_x(N, lds, y0, y1);
 }
 
 }  // namespace tvk
