@@ -40,3 +40,15 @@ def test_brick_size_autotuner_picks_the_fastest():
     # another brick size moves the per-brick restarts: close, not equal
     assert np.abs(after - before).mean() < 0.01
     ren.close()
+
+
+def test_oversized_bricks_are_rejected():
+    """Bricks whose blocked layout needs more than 32-bit byte offsets (or with
+    a side above 4096) are refused before any device memory is allocated."""
+    vol = np.random.default_rng(22).integers(0, 256, size=(24, 20, 16)).astype(np.uint8)
+    r = HIPRaycaster(0)
+    r.LoadVolume(vol, (10, 10, 10), 1, max_lods=1, keep_linear=True)
+    for brick in ((4097, 8, 8), (2048, 2048, 2048)):
+        with pytest.raises(Exception, match="brick"):
+            r.Rebrick(brick, 1, max_lods=0)
+    r.close()

@@ -13,6 +13,12 @@ bool Layout::build(const uint64_t d[3], const uint32_t b[3], uint32_t ov, int by
     brick[i] = b[i];
     if (d[i] == 0) { if (err) *err = "volume dimension is zero"; return false; }
     if (b[i] <= 2 * ov) { if (err) *err = "brick size must exceed 2*overlap"; return false; }
+    if (b[i] > 4096) { if (err) *err = "brick size above 4096"; return false; }
+  }
+  // the kernels address a brick's voxels with 32-bit byte offsets
+  if (padded_count(b, bytes == 1 ? 3 : 2) * (uint64_t)bytes > 0xffffffffull) {
+    if (err) *err = "brick too large: its blocked layout exceeds 4 GiB";
+    return false;
   }
   overlap = ov;
   bpv = bytes;

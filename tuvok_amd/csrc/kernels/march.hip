@@ -438,12 +438,14 @@ __device__ __forceinline__ V3 normal_eye(V3 g, const float* nm, V3 ds) {
 // LEARN: 0 fixed step; 1 step proposal by the network (LearnConst); 2 fixed step
 // that also records segment samples (SegOut) with the segment's RGBA at step
 // multipliers 1, 2, 4 and 8.  sn_lds: the wave's network scratch (LEARN > 0).
+// cols: the wave's 64-word LDS copy of the 2D TF's non-zero-alpha column mask
+// (MODE 1).
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
 __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __restrict__ tf1, const ViewConst* VC,
                                            const FrameBrick* __restrict__ list, float* __restrict__ out, int tx0,
                                            int ty0, int tw, int th, unsigned long long* counter, uint32_t* touched,
                                            const LearnConst* LC, const SegOut* SO, uint16_t* sn_lds,
-                                           uint32_t view_id) {
+                                           uint32_t view_id, uint64_t* cols) {
   const int lane = threadIdx.x & 63;
   const int px = tx0 + lane % tw, py = ty0 + lane / tw;
   const int W = (int)VC->W, H = (int)VC->H;
@@ -464,6 +466,15 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
   const V3 amb = mk(S.amb[0], S.amb[1], S.amb[2]), dif = mk(S.dif[0], S.dif[1], S.dif[2]);
   const V3 spc = mk(S.spc[0], S.spc[1], S.spc[2]), ldir = mk(S.ldir[0], S.ldir[1], S.ldir[2]);
   const V3 dsc = mk(S.dscale[0], S.dscale[1], S.dscale[2]);
+  // the view's normal matrix in scalar registers: read once, not per sample
+  float nm[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    nm[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, VC->nm[i])));
+  if constexpr (MODE == 1) {
+    cols[lane] = S.tf2_cols[lane];
+    sn_wave_sync();
+  }
   const float eps = S.ert_eps;
   const bool edge = S.clamp_edge != 0;
 
@@ -588,7 +599,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
             if (LIT && s3 != 0.0f) {   // a transparent sample composites to nothing: skip its shading
               const V3 gg = cs.gradient(vstep);
-              const V3 nn = normal_eye(gg, VC->nm, dsc);
+              const V3 nn = normal_eye(gg, nm, dsc);
               const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
               s0 = lc.x; s1 = lc.y; s2 = lc.z;
             }
@@ -597,7 +608,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           } else {
             const float sx = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
             const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
-            if (((S.tf2_cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
+            if (((cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
               // every row of this value column is transparent: alpha 0 whatever the gradient
               s0 = s1 = s2 = s3 = 0.0f;
             } else {
@@ -608,7 +619,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
               const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];   // byte/255, precomputed
               s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
               if (LIT) {
-                const V3 nn = normal_eye(gg, VC->nm, dsc);
+                const V3 nn = normal_eye(gg, nm, dsc);
                 const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
                 const float tt = 1.0f - s3;
                 s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
@@ -856,7 +867,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
       const T* __restrict__ hd = reinterpret_cast<const T*>(S.pool) + hb->offset;
       hit_n = normal_eye(gradient<T>(hd, (int)hb->n[0], (int)hb->n[1], (int)hb->n[2], edge, hit_tex,
                                      mk(hb->vstep[0], hb->vstep[1], hb->vstep[2])),
-                         VC->nm, dsc);
+                         nm, dsc);
     }
     if (have_hit && hit_param != 0.0f) {
       const V3 isoc = mk(S.iso_col[0], S.iso_col[1], S.iso_col[2]);
@@ -876,6 +887,7 @@ template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
 __device__ __forceinline__ void march_group(const MarchArgs& A) {
   extern __shared__ float4 s_tf[];  // 1D TF (tf_n entries, MODE 0 only)
   __shared__ __align__(16) uint16_t s_sn[LEARN > 0 ? 4 * SN_WAVE_LDS : 8];   // step network scratch, one slice per wave
+  __shared__ uint64_t s_cols[MODE == 1 ? 4 * 64 : 1];                          // 2D TF column masks, one slice per wave
 
   // XCD-aware remap: consecutive workgroups go round-robin over the 8 XCDs;
   // give each XCD a contiguous range of 16x16 groups.
@@ -898,7 +910,8 @@ __device__ __forceinline__ void march_group(const MarchArgs& A) {
   const int tx = (int)gx * 2 + (wave & 1), ty = (int)gy * 2 + (wave >> 1);
   march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
                                          A.counter, COUNT ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
-                                         &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v);
+                                         &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v,
+                                         s_cols + (MODE == 1 ? wave * 64 : 0));
 }
 
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
@@ -924,6 +937,7 @@ template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(
     BatchArgs B, const BatchRequest* __restrict__ req) {
   __shared__ __align__(16) uint16_t s_sn[LEARN > 0 ? 4 * SN_WAVE_LDS : 8];
+  __shared__ uint64_t s_cols[MODE == 1 ? 4 * 64 : 1];
   const uint32_t xcd = blockIdx.x & 7u;
   const uint32_t chunk = (B.tiles_total + 7u) >> 3;
   const int lane = threadIdx.x & 63;
@@ -950,7 +964,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
                                              (int)(tx * B.tile_w), (int)(ty * B.tile_h), (int)B.tile_w,
                                              (int)B.tile_h, B.counter, COUNT ? B.touched + R->touched_offset : nullptr,
                                              &B.learn, nullptr, s_sn + (LEARN > 0 ? (threadIdx.x >> 6) * SN_WAVE_LDS : 0),
-                                             0u);
+                                             0u, s_cols + (MODE == 1 ? (threadIdx.x >> 6) * 64 : 0));
     }
   }
 }
