@@ -20,6 +20,20 @@ __device__ __forceinline__ V3 operator/(V3 a, float s) { return mk(a.x / s, a.y 
 __device__ __forceinline__ V3 operator/(V3 a, V3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+
+// sqrtf(x) for x >= 0: the hardware square root (within 1 ulp) moved to the
+// neighbour whose residual brackets x, which is the correctly rounded root (the
+// same steps as the compiler's sqrtf).  sqrtf also rescales every input below
+// 2^-96, where the residuals would underflow; here such inputs (rare: the
+// squared gradient of a nearly flat neighbourhood) take sqrtf itself.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(x);
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float lo = __builtin_bit_cast(float, __builtin_bit_cast(int, s) - 1);
+  const float hi = __builtin_bit_cast(float, __builtin_bit_cast(int, s) + 1);
+  const float rlo = fmaf(-lo, s, x), rhi = fmaf(-hi, s, x);
+  return rhi > 0.0f ? hi : (rlo <= 0.0f ? lo : s);
+}
 // GLSL normalize(): one correctly rounded reciprocal of the length, then three products
 __device__ __forceinline__ V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }

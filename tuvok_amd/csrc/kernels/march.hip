@@ -383,10 +383,12 @@ struct CrossSampler<T, true> {
 // for shading directions only, never for positions, step counts or TF indices)
 __device__ __forceinline__ V3 normalize_rsq(V3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
 
-// lighting.glsl:33-50 (eye at the origin)
+// lighting.glsl:33-50 (eye at the origin).  view and n are unit vectors (or
+// n = 0), so reflect(view, n) is one up to rounding and its normalize() is
+// left out: the specular term moves by about 1e-6 relative.
 __device__ __forceinline__ V3 lighting(V3 pos, V3 n, V3 amb, V3 dif, V3 spc, V3 L) {
   V3 view = normalize_rsq(mk(0, 0, 0) - pos);
-  V3 R = normalize_rsq(reflect(view, n));
+  V3 R = reflect(view, n);
   float dd = fmaxf(fabsf(dot(n, L)), 0.0f);
   float ss = pow8(fmaxf(dot(R, L), 0.0f));
   V3 c = amb + dif * dd + spc * ss;
@@ -613,7 +615,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
               s0 = s1 = s2 = s3 = 0.0f;
             } else {
               const V3 gg = cs.gradient(vstep);
-              const float gm = length(gg);
+              const float gm = sqrt_rn(dot(gg, gg));   // length(gg), bit for bit
               const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
               const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
               const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];   // byte/255, precomputed
