@@ -60,6 +60,64 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
                        f"({W}x{hi - lo} rays, {ns_total} samples, {dt_total:.1f} s, {threads} threads)")
 
 
+def other_workloads():
+    """Frame times of the other full-size workloads (reported beside the main
+    line, not part of its value): the 2048^3 uint8 SDF isosurface at 1080p,
+    the 48-request batch over three uint8 volumes in one persistent launch, and
+    the 32-request batch over the 64 GiB uint16 volume with its 5-level pyramid
+    in the HBM pool.  Each is built, timed over a few frames and freed."""
+    from tuvok_amd import workloads
+    res = {}
+    try:
+        c = workloads.SDF_ISO_2048
+        r = workloads.setup(c)
+        r.Paint(download=False)
+        ms = []
+        for _ in range(5):
+            r.Paint(download=False)
+            ms.append(r.last_stats.ms)
+        r.close()
+        m = float(np.median(ms))
+        res["iso_sdf2048"] = dict(workload=c["name"], ms_per_frame=round(m, 3), frames_per_s=round(1e3 / m, 1))
+    except Exception as e:
+        res["iso_sdf2048"] = dict(error=f"{type(e).__name__}: {e}")
+    print(f"[bench] iso: {res['iso_sdf2048']}", file=sys.stderr, flush=True)
+    try:
+        reqs = workloads.batch48()
+        batch, rens, owners = workloads.setup_batch(reqs, ert_eps=ERT_EPS)
+        best, _ = batch.Autotune(reps=1)
+        batch.Render()
+        ms = [batch.Render().ms for _ in range(5)]
+        px = sum(w * h for (_, _, _, w, h) in reqs)
+        m = float(np.median(ms))
+        res["batch48"] = dict(requests=len(reqs), mpixels=round(px / 1e6, 2), ms_per_frame=round(m, 3),
+                              mpixels_per_s=round(px / m / 1e3, 1), tuning=best)
+        batch.close()
+        for x in rens + list(owners.values()):
+            x.close()
+    except Exception as e:
+        res["batch48"] = dict(error=f"{type(e).__name__}: {e}")
+    print(f"[bench] batch48: {res['batch48']}", file=sys.stderr, flush=True)
+    try:
+        reqs = workloads.lod_requests()
+        batch, rens, owner = workloads.setup_lod_batch(reqs)
+        d = owner.GetDatasetDesc()
+        batch.Render()
+        ms = [batch.Render().ms for _ in range(3)]
+        px = sum(w * h for (_, _, w, h) in reqs)
+        m = float(np.median(ms))
+        res["lod64g"] = dict(requests=len(reqs), mpixels=round(px / 1e6, 2), ms_per_frame=round(m, 3),
+                             mpixels_per_s=round(px / m / 1e3, 1), pool_gib=round(d.pool_bytes / 2 ** 30, 1),
+                             levels=sorted(set(int(x.CurrentLOD()) for x in rens)))
+        batch.close()
+        for x in rens + [owner]:
+            x.close()
+    except Exception as e:
+        res["lod64g"] = dict(error=f"{type(e).__name__}: {e}")
+    print(f"[bench] lod64g: {res['lod64g']}", file=sys.stderr, flush=True)
+    return res
+
+
 def learned_leg(ren, views, cfg, train_views=48, seed=123):
     """Learned step vs fixed step on the same volume, TF and views, at the
     0.25-voxel reference step (srm 2): chunk samples from `train_views` random
@@ -143,6 +201,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-step legs")
     ap.add_argument("--no-phantom", action="store_true", help="skip the learned-step leg on the 2048^3 phantom")
     ap.add_argument("--size", type=int, default=1024, help="volume edge (default: the 1024^3 workload)")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the isosurface, 48-request batch and 64 GiB LOD batch measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
@@ -225,6 +285,10 @@ def main():
             phantom = dict(error=f"{type(e).__name__}: {e}")
         print(f"[bench] learned (phantom): {phantom}", file=sys.stderr, flush=True)
 
+    other = None
+    if not args.no_other:
+        other = other_workloads()
+
     out = {
         "metric": "Gsamples/s (fixed-step-equivalent) + frames/s @1080p",
         "value": round(value, 3),
@@ -257,6 +321,7 @@ def main():
         "cpu_baseline": cpu,
         "learned_step": learned,
         "learned_step_phantom": phantom,
+        "other_workloads": other,
     }
     print(json.dumps(out))
 

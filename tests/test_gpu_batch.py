@@ -170,6 +170,42 @@ def test_iso_sdf_refine4_matches_oracle():
     r.close()
 
 
+def test_sdf_iso_2048_full_size():
+    """The isosurface workload at full size: 2048^3 uint8 signed distance of 500
+    ellipsoids (128^3 cores + 1 overlap), iso 128, 4 bisection steps, gradient
+    shading, 1920x1080.  The frame is opaque-or-empty everywhere and a 96x64
+    window at the centre equals the oracle (LOD0 bricking of the downloaded
+    volume) within the gate."""
+    print("sdf iso 2048: generating", flush=True)
+    r = workloads.setup(workloads.SDF_ISO_2048, keep_linear=True)
+    r.SetEarlyRayTermination(0.0)
+    img = r.Paint()
+    ms = []
+    for _ in range(3):
+        r.Paint(download=False)
+        ms.append(r.last_stats.ms)
+    print(f"sdf iso 2048: {min(ms):.2f} ms per 1080p frame", flush=True)
+    a = img[..., 3]
+    assert np.isfinite(img).all()
+    assert set(np.unique(a)).issubset({0.0, 1.0})
+    hits = int((a > 0).sum())
+    print(f"sdf iso 2048: {hits} hit pixels", flush=True)
+    assert hits > 0.05 * a.size
+    assert img[..., :3].min() >= 0.0 and img[..., :3].max() <= 1.0
+    print("sdf iso 2048: oracle bricking (LOD0)", flush=True)
+    vol = r.DownloadVolume()
+    ds = O.Dataset(vol, workloads.SDF_ISO_2048["brick"], 1, lod0_only=True)
+    del vol
+    x0, y0 = 960 - 48, 540 - 32
+    ref, _ = O.render(ds, oracle_camera_from(r), oracle_params_from(r), roi=(x0, y0, x0 + 96, y0 + 64))
+    got = img[y0:y0 + 64, x0:x0 + 96]
+    mx, mean = err_stats(got, ref)
+    print(f"sdf iso 2048 window: max {mx:.3g} mean {mean:.3g} hits {int((ref[..., 3] > 0).sum())}", flush=True)
+    assert (ref[..., 3] > 0).sum() > 100
+    assert mx <= TOL_MAX and mean < TOL_MEAN
+    r.close()
+
+
 def test_batch48_full_size():
     """The 48-request batch over 256^3 / 1024^3 / 2048^3 uint8 volumes in one frame:
     premultiplied colour bounds hold everywhere and three requests equal their
