@@ -60,12 +60,13 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
                        f"({W}x{hi - lo} rays, {ns_total} samples, {dt_total:.1f} s, {threads} threads)")
 
 
-def other_workloads():
+def other_workloads(learned=True):
     """Frame times of the other full-size workloads (reported beside the main
     line, not part of its value): the 2048^3 uint8 SDF isosurface at 1080p,
     the 48-request batch over three uint8 volumes in one persistent launch, and
     the 32-request batch over the 64 GiB uint16 volume with its 5-level pyramid
-    in the HBM pool.  Each is built, timed over a few frames and freed."""
+    in the HBM pool, also with the learned step proposal (learned=True).  Each
+    is built, timed over a few frames and freed."""
     from tuvok_amd import workloads
     res = {}
     try:
@@ -98,9 +99,11 @@ def other_workloads():
     except Exception as e:
         res["batch48"] = dict(error=f"{type(e).__name__}: {e}")
     print(f"[bench] batch48: {res['batch48']}", file=sys.stderr, flush=True)
+    owner = None
     try:
         reqs = workloads.lod_requests()
-        batch, rens, owner = workloads.setup_lod_batch(reqs)
+        owner = workloads.lod_owner()
+        batch, rens, _ = workloads.setup_lod_batch(reqs, owner=owner)
         d = owner.GetDatasetDesc()
         batch.Render()
         ms = [batch.Render().ms for _ in range(3)]
@@ -110,11 +113,25 @@ def other_workloads():
                              mpixels_per_s=round(px / m / 1e3, 1), pool_gib=round(d.pool_bytes / 2 ** 30, 1),
                              levels=sorted(set(int(x.CurrentLOD()) for x in rens)))
         batch.close()
-        for x in rens + [owner]:
+        for x in rens:
             x.close()
     except Exception as e:
         res["lod64g"] = dict(error=f"{type(e).__name__}: {e}")
     print(f"[bench] lod64g: {res['lod64g']}", file=sys.stderr, flush=True)
+    if owner is not None and learned:
+        # the same batch with the learned step proposal (trained on 12 other
+        # views at 960x540, calibrated on 4 other requests within 0.5/255)
+        try:
+            cal = workloads.lod_requests(n=4, seed=19)
+            train = [(d_, rot, 960, 540) for (d_, rot, _, _) in workloads.lod_requests(n=12, seed=29)]
+            lr = workloads.lod_learned(owner, reqs, cal, train)
+            lr["calibration"] = sorted(lr["calibration"], key=lambda t: t[2])[:4]
+            res["lod64g_learned"] = lr
+        except Exception as e:
+            res["lod64g_learned"] = dict(error=f"{type(e).__name__}: {e}")
+        print(f"[bench] lod64g learned: {res['lod64g_learned']}", file=sys.stderr, flush=True)
+    if owner is not None:
+        owner.close()
     return res
 
 
@@ -287,7 +304,7 @@ def main():
 
     other = None
     if not args.no_other:
-        other = other_workloads()
+        other = other_workloads(learned=not args.no_learned)
 
     out = {
         "metric": "Gsamples/s (fixed-step-equivalent) + frames/s @1080p",

@@ -257,3 +257,23 @@ def test_lod64g_batch_full_size():
         if i in (0, 31):
             assert np.array_equal(f, rens[i].Paint()), i
     batch.close()
+
+
+def test_lod64g_learned_batch():
+    """The 32-request batch over the 64 GiB volume with the learned step
+    proposal: a network trained on chunk samples of 12 other views (their own
+    levels), calibrated on 4 further requests within 0.5/255 of the fixed
+    batch, renders the 32 requests within 1/255 of the fixed batch."""
+    print("lod64g learned: generating 64 GiB volume + pyramid", flush=True)
+    owner = workloads.lod_owner()
+    reqs = workloads.lod_requests()
+    cal = workloads.lod_requests(n=4, seed=19)
+    train = [(d, rot, 960, 540) for (d, rot, _, _) in workloads.lod_requests(n=12, seed=29)]
+    res = workloads.lod_learned(owner, reqs, cal, train)
+    print(f"lod64g learned: fixed {res['fixed_ms']} ms, learned {res['learned_ms']} ms ({res['speedup']}x), "
+          f"max err {res['max_err_x255']}/255, proposal {res['proposal']}, chunks {res['chunks']}", flush=True)
+    for t in sorted(res["calibration"], key=lambda t: t[2])[:4]:
+        print(f"  calibration: {t}", flush=True)
+    assert res["chunks"] > 1000 and res["loss_last"] < res["loss_first"]
+    assert res["max_err"] <= TOL_MAX
+    owner.close()

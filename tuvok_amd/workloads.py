@@ -60,34 +60,115 @@ def lod_requests(n=32, seed=9, near=2.0, far=8.0):
     return out
 
 
-def setup_lod_batch(requests, cfg=LOD_64G, device=0, ert_eps=2.0 ** -11, dims=None):
-    """The large volume (all kept levels in the HBM pool) shared by one renderer
-    per request, each with its own camera distance and screen-space-error level
-    choice, in one RenderBatch."""
-    from .batch import RenderBatch
+def lod_owner(cfg=LOD_64G, device=0, dims=None):
+    """The renderer that holds the large volume (every kept level in the HBM
+    pool, linear volume kept for the gradient maximum)."""
     owner = HIPRaycaster(device)
     owner.GenerateVolume(cfg["gen"], cfg["seed"], cfg["dtype"], dims or cfg["dims"], cfg["brick"], cfg["overlap"],
                          max_lods=cfg["max_lods"], keep_linear=True)
-    tf = tf2d_gaussian_lobes().GetByteArray()
+    return owner
+
+
+def lod_renderer(owner, dist, rot, w, h, cfg=LOD_64G, device=0, ert_eps=2.0 ** -11, gmax=None):
+    """A renderer of the large volume sharing owner's pool: lit 2D TF, its
+    own eye distance and screen-space-error level choice."""
+    g = owner.MaxGradientMagnitude() if gmax is None else gmax
+    r = HIPRaycaster(device)
+    r.ShareDataset(owner)
+    r.Set2DTrans(tf2d_gaussian_lobes().GetByteArray())
+    r.SetRendermode(cfg["mode"])
+    r.SetUseLighting(cfg["lighting"])
+    r.SetGradientScale(1.0 / g if g > 0 else 1.0)
+    r.SetLOD(-1)
+    r.SetEarlyRayTermination(ert_eps)
+    r.SetViewParameters(cfg["fov"], 0.01, 1000.0)
+    r.SetViewPosition((0.0, 0.0, dist))
+    r.Resize(w, h)
+    r.SetRotation(rot)
+    return r
+
+
+def setup_lod_batch(requests, cfg=LOD_64G, device=0, ert_eps=2.0 ** -11, dims=None, owner=None):
+    """The large volume (all kept levels in the HBM pool) shared by one renderer
+    per request, each with its own camera distance and screen-space-error level
+    choice, in one RenderBatch.  owner: an existing lod_owner() to share."""
+    from .batch import RenderBatch
+    owner = owner or lod_owner(cfg, device, dims)
     g = owner.MaxGradientMagnitude()
     batch = RenderBatch(device)
     rens = []
     for (dist, rot, w, h) in requests:
-        r = HIPRaycaster(device)
-        r.ShareDataset(owner)
-        r.Set2DTrans(tf)
-        r.SetRendermode(cfg["mode"])
-        r.SetUseLighting(cfg["lighting"])
-        r.SetGradientScale(1.0 / g if g > 0 else 1.0)
-        r.SetLOD(-1)
-        r.SetEarlyRayTermination(ert_eps)
-        r.SetViewParameters(cfg["fov"], 0.01, 1000.0)
-        r.SetViewPosition((0.0, 0.0, dist))
-        r.Resize(w, h)
-        r.SetRotation(rot)
+        r = lod_renderer(owner, dist, rot, w, h, cfg, device, ert_eps, gmax=g)
         batch.Add(r, rot, w, h)
         rens.append(r)
     return batch, rens, owner
+
+
+def lod_learned(owner, requests, cal_requests, train_requests, bound=0.5 / 255, seed=0, cap=1_000_000,
+                epochs=20):
+    """Learned step proposal on the large-volume batch.  Chunk samples of the
+    fixed march over train_requests (eye distance, rotation, width, height)
+    train a network; the setting of it that renders the cal_requests batch
+    fastest within `bound` of the fixed batch is chosen (the fixed march is a
+    candidate); then the `requests` batch is timed with and without it and
+    compared frame by frame.  Returns a dict of the figures."""
+    from .stepnet import StepNet, labels, split_samples
+    g = owner.MaxGradientMagnitude()
+    rec = []
+    for k, (dist, rot, w, h) in enumerate(train_requests):
+        r = lod_renderer(owner, dist, rot, w, h, gmax=g)
+        rec.append(r.GenerateSegments([rot], cap // max(1, len(train_requests)), keep=0.05, seed=seed + k, fit=True))
+        r.close()
+    rec = np.concatenate(rec)
+    tau = 0.25 / 255
+    tgt = labels(rec, tau, tau / 2)
+    net = StepNet(0, seed=seed)
+    net.set_data(split_samples(rec)[0], tgt)
+    losses = net.train(epochs=epochs)
+
+    def frames(b, n):
+        return [b.Frame(i) for i in range(n)]
+
+    def timed(b, reps=3):
+        b.Render()
+        return float(np.median([b.Render().ms for _ in range(reps)]))
+
+    cal, cal_rens, _ = setup_lod_batch(cal_requests, owner=owner)
+    cal.SetStepProposal(None)
+    cal_ms = timed(cal)
+    cal_ref = frames(cal, len(cal_requests))
+    best, best_ms, tried = None, cal_ms, []
+    for fb in (0.0, 0.02, 0.005):
+        for m in (0.0, 1.0, 2.0):
+            prop = dict(margin=m, skip_threshold=1.0, fallback_bound=fb)
+            cal.SetStepProposal(net, **prop)
+            ms = timed(cal, reps=1)
+            err = max(float(np.abs(a - b).max()) for a, b in zip(frames(cal, len(cal_requests)), cal_ref))
+            tried.append((prop, round(err * 255, 3), round(ms, 3)))
+            if err <= bound and ms < best_ms:
+                best, best_ms = prop, ms
+    cal.close()
+    for x in cal_rens:
+        x.close()
+
+    batch, rens, _ = setup_lod_batch(requests, owner=owner)
+    batch.SetStepProposal(None)
+    fixed_ms = timed(batch)
+    ref = frames(batch, len(requests))
+    if best is not None:
+        batch.SetStepProposal(net, **best)
+    learned_ms = timed(batch)
+    err = max(float(np.abs(a - b).max()) for a, b in zip(frames(batch, len(requests)), ref))
+    del ref
+    batch.close()
+    for x in rens:
+        x.close()
+    net.close()
+    return dict(fixed_ms=round(fixed_ms, 3), learned_ms=round(learned_ms, 3), speedup=round(fixed_ms / learned_ms, 3),
+                max_err=err, max_err_x255=round(err * 255, 3),
+                proposal=best if best is not None else "fixed march (no setting faster within the bound)",
+                calibration_bound_x255=round(bound * 255, 3), chunks=int(len(rec)),
+                loss_first=float(losses[0]), loss_last=float(losses[-1]), calibration=tried)
 
 
 # the three volumes of the batch: (generator, seed, edge)
