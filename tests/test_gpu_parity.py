@@ -162,6 +162,41 @@ def test_render_modes_match_oracle(ren, mode, lit):
     assert ns_gpu == ns_ref
 
 
+@pytest.mark.parametrize("mode,lit,srm,lod", [(RM_1DTRANS, True, 2.0, 0), (RM_2DTRANS, True, 2.0, 0),
+                                               (RM_1DTRANS, False, 0.5, 0), (RM_1DTRANS, True, 4.0, 0),
+                                               (RM_1DTRANS, True, 0.7, 0), (RM_1DTRANS, True, 1.0, 1),
+                                               (RM_2DTRANS, True, 1.0, 2), (RM_2DTRANS, False, 2.0, 1)])
+def test_sample_rate_and_lod_match_oracle(ren, mode, lit, srm, lod):
+    """Opacity correction 1 - (1 - a)^s with s = (1/srm) x the level's domain
+    ratio (0.25 .. 4 here, and 1/0.7): frames and sample counts equal the
+    oracle's at other sample rates and at coarser levels."""
+    print(f"srm={srm} lod={lod} mode={mode} lit={lit}", flush=True)
+    n = 64
+    ren.GenerateVolume(GEN_BLOBS, 0, np.uint8, (n, n, n), (18, 18, 18), 1, max_lods=0, keep_linear=True)
+    vol = ren.DownloadVolume()
+    tf = tf1d_blue_white_red().GetByteArray()
+    ren.Set1DTrans(tf)
+    tf2 = tf2d_gaussian_lobes().GetByteArray()
+    ren.Set2DTrans(tf2)
+    ren.SetViewParameters(45.0, 0.01, 1000.0)
+    ren.SetViewPosition((0.2, -0.1, 2.0))
+    ren.Resize(96, 80)
+    ren.SetRendermode(mode)
+    ren.SetUseLighting(lit)
+    ren.SetGradientScale(1.0 / max(ren.MaxGradientMagnitude(), 1e-6))
+    ren.SetEarlyRayTermination(0.0)
+    ren.SetSampleRateModifier(srm)
+    ren.SetLOD(lod)
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    img, ref, mx, mean, ns_ref, ns_gpu = compare(ren, ds, tf1d=tf, tf2d=tf2)
+    print(f"  max {mx:.3g} mean {mean:.3g} samples oracle {ns_ref} gpu {ns_gpu}", flush=True)
+    ren.SetSampleRateModifier(1.0)
+    ren.SetLOD(0)
+    assert ref[..., 3].max() > 0.05
+    assert mx <= TOL_MAX and mean < TOL_MEAN
+    assert ns_gpu == ns_ref
+
+
 def test_blobs_256_full_frame(ren):
     """The 256^3 uint8 1D-TF workload at 512x512, whole frame against the oracle."""
     print("blobs 256^3, 512x512", flush=True)
