@@ -111,6 +111,8 @@ int orc_render(const orc_ds*, const orc_view*, const orc_params*, const int roi[
  * per-brick restart, continuous trilinear field, no early termination).      */
 int orc_render_fp64(const orc_ds*, const orc_view*, const orc_params*, const int roi[4],
                     double step_model, float* out_rgba, int n_threads);
+/* the opacity correction's pow: out[i] = pow(y[i], s[i]) as the march evaluates it */
+void orc_opacity_pow(const float* y, const float* s, int n, float* out);
 
 /* camera matrices (Vectors.h:1250-1277), row-vector layout */
 void orc_look_at(const float eye[3], const float at[3], const float up[3], float out[16]);

@@ -34,6 +34,24 @@ struct V3 {
   float minVal() const { return std::min(x, std::min(y, z)); }
 };
 inline float dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// GLSL pow(y, s) of the opacity correction 1 - (1 - a)^s (VRender1D.glsl:47-57,
+// OpacityCorrectColor of GLGridLeaper-blend.glsl; the built-in's precision is
+// implementation defined).  Exponents that are powers of two from 1/4 to 16 --
+// all that a sample-rate modifier of 2^k and a level's 2^k domain ratio give --
+// are evaluated by square roots and repeated squaring, any other by powf.  The
+// HIP kernels take the same steps, so the two agree bit for bit.
+inline float opacity_pow(float y, float s) {
+  if (s == 1.0f) return y;
+  if (s == 0.5f) return std::sqrt(y);
+  if (s == 0.25f) return std::sqrt(std::sqrt(y));
+  if (s == 2.0f || s == 4.0f || s == 8.0f || s == 16.0f) {
+    float r = y * y;
+    for (float e = 4.0f; e <= s; e *= 2.0f) r = r * r;
+    return r;
+  }
+  return std::pow(y, s);
+}
 inline float length(const V3& a) { return std::sqrt(dot(a, a)); }
 inline V3 cross(const V3& a, const V3& b) {  // Vectors.h operator% (cross product)
   return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);

@@ -448,7 +448,7 @@ void pass_pixel(const Leaper& L, int px, int py, const Ray& r, float out[4], flo
           }
           float c[4];
           color_at(L, tx, b, (b - tr) / sc, c);
-          c[3] = 1.0f - (oc == 1.0f ? 1.0f - c[3] : std::pow(1.0f - c[3], oc));   // OpacityCorrectColor
+          c[3] = 1.0f - opacity_pow(1.0f - c[3], oc);   // OpacityCorrectColor
           // UnderCompositing (Compositing.glsl:33-38)
           acc[0] += c[0] * (1.0f - acc[3]) * c[3];
           acc[1] += c[1] * (1.0f - acc[3]) * c[3];

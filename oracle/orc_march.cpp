@@ -110,8 +110,8 @@ struct Tf {
 inline V3 reflect(const V3& I, const V3& N) { return I - N * (2.0f * dot(N, I)); }
 // pow(x, 8.0) of lighting.glsl evaluated by repeated squaring
 inline float pow8(float x) { float x2 = x * x; float x4 = x2 * x2; return x4 * x4; }
-// 1 - pow(1 - a, s): pow(y, 1) is y exactly
-inline float opacity_correct(float a, float s) { float t = 1.0f - a; return 1.0f - (s == 1.0f ? t : std::pow(t, s)); }
+// 1 - pow(1 - a, s)
+inline float opacity_correct(float a, float s) { return 1.0f - opacity_pow(1.0f - a, s); }
 inline float clamp01(float v) { return std::min(std::max(v, 0.0f), 1.0f); }
 
 // GLSL normalize(): one reciprocal of the length, then three products (the GLSL
@@ -584,4 +584,9 @@ extern "C" int orc_render_fp64(const orc_ds* ds, const orc_view* v, const orc_pa
     }
   }
   return 0;
+}
+
+// opacity_pow for the tests that pin it against the closed form
+extern "C" void orc_opacity_pow(const float* y, const float* s, int n, float* out) {
+  for (int i = 0; i < n; ++i) out[i] = opacity_pow(y[i], s[i]);
 }

@@ -66,6 +66,8 @@ def lib():
     L.orc_ds_create_lod0.argtypes = [C.c_void_p, C.c_int, u64p, C.POINTER(C.c_uint32), C.c_uint32,
                                      C.c_int, C.POINTER(C.c_float)]
     L.orc_ds_destroy.argtypes = [C.c_void_p]
+    L.orc_opacity_pow.restype = None
+    L.orc_opacity_pow.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]
     L.orc_ds_lod_count.restype = C.c_int
     L.orc_ds_lod_count.argtypes = [C.c_void_p]
     L.orc_ds_brick_count.restype = C.c_uint64
@@ -285,6 +287,15 @@ def plan(ds, camera, params):
 def min_lod(ds, camera, sse=1.0):
     v = make_view(camera)
     return lib().orc_min_lod(ds.h, C.byref(v), sse)
+
+
+def opacity_pow(y, s):
+    """pow(y, s) of the oracle's opacity correction, elementwise (s broadcast)."""
+    y = np.ascontiguousarray(y, np.float32)
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(s, np.float32), y.shape))
+    out = np.empty_like(y)
+    lib().orc_opacity_pow(_p(y, C.c_float), _p(s, C.c_float), int(y.size), _p(out, C.c_float))
+    return out
 
 
 def render(ds, camera, params, roi=None, threads=0):

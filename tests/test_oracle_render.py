@@ -10,7 +10,9 @@ Checks chosen so that a plausible mistake fails one of them:
   * geometry / axes / gradient sign / lighting: mirror symmetry, and agreement
     with the FP64 integrator of the same emission-absorption model;
   * TF bytes and limits: the known truncation answer of the 1D workload TF;
-  * early ray termination: the error is bounded by the threshold.
+  * early ray termination: the error is bounded by the threshold;
+  * opacity correction pow(1 - a, s): against float64 within the rounding
+    steps of square roots and repeated squaring (powers of two) or powf.
 """
 import numpy as np
 import pytest
@@ -95,6 +97,27 @@ def test_constant_field_compositing_closed_form():
 
 def core_frac(brick, ov):
     return (brick - 2 * ov) / brick
+
+
+@pytest.mark.parametrize("s,c", [(0.25, 1.5), (0.5, 1.0), (1.0, 0.0), (2.0, 1.0), (4.0, 3.0), (8.0, 7.0),
+                                 (16.0, 15.0), (1.0 / 0.7, 4.0), (3.0, 4.0), (0.3, 4.0)])
+def test_opacity_pow_closed_form(s, c):
+    """pow(y, s) of the opacity correction against float64, relative error at
+    most c * 2^-24: a correctly rounded square root or product adds 2^-24, a
+    squaring doubles the error it is given (2^k - 1 for k squarings), a square
+    root halves it; other exponents take powf (2 ulp allowed)."""
+    rng = np.random.default_rng(5)
+    y = np.concatenate([np.linspace(0.0, 1.0, 4097), rng.uniform(0.0, 1.0, 20000), 1.0 - rng.uniform(0, 1e-3, 4000),
+                        rng.uniform(0.0, 1e-3, 4000)]).astype(np.float32)
+    s32 = np.float32(s)
+    got = O.opacity_pow(y, s32).astype(np.float64)
+    want = y.astype(np.float64) ** np.float64(s32)
+    err = np.abs(got - want)
+    # float32 underflow below 2^-126 (y^16 of small y) is not a rounding error of the steps
+    tol = c * 2.0 ** -24 * want + 2.0 ** -126
+    assert (err <= tol).all(), (s, float((err / np.maximum(want, 1e-300)).max() * 2 ** 24))
+    if s == 1.0:
+        assert np.array_equal(got, y.astype(np.float64))
 
 
 def test_mirror_symmetry():

@@ -34,6 +34,22 @@ __device__ __forceinline__ float sqrt_rn(float x) {
   const float rlo = fmaf(-lo, s, x), rhi = fmaf(-hi, s, x);
   return rhi > 0.0f ? hi : (rlo <= 0.0f ? lo : s);
 }
+// GLSL pow(y, s) of the opacity correction 1 - (1 - a)^s: exponents that are
+// powers of two from 1/4 to 16 (a sample-rate modifier of 2^k, a level's 2^k
+// domain ratio) by correctly rounded square roots and repeated squaring, any
+// other by powf -- the oracle's steps, so the two agree bit for bit.  s is the
+// same for every lane of a brick, so the branches are uniform in the march.
+__device__ __forceinline__ float opacity_pow(float y, float s) {
+  if (s == 1.0f) return y;
+  if (s == 0.5f) return sqrt_rn(y);
+  if (s == 0.25f) return sqrt_rn(sqrt_rn(y));
+  if (s == 2.0f || s == 4.0f || s == 8.0f || s == 16.0f) {
+    float r = y * y;
+    for (float e = 4.0f; e <= s; e *= 2.0f) r = r * r;
+    return r;
+  }
+  return powf(y, s);
+}
 // GLSL normalize(): one correctly rounded reciprocal of the length, then three products
 __device__ __forceinline__ V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }

@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
                 c[0] = lc.x; c[1] = lc.y; c[2] = lc.z;
               }
             }
-            c[3] = 1.0f - (oc == 1.0f ? 1.0f - c[3] : powf(1.0f - c[3], oc));
+            c[3] = 1.0f - opacity_pow(1.0f - c[3], oc);
             // UnderCompositing (Compositing.glsl:33-38)
             acc[0] += c[0] * (1.0f - acc[3]) * c[3];
             acc[1] += c[1] * (1.0f - acc[3]) * c[3];

@@ -606,7 +606,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
               s0 = lc.x; s1 = lc.y; s2 = lc.z;
             }
             const float tt = 1.0f - s3;
-            s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
+            s3 = 1.0f - opacity_pow(tt, ssc);
           } else {
             const float sx = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
             const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
@@ -624,11 +624,11 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
                 const V3 nn = normal_eye(gg, nm, dsc);
                 const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
                 const float tt = 1.0f - s3;
-                s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
+                s3 = 1.0f - opacity_pow(tt, ssc);
                 s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
               } else {
                 const float tt = 1.0f - s3;
-                s3 = 1.0f - (ssc == 1.0f ? tt : powf(tt, ssc));
+                s3 = 1.0f - opacity_pow(tt, ssc);
               }
             }
           }
