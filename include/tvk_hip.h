@@ -121,7 +121,9 @@ int tvk_default_params(tvk_params* p);   /* AbstrRenderer constructor defaults *
  * brick = stored brick size incl. 2*overlap, LOD pyramid by 2x box filter
  * (mean, truncating), per-brick min/max over all stored voxels.  max_lods = 0
  * builds every level down to 1x1x1 like the reference; k > 0 keeps the first
- * k levels.  keep_linear retains the linear LOD0 volume for tvk_download_volume. */
+ * k levels.  keep_linear retains the linear LOD0 volume for tvk_download_volume.
+ * Bricks are at most 4096 voxels a side and their blocked device layout at most
+ * 4 GiB (the kernels use 32-bit offsets inside a brick): TVK_ERR_ARG otherwise. */
 int tvk_load_volume(tvk_ctx* ctx, const void* host_voxels, int dtype, const uint64_t dims[3],
                     const uint32_t brick[3], uint32_t overlap, int clamp_to_edge, int max_lods,
                     int keep_linear);
