@@ -266,6 +266,42 @@ __device__ __forceinline__ void run2(const uint8_t* __restrict__ b, uint32_t r, 
   }
 }
 
+// sample_brick for the isosurface search: where every active lane's eight taps
+// lie inside the brick (no border colour, no clamping), each x pair is one
+// run2 load -- four loads instead of eight gathers -- and the weights, values
+// and float operations are sample_brick's; otherwise sample_brick itself.
+template <typename T>
+__device__ __forceinline__ float sample_brick_iso(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p) {
+  int i0[3];
+  float f[3];
+  const int n[3] = {nx, ny, nz};
+  const float pc[3] = {p.x, p.y, p.z};
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float s = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = s * (float)n[a] - 0.5f, fl = floorf(u);
+    i0[a] = (int)fl;
+    f[a] = u - fl;
+    in = in && i0[a] >= 0 && i0[a] + 1 < n[a];
+  }
+  if (__ballot(!in) != 0) return sample_brick<T>(d, nx, ny, nz, edge, p);
+  const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
+  const uint8_t* __restrict__ b = reinterpret_cast<const uint8_t*>(d);
+  const uint32_t x0 = ox<T>(i0[0]);
+  float c[2][2][2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      uint32_t v[2];
+      run2<T>(b, (x0 + oy<T>(i0[1] + j, nbx) + oz<T>(i0[2] + k, nbxy)) * (uint32_t)sizeof(T), v);
+      c[0][j][k] = (float)v[0];
+      c[1][j][k] = (float)v[1];
+    }
+  return tri_lerp(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1],
+                  f[0], f[1], f[2]) * Unorm<T>::inv();
+}
+
 // Interior form (every lane's 4x4x4 neighbourhood inside the brick): the four
 // central x-runs (x0-1 .. x0+2 at y0..y0+1, z0..z0+1) are one load each and
 // give the centre cube and the x arms; each y or z arm (x0, x0+1) is one more
@@ -552,7 +588,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         bool found = false;
         for (int i = 0; i < steps; ++i) {
           ++ns;
-          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= S.iso_n) { found = true; break; }
+          if (sample_brick_iso<T>(bd, nx, ny, nz, edge, pos) >= S.iso_n) { found = true; break; }
           pos = pos + incTex;
         }
         if (!found) continue;
@@ -560,7 +596,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         pos = pos - rd;
         for (int kk = 0; kk < S.refine_steps; ++kk) {
           rd = rd / 2.0f;
-          if (sample_brick<T>(bd, nx, ny, nz, edge, pos) >= S.iso_n) pos = pos - rd; else pos = pos + rd;
+          if (sample_brick_iso<T>(bd, nx, ny, nz, edge, pos) >= S.iso_n) pos = pos - rd; else pos = pos + rd;
         }
         const float param = length(pos - entryTex) / length(exitTex - entryTex);
         const V3 he = entry * (1.0f - param) + exitp * param;
