@@ -62,13 +62,56 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
 
 def other_workloads(learned=True):
     """Frame times of the other full-size workloads (reported beside the main
-    line, not part of its value): the 2048^3 uint8 SDF isosurface at 1080p,
+    line, not part of its value): device bricking of the 1024^3 uint16 volume
+    and one all-resident GridLeaper pass over it, the 2048^3 uint8 SDF isosurface at 1080p,
     the 48-request batch over three uint8 volumes in one persistent launch, and
     the 32-request batch over the 64 GiB uint16 volume with its 5-level pyramid
     in the HBM pool, also with the learned step proposal (learned=True).  Each
     is built, timed over a few frames and freed."""
     from tuvok_amd import workloads
     res = {}
+    try:
+        # device bricking of the 1024^3 uint16 volume from its kept linear copy
+        # (gather with overlap, extended rows, per-brick min/max), wall clock
+        c = dict(workloads.FBM_SHELLS_1024)
+        r = workloads.setup(c, keep_linear=True)
+        r.Rebrick(c["brick"], c["overlap"], max_lods=1)
+        ts, ks = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r.Rebrick(c["brick"], c["overlap"], max_lods=1)
+            ts.append(time.perf_counter() - t0)
+            ks.append(r.GetDatasetDesc().build_ms)
+        d = r.GetDatasetDesc()
+        vox = int(np.prod(c["dims"]))
+        stored = d.brick_count * int(np.prod(c["brick"])) * 2
+        t, k = float(np.median(ts)), float(np.median(ks))
+        # algorithmic bytes: the linear volume read once, every stored brick voxel written once
+        res["bricking_1024"] = dict(workload="fbm_shells1024_u16 -> 512 bricks of 130^3", device_ms=round(k, 3),
+                                    wall_ms=round(t * 1e3, 2), algorithmic_bytes=vox * 2 + stored,
+                                    gb_per_s=round((vox * 2 + stored) / (k / 1e3) / 1e9, 1),
+                                    hbm_frac=round((vox * 2 + stored) / (k / 1e3) / 1e9 / HBM_PEAK_GBS, 3))
+        # GridLeaper: one all-resident pass over every level, 1080p, lit 2D TF
+        r.Rebrick(c["brick"], c["overlap"], max_lods=0)
+        r.SetRendermode(c["mode"])
+        r.LeapSetResident(None)
+        r.SetCountSamples(True)
+        _, st = r.LeapRender(first=True)
+        samples = int(st.samples)
+        r.SetCountSamples(False)
+        lt = []
+        for _ in range(3):
+            _, st = r.LeapRender(first=True)
+            lt.append(st.ms)
+        m = float(np.median(lt))
+        res["leaper_1024"] = dict(workload="fbm_shells1024_u16, all levels resident, lit 2D TF, 1920x1080, one pass",
+                                  ms_per_pass=round(m, 3), samples=samples,
+                                  gsamples_per_s=round(samples / m / 1e6, 1))
+        r.close()
+    except Exception as e:
+        res["bricking_1024"] = dict(error=f"{type(e).__name__}: {e}")
+    print(f"[bench] bricking: {res.get('bricking_1024')} leaper: {res.get('leaper_1024')}", file=sys.stderr,
+          flush=True)
     try:
         c = workloads.SDF_ISO_2048
         r = workloads.setup(c)

@@ -102,6 +102,8 @@ typedef struct {
   uint64_t dims[3];
   uint32_t brick_size[3];
   uint32_t overlap;
+  float build_ms;        /* device time of the last bricking: gathers, pyramid,
+                            overlap fill, row copies, min/max                    */
 } tvk_dataset_desc;
 
 typedef struct {

@@ -49,7 +49,8 @@ class DatasetDesc(C.Structure):
     _fields_ = [("lod_count", C.c_int), ("brick_count", C.c_uint64), ("pool_bytes", C.c_uint64),
                 ("brick_bytes", C.c_uint64),
                 ("range_min", C.c_double), ("range_max", C.c_double), ("bit_width", C.c_int),
-                ("dims", C.c_uint64 * 3), ("brick_size", C.c_uint32 * 3), ("overlap", C.c_uint32)]
+                ("dims", C.c_uint64 * 3), ("brick_size", C.c_uint32 * 3), ("overlap", C.c_uint32),
+                ("build_ms", C.c_float)]
 
 
 class Tuning(C.Structure):

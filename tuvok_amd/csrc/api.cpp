@@ -48,6 +48,17 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
   void* tmp[2] = {nullptr, nullptr};
   const void* src = d_lin;
   int rc = TVK_OK;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) ev[0] = ev[1] = nullptr;
+  if (ev[0]) (void)hipEventRecord(ev[0], c->stream);
+  // per-brick min/max: folded into the gather; recomputed for the levels whose
+  // overlaps FillOverlap rewrites afterwards
+  uint32_t* d_mm = nullptr;
+  {
+    hipError_t e = hipMalloc(&d_mm, 2 * gb.size() * sizeof(uint32_t));
+    if (e != hipSuccess) rc = hipfail(c, "hipMalloc(minmax)", e);
+  }
+  bool refilled = false;
   for (size_t lod = 0; lod < keep && rc == TVK_OK; ++lod) {
     const LodInfo& li = L.lods[lod];
     if (lod > 0) {
@@ -69,9 +80,10 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
       maxv = std::max<uint32_t>(maxv, (uint32_t)t.count);
     }
     hipError_t e = launch_gather(src, bpv, li.size, d_gb + li.offset, nb, maxv, c->d_pool, core, L.overlap, c->clamp,
-                                 c->stream);
+                                 d_mm + 2 * li.offset, c->stream);
     if (e != hipSuccess) { rc = hipfail(c, "gather", e); break; }
     if (lod > 0 && !c->clamp && nb > 1) {
+      refilled = true;
       // reproduce FillOverlap's copy order on the coarse level (see bricking.hip)
       const uint32_t bxyz[3] = {(uint32_t)li.bricks[0], (uint32_t)li.bricks[1], (uint32_t)li.bricks[2]};
       std::vector<uint32_t> order;
@@ -104,22 +116,19 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
       if (e != hipSuccess) { rc = hipfail(c, "fill_overlap", e); break; }
     }
   }
-  if (rc == TVK_OK) {
-    // rows repeat the next block's first voxels (see tvk_types.hpp)
+  if (rc == TVK_OK && refilled && keep > 1) {
+    // the coarse levels' rows repeat voxels FillOverlap may have rewritten:
+    // refresh them (tvk_types.hpp) and their min/max
+    const size_t first = L.lods[1].offset;
     uint32_t maxv = 0;
-    for (const TocBrick& t : L.toc) maxv = std::max<uint32_t>(maxv, (uint32_t)t.count);
-    hipError_t e = launch_extend(c->d_pool, bpv, d_gb, (uint32_t)gb.size(), maxv, c->stream);
-    if (e != hipSuccess) rc = hipfail(c, "extend", e);
+    for (size_t i = first; i < L.toc.size(); ++i) maxv = std::max<uint32_t>(maxv, (uint32_t)L.toc[i].count);
+    hipError_t e = launch_extend(c->d_pool, bpv, d_gb + first, (uint32_t)(gb.size() - first), maxv, c->stream);
+    if (e == hipSuccess)
+      e = launch_minmax(c->d_pool, bpv, d_gb + first, (uint32_t)(gb.size() - first), maxv, d_mm + 2 * first,
+                        c->stream);
+    if (e != hipSuccess) rc = hipfail(c, "extend/minmax", e);
   }
-  uint32_t* d_mm = nullptr;
-  if (rc == TVK_OK) {
-    hipError_t e = hipMalloc(&d_mm, 2 * gb.size() * sizeof(uint32_t));
-    if (e != hipSuccess) rc = hipfail(c, "hipMalloc(minmax)", e);
-  }
-  if (rc == TVK_OK) {
-    hipError_t e = launch_minmax(c->d_pool, bpv, d_gb, (uint32_t)gb.size(), d_mm, c->stream);
-    if (e != hipSuccess) rc = hipfail(c, "minmax", e);
-  }
+  if (ev[1]) (void)hipEventRecord(ev[1], c->stream);
   std::vector<uint32_t> mm(2 * gb.size());
   if (rc == TVK_OK) {
     hipError_t e = hipMemcpyAsync(mm.data(), d_mm, mm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
@@ -127,6 +136,10 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
     if (e != hipSuccess) rc = hipfail(c, "minmax download", e);
   }
   (void)hipStreamSynchronize(c->stream);
+  c->build_ms = 0.0f;
+  if (ev[0] && ev[1] && rc == TVK_OK) (void)hipEventElapsedTime(&c->build_ms, ev[0], ev[1]);
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (d_mm) (void)hipFree(d_mm);
   (void)hipFree(d_gb);
   for (void* t : tmp) if (t) (void)hipFree(t);
@@ -412,6 +425,7 @@ int tvk_dataset_info(tvk_ctx* c, tvk_dataset_desc* o) {
   o->bit_width = bit_width(c);
   for (int i = 0; i < 3; ++i) { o->dims[i] = c->L.dims[i]; o->brick_size[i] = c->L.brick[i]; }
   o->overlap = c->L.overlap;
+  o->build_ms = c->build_ms;
   return TVK_OK;
 }
 

@@ -19,11 +19,12 @@ namespace tvk {
 hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st, int learn);
 hipError_t launch_batch(const BatchArgs& b, int mode, bool lit, int bpv, int groups, hipStream_t st, int learn);
 hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
-                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, hipStream_t st);
+                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, uint32_t* mm,
+                         hipStream_t st);
 hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
                              hipStream_t st);
-hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t* out,
-                         hipStream_t st);
+hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,
+                         uint32_t* out, hipStream_t st);
 hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bricks, const uint64_t* d_ref_off,
                             uint32_t nb, uint32_t max_vox, void* out, hipStream_t st);
 hipError_t launch_max_gradient(const void* lin, int bpv, const uint64_t d[3], int clamp, double* out, uint32_t blocks,
@@ -82,6 +83,7 @@ struct tvk_ctx {
   uint8_t* d_tf2d = nullptr;
   float4* d_tf1f = nullptr;    // tables as the sampler returns them (byte / 255)
   float4* d_tf2f = nullptr;
+  float build_ms = 0.0f;         // device time of the last bricking (build_pool)
   uint64_t* d_tf2cols = nullptr;  // 64 words: value columns of the 2D TF with a non-zero alpha
   size_t tf1d_cap = 0, tf2d_cap = 0, tf1f_cap = 0, tf2f_cap = 0;
   // view + parameters

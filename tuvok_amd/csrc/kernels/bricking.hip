@@ -50,35 +50,97 @@ __global__ __launch_bounds__(256) void extend_rows(T* __restrict__ pool, const G
   }
 }
 
-// grid: x = chunks of the brick's storage, y = brick.  Threads walk the brick in
-// storage order (coalesced stores); padding voxels repeat the last real voxel.
+// LOD brick gather through LDS.  A work item is one z/y block row of a brick
+// and a chunk of GATHER_CB blocks along x: its 16 x-rows (4 y by 4 z, clamped
+// to the brick: padding rows repeat the last one) are read from the linear
+// volume with consecutive lanes on consecutive voxels, then written as the
+// blocked layout's 16-byte rows (own W voxels + the W they repeat, the last
+// block's clamped to the brick's last voxel) with consecutive lanes on
+// consecutive rows.  When mm is given, the brick's min/max over the own voxels
+// (every stored voxel is the own voxel of exactly one row; padding repeats real
+// ones) is folded into mm[2 * brick] / mm[2 * brick + 1] (init_minmax first).
+// grid: x = work items of a brick (grid-stride), y = brick.
+constexpr int GATHER_CB = 64;
 template <typename T>
-__global__ __launch_bounds__(256) void gather_bricks(const T* __restrict__ lin, uint64_t X, uint64_t Y,
-                                                     uint64_t Z, const GatherBrick* __restrict__ bricks,
-                                                     T* __restrict__ pool, uint32_t core_x, uint32_t core_y,
-                                                     uint32_t core_z, uint32_t ov, int clamp) {
+__global__ __launch_bounds__(256) void gather_rows(const T* __restrict__ lin, uint64_t X, uint64_t Y, uint64_t Z,
+                                                   const GatherBrick* __restrict__ bricks, T* __restrict__ pool,
+                                                   uint32_t core_x, uint32_t core_y, uint32_t core_z, uint32_t ov,
+                                                   int clamp, uint32_t* __restrict__ mm) {
+  constexpr int lx = sizeof(T) == 1 ? 3 : 2, W = 1 << lx;
+  constexpr int L = GATHER_CB * W + W;          // voxels of one LDS row (8-byte multiple)
+  __shared__ __align__(16) T row[16][L];
   const GatherBrick b = bricks[blockIdx.y];
-  const int lx = lx_of<T>();
-  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2);
-  const uint64_t cnt = padded_count(b.n, lx);
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * 256) {
-    uint32_t x, y, z;
-    unswz(i, nbx, nby, lx, x, y, z);
-    x = min(x, b.n[0] - 1); y = min(y, b.n[1] - 1); z = min(z, b.n[2] - 1);
-    int64_t gx = (int64_t)b.c[0] * core_x - ov + x;
-    int64_t gy = (int64_t)b.c[1] * core_y - ov + y;
-    int64_t gz = (int64_t)b.c[2] * core_z - ov + z;
-    T v = 0;
-    if (clamp) {
-      gx = gx < 0 ? 0 : (gx >= (int64_t)X ? (int64_t)X - 1 : gx);
-      gy = gy < 0 ? 0 : (gy >= (int64_t)Y ? (int64_t)Y - 1 : gy);
-      gz = gz < 0 ? 0 : (gz >= (int64_t)Z ? (int64_t)Z - 1 : gz);
-      v = lin[(uint64_t)gx + X * ((uint64_t)gy + Y * (uint64_t)gz)];
-    } else if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int64_t)X && gy < (int64_t)Y && gz < (int64_t)Z) {
-      v = lin[(uint64_t)gx + X * ((uint64_t)gy + Y * (uint64_t)gz)];
+  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2), nbz = nblocks(b.n[2], 2);
+  const uint32_t nch = (nbx + GATHER_CB - 1) / GATHER_CB;
+  const uint32_t items = nch * nby * nbz;
+  const int64_t ox0 = (int64_t)b.c[0] * core_x - ov, oy0 = (int64_t)b.c[1] * core_y - ov;
+  const int64_t oz0 = (int64_t)b.c[2] * core_z - ov;
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  uint32_t mn = 0xffffffffu, mx = 0u;
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t ch = it % nch, yb = (it / nch) % nby, zb = it / (nch * nby);
+    const uint32_t bx0 = ch * GATHER_CB, nb = min((uint32_t)GATHER_CB, nbx - bx0);
+    const uint32_t x0 = bx0 << lx, nl = (nb << lx) + W;   // voxels loaded per row
+    __syncthreads();   // the previous item's rows are consumed
+    // wave wv loads rows 4wv .. 4wv+3 (y = r & 3, z = r >> 2 of the block row)
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t r = (uint32_t)(wv * 4 + k);
+      const uint32_t y = min((yb << 2) | (r & 3u), b.n[1] - 1), z = min((zb << 2) | (r >> 2), b.n[2] - 1);
+      int64_t gy = oy0 + y, gz = oz0 + z;
+      bool rin = true;
+      if (clamp) {
+        gy = gy < 0 ? 0 : (gy >= (int64_t)Y ? (int64_t)Y - 1 : gy);
+        gz = gz < 0 ? 0 : (gz >= (int64_t)Z ? (int64_t)Z - 1 : gz);
+      } else {
+        rin = gy >= 0 && gz >= 0 && gy < (int64_t)Y && gz < (int64_t)Z;
+      }
+      const T* __restrict__ src = lin + (rin ? X * ((uint64_t)gy + Y * (uint64_t)gz) : 0);
+      for (uint32_t xi = (uint32_t)ln; xi < nl; xi += 64) {
+        int64_t gx = ox0 + min(x0 + xi, b.n[0] - 1);
+        T v = 0;
+        if (clamp) {
+          gx = gx < 0 ? 0 : (gx >= (int64_t)X ? (int64_t)X - 1 : gx);
+          v = src[gx];
+        } else if (rin && gx >= 0 && gx < (int64_t)X) {
+          v = src[gx];
+        }
+        row[r][xi] = v;
+      }
     }
-    pool[b.offset + i] = v;
+    __syncthreads();
+    // consecutive lanes on consecutive 16-byte rows: own W voxels + the W repeated
+    const uint64_t blk0 = ((uint64_t)zb * nby + yb) * nbx + bx0;
+    for (uint32_t t = threadIdx.x; t < nb * 16u; t += 256) {
+      const uint32_t bl = t >> 4, r = t & 15u;
+      const uint2 own = *reinterpret_cast<const uint2*>(&row[r][bl << lx]);
+      const uint2 rep = *reinterpret_cast<const uint2*>(&row[r][(bl << lx) + W]);
+      const uint32_t ww[2] = {own.x, own.y};
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        const uint32_t v =
+            (ww[e * (int)sizeof(T) / 4] >> (8 * ((e * (int)sizeof(T)) & 3))) & ((1u << (8 * sizeof(T))) - 1u);
+        mn = min(mn, v);
+        mx = max(mx, v);
+      }
+      *reinterpret_cast<uint4*>(pool + b.offset + ((blk0 + bl) * 16u + r) * (2 * W)) =
+          make_uint4(own.x, own.y, rep.x, rep.y);
+    }
   }
+  if (mm) {
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, 64));
+      mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(mm + 2 * blockIdx.y, mn);
+      atomicMax(mm + 2 * blockIdx.y + 1, mx);
+    }
+  }
+}
+
+__global__ void init_minmax(uint32_t* mm, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { mm[2 * i] = 0xffffffffu; mm[2 * i + 1] = 0u; }
 }
 
 // blocked pool -> reference layout (x fastest, unpadded) at ref offsets
@@ -128,30 +190,34 @@ __global__ __launch_bounds__(256) void downsample(const T* __restrict__ src, uin
   }
 }
 
-// per-brick min/max over all stored voxels; one workgroup per brick
+// per-brick min/max over all stored voxels: one thread per 16-byte row (its own
+// W voxels; the repeated half and the padding hold copies of stored voxels),
+// folded into out[2 * brick] / out[2 * brick + 1] (initialised by init_minmax).
+// grid: x = chunks of the brick's rows, y = brick.
 template <typename T>
 __global__ __launch_bounds__(256) void brick_minmax(const T* __restrict__ pool, const GatherBrick* __restrict__ bricks,
                                                     uint32_t* __restrict__ out) {
-  const GatherBrick b = bricks[blockIdx.x];
-  const uint64_t nvox = padded_count(b.n, lx_of<T>());   // padding repeats real voxels
-  uint32_t mn = 0xffffffffu, mx = 0;
-  for (uint64_t i = threadIdx.x; i < nvox; i += 256) {
-    const uint32_t v = pool[b.offset + i];
-    mn = min(mn, v);
-    mx = max(mx, v);
+  constexpr int lx = sizeof(T) == 1 ? 3 : 2, W = 1 << lx;
+  const GatherBrick b = bricks[blockIdx.y];
+  const uint64_t rows = padded_count(b.n, lx) >> (lx + 1);
+  uint32_t mn = 0xffffffffu, mx = 0u;
+  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < rows; j += (uint64_t)gridDim.x * 256) {
+    const uint2 w = *reinterpret_cast<const uint2*>(pool + b.offset + j * (2 * W));   // the own half
+    const uint32_t ww[2] = {w.x, w.y};
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const uint32_t v = (ww[e * (int)sizeof(T) / 4] >> (8 * ((e * (int)sizeof(T)) & 3))) & ((1u << (8 * sizeof(T))) - 1u);
+      mn = min(mn, v);
+      mx = max(mx, v);
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, 64));
     mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
   }
-  __shared__ uint32_t smn[4], smx[4];
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) { smn[w] = mn; smx[w] = mx; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < 4; ++k) { mn = min(mn, smn[k]); mx = max(mx, smx[k]); }
-    out[2 * blockIdx.x] = mn;
-    out[2 * blockIdx.x + 1] = mx;
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(out + 2 * blockIdx.y, mn);
+    atomicMax(out + 2 * blockIdx.y + 1, mx);
   }
 }
 
@@ -330,18 +396,24 @@ hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bri
 }
 
 hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
-                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, hipStream_t st) {
-  uint32_t gx = (max_vox + 255) / 256;
-  if (gx > 2048) gx = 2048;
+                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, uint32_t* mm,
+                         hipStream_t st) {
+  // max_vox: the largest padded element count; one work item per 16 x 16-byte
+  // rows of GATHER_CB blocks
+  uint32_t gx = (uint32_t)(max_vox / (16u * 2u * GATHER_CB * (bpv == 1 ? 8u : 4u))) + 1u;
+  if (gx > 4096) gx = 4096;
+  if (mm) hipLaunchKernelGGL(init_minmax, dim3((nb + 255) / 256), dim3(256), 0, st, mm, nb);
   for (uint32_t first = 0; first < nb; first += 65535) {
     const uint32_t cnt = nb - first < 65535 ? nb - first : 65535;
     dim3 grid(gx, cnt);
     if (bpv == 1)
-      hipLaunchKernelGGL(gather_bricks<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)lin, dims[0], dims[1],
-                         dims[2], d_bricks + first, (uint8_t*)pool, core[0], core[1], core[2], ov, clamp);
+      hipLaunchKernelGGL(gather_rows<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)lin, dims[0], dims[1],
+                         dims[2], d_bricks + first, (uint8_t*)pool, core[0], core[1], core[2], ov, clamp,
+                         mm ? mm + 2 * first : nullptr);
     else
-      hipLaunchKernelGGL(gather_bricks<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)lin, dims[0], dims[1],
-                         dims[2], d_bricks + first, (uint16_t*)pool, core[0], core[1], core[2], ov, clamp);
+      hipLaunchKernelGGL(gather_rows<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)lin, dims[0], dims[1],
+                         dims[2], d_bricks + first, (uint16_t*)pool, core[0], core[1], core[2], ov, clamp,
+                         mm ? mm + 2 * first : nullptr);
   }
   return hipGetLastError();
 }
@@ -376,16 +448,20 @@ hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bric
   return hipGetLastError();
 }
 
-hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t* out,
-                         hipStream_t st) {
-  for (uint32_t first = 0; first < nb; first += 1u << 30) {
-    const uint32_t cnt = nb - first;
+hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,
+                         uint32_t* out, hipStream_t st) {
+  uint32_t gx = (max_vox / (bpv == 1 ? 16 : 8) + 255) / 256;
+  if (gx > 512) gx = 512;
+  if (gx == 0) gx = 1;
+  hipLaunchKernelGGL(init_minmax, dim3((nb + 255) / 256), dim3(256), 0, st, out, nb);
+  for (uint32_t first = 0; first < nb; first += 65535u) {
+    const uint32_t cnt = std::min(nb - first, 65535u);
     if (bpv == 1)
-      hipLaunchKernelGGL(brick_minmax<uint8_t>, dim3(cnt), dim3(256), 0, st, (const uint8_t*)pool, d_bricks + first,
-                         out + 2 * first);
+      hipLaunchKernelGGL(brick_minmax<uint8_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint8_t*)pool,
+                         d_bricks + first, out + 2 * first);
     else
-      hipLaunchKernelGGL(brick_minmax<uint16_t>, dim3(cnt), dim3(256), 0, st, (const uint16_t*)pool, d_bricks + first,
-                         out + 2 * first);
+      hipLaunchKernelGGL(brick_minmax<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint16_t*)pool,
+                         d_bricks + first, out + 2 * first);
   }
   return hipGetLastError();
 }
