@@ -12,6 +12,7 @@ bool Layout::build(const uint64_t d[3], const uint32_t b[3], uint32_t ov, int by
     dims[i] = d[i];
     brick[i] = b[i];
     if (d[i] == 0) { if (err) *err = "volume dimension is zero"; return false; }
+    if (d[i] > 0x7fffffffull) { if (err) *err = "volume dimension above 2^31-1"; return false; }
     if (b[i] <= 2 * ov) { if (err) *err = "brick size must exceed 2*overlap"; return false; }
     if (b[i] > 4096) { if (err) *err = "brick size above 4096"; return false; }
   }

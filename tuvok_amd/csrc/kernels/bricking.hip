@@ -94,14 +94,17 @@ __global__ __launch_bounds__(256) void gather_rows(const T* __restrict__ lin, ui
       } else {
         rin = gy >= 0 && gz >= 0 && gy < (int64_t)Y && gz < (int64_t)Z;
       }
+      // a row of the linear volume is addressed from one (wave-uniform) base
+      // with 32-bit voxel offsets (X is at most 2^31)
       const T* __restrict__ src = lin + (rin ? X * ((uint64_t)gy + Y * (uint64_t)gz) : 0);
+      const int32_t ox = (int32_t)ox0, xmax = (int32_t)X - 1;
       for (uint32_t xi = (uint32_t)ln; xi < nl; xi += 64) {
-        int64_t gx = ox0 + min(x0 + xi, b.n[0] - 1);
+        int32_t gx = ox + (int32_t)min(x0 + xi, b.n[0] - 1);
         T v = 0;
         if (clamp) {
-          gx = gx < 0 ? 0 : (gx >= (int64_t)X ? (int64_t)X - 1 : gx);
+          gx = min(max(gx, 0), xmax);
           v = src[gx];
-        } else if (rin && gx >= 0 && gx < (int64_t)X) {
+        } else if (rin && gx >= 0 && gx <= xmax) {
           v = src[gx];
         }
         row[r][xi] = v;
