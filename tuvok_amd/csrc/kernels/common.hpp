@@ -92,6 +92,47 @@ __device__ __forceinline__ uint32_t oz(int z, uint32_t nbxy) {
   return (((uint32_t)(z >> 2) * nbxy) << (L + 5)) + ((uint32_t)(z & 3) << (L + 3));
 }
 
+// Runs of voxels along x of the blocked layout.  A row holds its block's W
+// voxels followed by the next block's first W (16 bytes), so voxels s .. s+m-1
+// of a row (s < W, s + m <= 2W) are contiguous.
+//   run4: voxels s .. s+3 of the row at byte offset r, from the dword-aligned
+//         12 (uint16) or 8 (uint8) bytes that start at the dword holding voxel s;
+//   run2: voxels x .. x+1 from byte offset r of voxel x itself, one 4-byte
+//         (uint16) or 2-byte (uint8) under-aligned load: the amdhsa code
+//         object runs with unaligned global access enabled, and the two voxels
+//         never leave their 16-byte row.
+// Measured on the lit 2D-TF 1024^3 uint16 workload (16 views, 1080p): all
+// twelve loads as 16-byte rows 74.9 ms; these forms 71.0 ms; under-aligned
+// 8-byte loads for the four central runs as well 76.9 ms.  Values are stored
+// integers.
+typedef uint32_t tvk_u32x3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+typedef uint32_t tvk_u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef uint32_t tvk_u32_a1 __attribute__((aligned(1)));
+typedef uint16_t tvk_u16_a1 __attribute__((aligned(1)));
+template <typename T>
+__device__ __forceinline__ void run4(const uint8_t* __restrict__ b, uint32_t r, int s, uint32_t o[4]) {
+  if (sizeof(T) == 2) {
+    const tvk_u32x3_a4 w = *reinterpret_cast<const tvk_u32x3_a4*>(b + (r + 4u * (uint32_t)(s >> 1)));
+    const uint32_t sh = (uint32_t)(s & 1) * 2u;
+    const uint32_t lo = __builtin_amdgcn_alignbyte(w.y, w.x, sh), hi = __builtin_amdgcn_alignbyte(w.z, w.y, sh);
+    o[0] = lo & 0xffffu; o[1] = lo >> 16; o[2] = hi & 0xffffu; o[3] = hi >> 16;
+  } else {
+    const tvk_u32x2_a4 w = *reinterpret_cast<const tvk_u32x2_a4*>(b + (r + 4u * (uint32_t)(s >> 2)));
+    const uint32_t v = __builtin_amdgcn_alignbyte(w.y, w.x, (uint32_t)(s & 3));
+    o[0] = v & 0xffu; o[1] = (v >> 8) & 0xffu; o[2] = (v >> 16) & 0xffu; o[3] = v >> 24;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void run2(const uint8_t* __restrict__ b, uint32_t r, uint32_t o[2]) {
+  if (sizeof(T) == 2) {
+    const uint32_t w = *reinterpret_cast<const tvk_u32_a1*>(b + r);
+    o[0] = w & 0xffffu; o[1] = w >> 16;
+  } else {
+    const uint32_t w = *reinterpret_cast<const tvk_u16_a1*>(b + r);
+    o[0] = w & 0xffu; o[1] = w >> 8;
+  }
+}
+
 __device__ __forceinline__ V3 reflect(V3 I, V3 N) { return I - N * (2.0f * dot(N, I)); }
 
 }  // namespace tvk

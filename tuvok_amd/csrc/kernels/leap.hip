@@ -20,6 +20,18 @@ namespace {
 
 enum : uint32_t { BI_MISSING = 0, BI_CHILD_EMPTY = 1, BI_EMPTY = 2, BI_FLAG_COUNT = 3 };
 
+// trilinear blend of eight scaled taps c[x][y][z] with weights (1-f, f)
+__device__ __forceinline__ float leap_tri(float c000, float c100, float c010, float c110, float c001, float c101,
+                                          float c011, float c111, float f0, float f1, float f2) {
+  const float c00 = c000 * (1.0f - f0) + c100 * f0;
+  const float c10 = c010 * (1.0f - f0) + c110 * f0;
+  const float c01 = c001 * (1.0f - f0) + c101 * f0;
+  const float c11 = c011 * (1.0f - f0) + c111 * f0;
+  const float c0 = c00 * (1.0f - f1) + c10 * f1;
+  const float c1 = c01 * (1.0f - f1) + c11 * f1;
+  return c0 * (1.0f - f2) + c1 * f2;
+}
+
 // texture3D (GL_LINEAR, GL_CLAMP border 0) at voxel position b of a stored brick
 template <typename T>
 __device__ __forceinline__ float sample_vox(const T* __restrict__ d, int nx, int ny, int nz, V3 b) {
@@ -46,14 +58,98 @@ __device__ __forceinline__ float sample_vox(const T* __restrict__ d, int nx, int
   const float c010 = tap(vxa && vyb && vza, zA + yB + xA), c110 = tap(vxb && vyb && vza, zA + yB + xB);
   const float c001 = tap(vxa && vya && vzb, zB + yA + xA), c101 = tap(vxb && vya && vzb, zB + yA + xB);
   const float c011 = tap(vxa && vyb && vzb, zB + yB + xA), c111 = tap(vxb && vyb && vzb, zB + yB + xB);
-  const float c00 = c000 * (1.0f - f[0]) + c100 * f[0];
-  const float c10 = c010 * (1.0f - f[0]) + c110 * f[0];
-  const float c01 = c001 * (1.0f - f[0]) + c101 * f[0];
-  const float c11 = c011 * (1.0f - f[0]) + c111 * f[0];
-  const float c0 = c00 * (1.0f - f[1]) + c10 * f[1];
-  const float c1 = c01 * (1.0f - f[1]) + c11 * f[1];
-  return c0 * (1.0f - f[2]) + c1 * f[2];
+  return leap_tri(c000, c100, c010, c110, c001, c101, c011, c111, f[0], f[1], f[2]);
 }
+
+// sample_vox at b and gradient_vox's six taps around it from one 32-voxel
+// cross, bit for bit: valid where every tap is inside the brick and each
+// tap's index is the centre's +-1 (coords() tests it per lane).  The four
+// central x-runs give the centre cube and the x arms; the y and z arms are
+// loaded only by gradient().
+template <typename T>
+struct LeapCross {
+  int ic[3];
+  float fc[3], fm[3], fq[3];
+  float c[2][2][2], xl[2][2], xr[2][2];   // scaled taps, [x][y][z] / [y][z]
+  const uint8_t* base;
+  uint32_t nbx, nbxy;
+
+  __device__ __forceinline__ bool coords(int nx, int ny, int nz, V3 b) {
+    const int n[3] = {nx, ny, nz};
+    const float bc[3] = {b.x, b.y, b.z};
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float s = fminf(fmaxf(bc[a], 0.0f), (float)n[a]), u = s - 0.5f, fl = floorf(u);
+      ic[a] = (int)fl; fc[a] = u - fl;
+      s = fminf(fmaxf(bc[a] + -1.0f, 0.0f), (float)n[a]); u = s - 0.5f; fl = floorf(u);
+      const int im = (int)fl;
+      fm[a] = u - fl;
+      s = fminf(fmaxf(bc[a] + 1.0f, 0.0f), (float)n[a]); u = s - 0.5f; fl = floorf(u);
+      const int iq = (int)fl;
+      fq[a] = u - fl;
+      ok = ok && ic[a] >= 1 && ic[a] + 2 < n[a] && im == ic[a] - 1 && iq == ic[a] + 1;
+    }
+    return ok;
+  }
+  __device__ __forceinline__ uint32_t vox(int x, int y, int z) const {
+    return (ox<T>(x) + oy<T>(y, nbx) + oz<T>(z, nbxy)) * (uint32_t)sizeof(T);
+  }
+  __device__ __forceinline__ float value(const T* __restrict__ d, int nx, int ny) {
+    base = reinterpret_cast<const uint8_t*>(d);
+    nbx = nblocks((uint32_t)nx, Blk<T>::LX);
+    nbxy = nbx * nblocks((uint32_t)ny, 2);
+    const float inv = Unorm<T>::inv();
+    const int xs = ic[0] - 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        uint32_t v[4];
+        run4<T>(base, vox(xs & ~((1 << Blk<T>::LX) - 1), ic[1] + j, ic[2] + k), xs & ((1 << Blk<T>::LX) - 1), v);
+        xl[j][k] = (float)v[0] * inv;
+        c[0][j][k] = (float)v[1] * inv;
+        c[1][j][k] = (float)v[2] * inv;
+        xr[j][k] = (float)v[3] * inv;
+      }
+    return leap_tri(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1],
+                    fc[0], fc[1], fc[2]);
+  }
+  __device__ __forceinline__ V3 gradient() const {
+    const float inv = Unorm<T>::inv();
+    float yl[2][2], yr[2][2], zl[2][2], zr[2][2];   // [x][z] / [x][y]
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      uint32_t a[2], b[2];
+      run2<T>(base, vox(ic[0], ic[1] - 1, ic[2] + k), a);
+      run2<T>(base, vox(ic[0], ic[1] + 2, ic[2] + k), b);
+      yl[0][k] = (float)a[0] * inv; yl[1][k] = (float)a[1] * inv;
+      yr[0][k] = (float)b[0] * inv; yr[1][k] = (float)b[1] * inv;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      uint32_t a[2], b[2];
+      run2<T>(base, vox(ic[0], ic[1] + j, ic[2] - 1), a);
+      run2<T>(base, vox(ic[0], ic[1] + j, ic[2] + 2), b);
+      zl[0][j] = (float)a[0] * inv; zl[1][j] = (float)a[1] * inv;
+      zr[0][j] = (float)b[0] * inv; zr[1][j] = (float)b[1] * inv;
+    }
+    // gradient_vox's taps: xp = +x, xm = -x, yp = -y, ym = +y, zp = +z, zm = -z
+    const float xp = leap_tri(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
+                              fq[0], fc[1], fc[2]);
+    const float xm = leap_tri(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
+                              fm[0], fc[1], fc[2]);
+    const float yp = leap_tri(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
+                              fc[0], fm[1], fc[2]);
+    const float ym = leap_tri(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
+                              fc[0], fq[1], fc[2]);
+    const float zp = leap_tri(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
+                              fc[0], fc[1], fq[2]);
+    const float zm = leap_tri(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
+                              fc[0], fc[1], fm[2]);
+    return mk(xm - xp, yp - ym, zm - zp) / 2.0f;
+  }
+};
 
 // GLGridLeaper-GradientTools.glsl ComputeGradient with one-voxel taps
 template <typename T>
@@ -264,7 +360,12 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
           V3 b = pin;
           for (int i = 0; i < steps; ++i) {
             ++ns;
-            const float data = sample_vox<T>(bd, nx, ny, nz, b);
+            // value (and, for gradients, the cross) from the 32-voxel cross when
+            // every active lane's neighbourhood lies inside its brick
+            LeapCross<T> X;
+            const bool cross = MODE != 2 && (LIT || MODE == 1) && __ballot(!X.coords(nx, ny, nz, b)) == 0;
+            const float data = cross ? X.value(bd, nx, ny) : sample_vox<T>(bd, nx, ny, nz, b);
+            auto grad = [&]() { return cross ? X.gradient() : gradient_vox<T>(bd, nx, ny, nz, b); };
             if constexpr (MODE == 2) {
               // GLGridLeaper-iso.glsl: first sample at or above the isovalue,
               // then RefineIsosurface (GLGridLeaper-Method-iso.glsl) in brick voxels
@@ -291,25 +392,31 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
               const float4 tv = S.tf1[ti];
               c[0] = tv.x; c[1] = tv.y; c[2] = tv.z; c[3] = tv.w;
               if (LIT) {
-                V3 g = gradient_vox<T>(bd, nx, ny, nz, b) * dsc;
+                V3 g = grad() * dsc;
                 const float l = length(g);
                 if (l > 0.0f) g = mk(g.x / l, g.y / l, g.z / l);
                 const V3 lc = lighting_eye(meye, (b - tr) / sc, g, amb, mk(c[0], c[1], c[2]) * dif, spc, mlight);
                 c[0] = lc.x; c[1] = lc.y; c[2] = lc.z;
               }
             } else {
-              const V3 g = gradient_vox<T>(bd, nx, ny, nz, b);
-              const float gm = length(g);
               const float sx = fminf(fmaxf(data * S.tf_scale, 0.0f), 1.0f);
-              const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
               const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
-              const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
-              const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];
-              c[0] = tv.x; c[1] = tv.y; c[2] = tv.z; c[3] = tv.w;
-              if (LIT) {
-                const V3 nrm = dsc * (gm > 0.0f ? g / gm : g);
-                const V3 lc = lighting_eye(meye, (b - tr) / sc, nrm, amb, mk(c[0], c[1], c[2]) * dif, spc, mlight);
-                c[0] = lc.x; c[1] = lc.y; c[2] = lc.z;
+              if (((S.tf2_cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
+                // every row of this value column is transparent: alpha 0 whatever
+                // the gradient, so the sample composites to nothing
+                c[0] = c[1] = c[2] = c[3] = 0.0f;
+              } else {
+                const V3 g = grad();
+                const float gm = length(g);
+                const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
+                const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
+                const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];
+                c[0] = tv.x; c[1] = tv.y; c[2] = tv.z; c[3] = tv.w;
+                if (LIT) {
+                  const V3 nrm = dsc * (gm > 0.0f ? g / gm : g);
+                  const V3 lc = lighting_eye(meye, (b - tr) / sc, nrm, amb, mk(c[0], c[1], c[2]) * dif, spc, mlight);
+                  c[0] = lc.x; c[1] = lc.y; c[2] = lc.z;
+                }
               }
             }
             c[3] = 1.0f - opacity_pow(1.0f - c[3], oc);
