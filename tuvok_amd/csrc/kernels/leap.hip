@@ -61,6 +61,41 @@ __device__ __forceinline__ float sample_vox(const T* __restrict__ d, int nx, int
   return leap_tri(c000, c100, c010, c110, c001, c101, c011, c111, f[0], f[1], f[2]);
 }
 
+// sample_vox where every active lane's eight taps lie inside its brick: each
+// x pair is one run2 load (four loads instead of eight gathers), the taps,
+// weights and float operations are sample_vox's; otherwise sample_vox itself.
+template <typename T>
+__device__ __forceinline__ float sample_vox_fast(const T* __restrict__ d, int nx, int ny, int nz, V3 b) {
+  const int n[3] = {nx, ny, nz};
+  const float bc[3] = {b.x, b.y, b.z};
+  int i0[3];
+  float f[3];
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float s = fminf(fmaxf(bc[a], 0.0f), (float)n[a]), u = s - 0.5f, fl = floorf(u);
+    i0[a] = (int)fl;
+    f[a] = u - fl;
+    in = in && i0[a] >= 0 && i0[a] + 1 < n[a];
+  }
+  if (__ballot(!in) != 0) return sample_vox<T>(d, nx, ny, nz, b);
+  const float inv = Unorm<T>::inv();
+  const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
+  const uint8_t* __restrict__ base = reinterpret_cast<const uint8_t*>(d);
+  float c[2][2][2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      uint32_t v[2];
+      run2<T>(base, (ox<T>(i0[0]) + oy<T>(i0[1] + j, nbx) + oz<T>(i0[2] + k, nbxy)) * (uint32_t)sizeof(T), v);
+      c[0][j][k] = (float)v[0] * inv;
+      c[1][j][k] = (float)v[1] * inv;
+    }
+  return leap_tri(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1],
+                  f[0], f[1], f[2]);
+}
+
 // sample_vox at b and gradient_vox's six taps around it from one 32-voxel
 // cross, bit for bit: valid where every tap is inside the brick and each
 // tap's index is the centre's +-1 (coords() tests it per lane).  The four
@@ -364,7 +399,7 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
             // every active lane's neighbourhood lies inside its brick
             LeapCross<T> X;
             const bool cross = MODE != 2 && (LIT || MODE == 1) && __ballot(!X.coords(nx, ny, nz, b)) == 0;
-            const float data = cross ? X.value(bd, nx, ny) : sample_vox<T>(bd, nx, ny, nz, b);
+            const float data = cross ? X.value(bd, nx, ny) : sample_vox_fast<T>(bd, nx, ny, nz, b);
             auto grad = [&]() { return cross ? X.gradient() : gradient_vox<T>(bd, nx, ny, nz, b); };
             if constexpr (MODE == 2) {
               // GLGridLeaper-iso.glsl: first sample at or above the isovalue,
@@ -374,7 +409,7 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
                 V3 bb = b - rd;
                 for (int k = 0; k < S.refine_steps; ++k) {
                   rd = rd / 2.0f;
-                  if (sample_vox<T>(bd, nx, ny, nz, bb) >= S.iso_n) bb = bb - rd; else bb = bb + rd;
+                  if (sample_vox_fast<T>(bd, nx, ny, nz, bb) >= S.iso_n) bb = bb - rd; else bb = bb + rd;
                 }
                 // the normal is taken once the brick loop is over
                 hit_b = bb; hit_tr = tr; hit_sc = sc; hit_d = bd; hit_n3[0] = nx; hit_n3[1] = ny; hit_n3[2] = nz;

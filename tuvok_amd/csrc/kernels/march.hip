@@ -631,21 +631,31 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         for (int i = 0; i < nsamp; ++i) {
           ++ns;
           float s0, s1, s2, s3;
-          // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
-          bool inner = true;
-          {
-            const float pc[3] = {pos.x, pos.y, pos.z};
-            const int nn3[3] = {nx, ny, nz};
+          if constexpr (MODE == 0 && !LIT) {
+            // no gradient: the 2x2x2 taps alone decide the fast form
+            const float val = sample_brick_iso<T>(bd, nx, ny, nz, edge, pos);
+            const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
+            const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
+            const float4 tv = tf1[ti];
+            s0 = tv.x; s1 = tv.y; s2 = tv.z;
+            s3 = 1.0f - opacity_pow(1.0f - tv.w, ssc);
+          } else {
+            // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
+            bool inner = true;
+            {
+              const float pc[3] = {pos.x, pos.y, pos.z};
+              const int nn3[3] = {nx, ny, nz};
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
-              const int ia = (int)floorf(u);
-              inner = inner && ia >= 1 && ia + 2 < nn3[a];
+              for (int a = 0; a < 3; ++a) {
+                const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
+                const int ia = (int)floorf(u);
+                inner = inner && ia >= 1 && ia + 2 < nn3[a];
+              }
             }
+            pos_s = pos;
+            if (__ballot(!inner) == 0) shade(std::true_type{}, s0, s1, s2, s3);
+            else shade(std::false_type{}, s0, s1, s2, s3);
           }
-          pos_s = pos;
-          if (__ballot(!inner) == 0) shade(std::true_type{}, s0, s1, s2, s3);
-          else shade(std::false_type{}, s0, s1, s2, s3);
           // UnderCompositing (Compositing.glsl:33-38)
           const float wgt = (1.0f - c3) * s3;
           c0 += s0 * wgt;
