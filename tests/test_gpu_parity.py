@@ -162,6 +162,25 @@ def test_render_modes_match_oracle(ren, mode, lit):
     assert ns_gpu == ns_ref
 
 
+@pytest.mark.parametrize("mode,lit", [(RM_1DTRANS, False), (RM_1DTRANS, True), (RM_ISOSURFACE, False)])
+def test_uint16_modes_match_oracle(ren, mode, lit):
+    """uint16 volumes through the 1D (unlit: the 2x2x2 fast form with 4-byte
+    x-pair loads) and isosurface paths: frames and sample counts equal the
+    oracle's."""
+    print(f"uint16 mode={mode} lit={lit}", flush=True)
+    vol, tf = setup_blobs(ren, dtype=np.uint16)
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    ren.SetRendermode(mode)
+    ren.SetUseLighting(lit)
+    ren.SetIsoValue(0.4 * float(vol.max()))
+    ren.SetEarlyRayTermination(0.0)
+    img, ref, mx, mean, ns_ref, ns_gpu = compare(ren, ds, tf1d=tf)
+    print(f"  max {mx:.3g} mean {mean:.3g} samples oracle {ns_ref} gpu {ns_gpu}", flush=True)
+    assert ref[..., 3].max() > 0.05
+    assert mx <= TOL_MAX and mean < TOL_MEAN
+    assert ns_gpu == ns_ref
+
+
 @pytest.mark.parametrize("mode,lit,srm,lod", [(RM_1DTRANS, True, 2.0, 0), (RM_2DTRANS, True, 2.0, 0),
                                                (RM_1DTRANS, False, 0.5, 0), (RM_1DTRANS, True, 4.0, 0),
                                                (RM_1DTRANS, True, 0.7, 0), (RM_1DTRANS, True, 1.0, 1),
