@@ -939,10 +939,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TVK_LEARN_W
 }
 
 // Persistent batch launch: every wave pulls tiles from the per-XCD queues until
-// all are drained.  Requests in one launch share MODE, LIT and T.  The learned
-// variant takes the learned march's wave cap (at 4 waves per SIMD it spills).
+// all are drained.  Requests in one launch share MODE, LIT and T.  A cap of 3
+// waves per SIMD lets the lit variants keep their registers (at 4 the lit 2D
+// one spills 2 VGPRs and the learned one 41; measured 44.1 against 44.4 ms on
+// the 64 GiB LOD batch); the unlit ones need far fewer and run at more waves.
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARN == 1 ? TVK_LEARN_WAVES : 4))) void batch_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(
     BatchArgs B, const BatchRequest* __restrict__ req) {
   __shared__ __align__(16) uint16_t s_sn[LEARN > 0 ? 4 * SN_WAVE_LDS : 8];
   __shared__ uint64_t s_cols[MODE == 1 ? 4 * 64 : 1];
