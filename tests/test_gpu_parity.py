@@ -35,6 +35,10 @@ BRICK_CASES = [
     ((37, 21, 50), (12, 10, 16), 1, False, np.uint16),
     ((40, 36, 44), (12, 10, 10), 2, True, np.uint16),
     ((33, 5, 1), (12, 6, 6), 1, False, np.uint8),
+    # rows longer than one gather chunk (64 blocks), the middle brick inside the volume
+    ((600, 12, 10), (258, 6, 6), 1, False, np.uint16),
+    ((1100, 8, 6), (514, 6, 6), 1, True, np.uint8),
+    ((300, 19, 15), (66, 10, 10), 2, False, np.uint16),
 ]
 
 
@@ -63,6 +67,34 @@ def test_device_bricking_bit_exact(ren, dims, brick, ov, clamp, dt):
         oc, oe, on = ds.brick_md(0, lin)
         assert np.array_equal(gc, oc) and np.array_equal(ge, oe) and np.array_equal(gn, on)
     assert d.range_max == ds.range_max()
+
+
+def test_device_bricking_thin_cores():
+    """Cores thinner than the overlap (5-voxel bricks, overlap 2): every brick is
+    the zero-padded volume cut at coords * core - overlap.  The reference's
+    converter fills some of those overlap voxels from other voxels (its copy
+    order, not reproduced), so the oracle is compared on the cores only."""
+    dims, brick, ov = (12, 9, 7), (6, 5, 5), 2
+    vol = np.random.default_rng(12).integers(1, 256, size=dims[::-1]).astype(np.uint8)
+    ds = O.Dataset(vol, brick, ov)
+    r = HIPRaycaster(0)
+    try:
+        r.LoadVolume(vol, brick, ov, max_lods=1)
+        pool = r.GetPool()
+    finally:
+        r.close()
+    core = [b - 2 * ov for b in brick]
+    pad = np.zeros([d + 2 * ov + b for d, b in zip(dims[::-1], brick[::-1])], np.uint8)
+    pad[ov:ov + dims[2], ov:ov + dims[1], ov:ov + dims[0]] = vol
+    for i in lod0_bricks(ds):
+        info = ds.brick_info(i)
+        (nx, ny, nz), c, o = info["size"], info["coords"], info["offset"]
+        g = pool[o:o + nx * ny * nz].reshape(nz, ny, nx)
+        want = pad[c[2] * core[2]:c[2] * core[2] + nz, c[1] * core[1]:c[1] * core[1] + ny,
+                   c[0] * core[0]:c[0] * core[0] + nx]
+        assert np.array_equal(g, want), info
+        ref = np.asarray(ds.brick_data(i)).reshape(nz, ny, nx)
+        assert np.array_equal(g[ov:nz - ov, ov:ny - ov, ov:nx - ov], ref[ov:nz - ov, ov:ny - ov, ov:nx - ov]), info
 
 
 @pytest.mark.parametrize("dims,brick,dt", [((44, 36, 40), (10, 10, 10), np.uint8),
