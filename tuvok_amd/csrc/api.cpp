@@ -79,8 +79,8 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
       const TocBrick& t = L.toc[li.offset + b];
       maxv = std::max<uint32_t>(maxv, (uint32_t)t.count);
     }
-    hipError_t e = launch_gather(src, bpv, li.size, d_gb + li.offset, nb, maxv, c->d_pool, core, L.overlap, c->clamp,
-                                 d_mm + 2 * li.offset, c->stream);
+    hipError_t e = launch_gather(src, bpv, li.size, d_gb + li.offset, nb, (uint32_t)li.bricks[0], maxv, c->d_pool,
+                                 core, L.overlap, c->clamp, d_mm + 2 * li.offset, c->stream);
     if (e != hipSuccess) { rc = hipfail(c, "gather", e); break; }
     if (lod > 0 && !c->clamp && nb > 1) {
       refilled = true;

@@ -19,8 +19,8 @@ namespace tvk {
 hipError_t launch_march(const MarchArgs& a, int bpv, hipStream_t st, int learn);
 hipError_t launch_batch(const BatchArgs& b, int mode, bool lit, int bpv, int groups, hipStream_t st, int learn);
 hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
-                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, uint32_t* mm,
-                         hipStream_t st);
+                         uint32_t row_len, uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp,
+                         uint32_t* mm, hipStream_t st);
 hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
                              hipStream_t st);
 hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,

@@ -59,17 +59,28 @@ __global__ __launch_bounds__(256) void extend_rows(T* __restrict__ pool, const G
 // consecutive rows.  When mm is given, the brick's min/max over the own voxels
 // (every stored voxel is the own voxel of exactly one row; padding repeats real
 // ones) is folded into mm[2 * brick] / mm[2 * brick + 1] (init_minmax first).
-// grid: x = work items of a brick (grid-stride), y = brick.
+// A workgroup is one (brick, slot) pair: slot s takes the items s, s + gx, ...
+// of its brick.  Workgroups are dealt to the 8 XCDs in turn, so the linear
+// workgroup id is split into (XCD, sequence) and each XCD gets a contiguous run
+// of the order (brick row, slot, brick within the row of row_len bricks along
+// x): the bricks of one row run side by side on one XCD, and the 128-byte lines
+// that neighbouring bricks' row segments share are read through one L2.
+// grid: 8 * per_xcd workgroups; those past the nbrk bricks return at once.
 constexpr int GATHER_CB = 64;
 template <typename T>
 __global__ __launch_bounds__(256) void gather_rows(const T* __restrict__ lin, uint64_t X, uint64_t Y, uint64_t Z,
                                                    const GatherBrick* __restrict__ bricks, T* __restrict__ pool,
                                                    uint32_t core_x, uint32_t core_y, uint32_t core_z, uint32_t ov,
-                                                   int clamp, uint32_t* __restrict__ mm) {
+                                                   int clamp, uint32_t* __restrict__ mm, uint32_t nbrk, uint32_t gx,
+                                                   uint32_t row_len, uint32_t per_xcd) {
   constexpr int lx = sizeof(T) == 1 ? 3 : 2, W = 1 << lx;
   constexpr int L = GATHER_CB * W + W;          // voxels of one LDS row (8-byte multiple)
   __shared__ __align__(16) T row[16][L];
-  const GatherBrick b = bricks[blockIdx.y];
+  const uint32_t v = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+  const uint32_t bc = v % row_len, slot = (v / row_len) % gx, brow = v / (row_len * gx);
+  const uint64_t bi = (uint64_t)brow * row_len + bc;
+  if (bi >= nbrk) return;   // the whole workgroup, before any barrier
+  const GatherBrick b = bricks[bi];
   const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2), nbz = nblocks(b.n[2], 2);
   const uint32_t nch = (nbx + GATHER_CB - 1) / GATHER_CB;
   const uint32_t items = nch * nby * nbz;
@@ -77,7 +88,7 @@ __global__ __launch_bounds__(256) void gather_rows(const T* __restrict__ lin, ui
   const int64_t oz0 = (int64_t)b.c[2] * core_z - ov;
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   uint32_t mn = 0xffffffffu, mx = 0u;
-  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+  for (uint32_t it = slot; it < items; it += gx) {
     const uint32_t ch = it % nch, yb = (it / nch) % nby, zb = it / (nch * nby);
     const uint32_t bx0 = ch * GATHER_CB, nb = min((uint32_t)GATHER_CB, nbx - bx0);
     const uint32_t x0 = bx0 << lx, nl = (nb << lx) + W;   // voxels loaded per row
@@ -135,8 +146,8 @@ __global__ __launch_bounds__(256) void gather_rows(const T* __restrict__ lin, ui
       mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
-      atomicMin(mm + 2 * blockIdx.y, mn);
-      atomicMax(mm + 2 * blockIdx.y + 1, mx);
+      atomicMin(mm + 2 * bi, mn);
+      atomicMax(mm + 2 * bi + 1, mx);
     }
   }
 }
@@ -399,24 +410,31 @@ hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bri
 }
 
 hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const GatherBrick* d_bricks, uint32_t nb,
-                         uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp, uint32_t* mm,
-                         hipStream_t st) {
+                         uint32_t row_len, uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp,
+                         uint32_t* mm, hipStream_t st) {
   // max_vox: the largest padded element count; one work item per 16 x 16-byte
   // rows of GATHER_CB blocks
   uint32_t gx = (uint32_t)(max_vox / (16u * 2u * GATHER_CB * (bpv == 1 ? 8u : 4u))) + 1u;
   if (gx > 4096) gx = 4096;
+  if (row_len == 0) row_len = 1;
+  // bricks per launch: at most 2^23 workgroups (the grid's work-item count
+  // stays below 2^32), whole brick rows where a row fits
+  uint32_t per = std::max(1u, (1u << 23) / gx);
+  if (per >= row_len) per -= per % row_len;
   if (mm) hipLaunchKernelGGL(init_minmax, dim3((nb + 255) / 256), dim3(256), 0, st, mm, nb);
-  for (uint32_t first = 0; first < nb; first += 65535) {
-    const uint32_t cnt = nb - first < 65535 ? nb - first : 65535;
-    dim3 grid(gx, cnt);
+  for (uint32_t first = 0; first < nb; first += per) {
+    const uint32_t cnt = std::min(nb - first, per), rl = std::min(row_len, cnt);
+    const uint64_t wgs = (uint64_t)((cnt + rl - 1) / rl) * rl * gx;   // rows * row_len * slots
+    const uint32_t per_xcd = (uint32_t)((wgs + 7) / 8);
+    const dim3 grid(8u * per_xcd);
     if (bpv == 1)
       hipLaunchKernelGGL(gather_rows<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)lin, dims[0], dims[1],
                          dims[2], d_bricks + first, (uint8_t*)pool, core[0], core[1], core[2], ov, clamp,
-                         mm ? mm + 2 * first : nullptr);
+                         mm ? mm + 2 * first : nullptr, cnt, gx, rl, per_xcd);
     else
       hipLaunchKernelGGL(gather_rows<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)lin, dims[0], dims[1],
                          dims[2], d_bricks + first, (uint16_t*)pool, core[0], core[1], core[2], ov, clamp,
-                         mm ? mm + 2 * first : nullptr);
+                         mm ? mm + 2 * first : nullptr, cnt, gx, rl, per_xcd);
   }
   return hipGetLastError();
 }
