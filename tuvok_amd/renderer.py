@@ -373,7 +373,10 @@ class HIPRaycaster:
         features, transmittance before, step count, RGBA at x1/x2/x4/x8).  With
         fit=True and more selected chunks than cap, the views are marched again
         with the keep fraction that selects about cap of them (an unbiased
-        sample instead of the first cap)."""
+        sample instead of the first cap).  The device appends records in
+        whatever order its waves finish; they are returned sorted by a 64-bit
+        FNV-1a hash of their 48 words, so the same views give the same records
+        in the same order (and training on them is repeatable)."""
         rots = np.ascontiguousarray(np.stack([np.asarray(r, np.float32).reshape(4, 4) for r in rotations]))
         out = np.zeros((int(cap), 48), np.float32)
         n, total = C.c_uint32(), C.c_uint32()
@@ -384,7 +387,13 @@ class HIPRaycaster:
             self._chk(self._L.tvk_generate_segments(self._h, rots.shape[0], _fp(rots), int(cap), keep, int(seed),
                                                     _fp(out), C.byref(n), C.byref(total)))
         self.last_segments_total = int(total.value)
-        return out[:n.value]
+        out = out[:n.value]
+        w = out.view(np.uint32)
+        h = np.full(out.shape[0], 0xcbf29ce484222325, np.uint64)
+        for k in range(w.shape[1]):
+            h ^= w[:, k]
+            h *= np.uint64(0x100000001b3)
+        return out[np.argsort(h, kind="stable")]
 
     def Display(self, bg_top=(0.0, 0.0, 0.0), bg_bottom=None):
         """The last rendered frame(s) as the window shows them: over the
