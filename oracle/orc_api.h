@@ -28,7 +28,8 @@ typedef struct orc_ds orc_ds;
 orc_ds* orc_ds_create(const void* voxels, int bytes_per_voxel, const uint64_t dims[3],
                       const uint32_t brick[3], uint32_t overlap, int clamp, int median,
                       const float scale[3]);
-/* Same, but only LOD 0 is converted (the table still lists every level). */
+/* Same, but only LOD 0 is converted (the table still lists every level); with
+ * voxels NULL nothing is converted (metadata and brick table only). */
 orc_ds* orc_ds_create_lod0(const void* voxels, int bytes_per_voxel, const uint64_t dims[3],
                            const uint32_t brick[3], uint32_t overlap, int clamp, const float scale[3]);
 void     orc_ds_destroy(orc_ds*);
@@ -46,6 +47,10 @@ void     orc_ds_brick_md(const orc_ds*, int lod, uint64_t lin_index, float cente
 void     orc_ds_texcoords(const orc_ds*, int lod, uint64_t lin_index, int pow2,
                           float tmin[3], float tmax[3]);
 double   orc_ds_range_max(const orc_ds*);
+/* iteration order of the brick table: (lod, 1D index) pairs; returns the brick count */
+uint64_t orc_ds_table_order(const orc_ds*, uint64_t* out, uint64_t cap);
+/* VolumeTools::Filter (VolumeTools.h:168-262) over n groups of taps (2, 4, 8) values */
+void orc_filter(int taps, int median, int bpv, const void* in, uint64_t n, void* out);
 
 /* ---- a10 / a7: transfer functions -------------------------------------- */
 /* TransferFunction1D::GetByteArray (TransferFunction1D.cpp:311-330) */
@@ -100,6 +105,12 @@ typedef struct {
 int orc_plan(const orc_ds*, const orc_view*, const orc_params*, orc_brick* out, int cap);
 /* ComputeMinLODForCurrentView (AbstrRenderer.cpp:789-803; CullingLOD.cpp:126-138) */
 int orc_min_lod(const orc_ds*, const orc_view*, float screen_space_error);
+/* CullingLOD::Update's six planes (x, y, z, w each; CullingLOD.cpp:89-124), IsVisible
+ * (:141-160) and GetLODLevel (:126-138, unclamped) for the view's modelview */
+void orc_cull_planes(const orc_view*, float planes[24]);
+int orc_cull_visible(const orc_view*, const float center[3], const float extent[3]);
+int orc_cull_lod_level(const orc_view*, float screen_space_error, const float center[3], const float extent[3],
+                       const uint32_t voxels[3]);
 
 /* ---- a13 / a14: the march ---------------------------------------------- */
 /* Renders pixels [x0,x1) x [y0,y1) of the frame into out_rgba (RGBA32F,

@@ -526,7 +526,8 @@ orc_ds* orc_ds_create_lod0(const void* voxels, int bpv, const uint64_t dims[3], 
   t.overlap = overlap;
   t.bpv = (uint32_t)bpv;
   try {
-    t.convert((const uint8_t*)voxels, clamp != 0, false, true);
+    if (voxels) t.convert((const uint8_t*)voxels, clamp != 0, false, true);
+    else t.compute_metadata();   // metadata and brick table only (no voxels)
   } catch (...) {
     delete ds;
     return nullptr;
@@ -572,6 +573,38 @@ double orc_ds_range_max(const orc_ds* ds) {
 }
 
 float orc_round_fp16(float f) { return round_fp16(f); }
+
+// VolumeTools::Filter as the pyramid builder applies it: n groups of `taps`
+// values (2, 4 or 8), mean (double, truncated) or median; bpv 1 or 2.
+void orc_filter(int taps, int median, int bpv, const void* in, uint64_t n, void* out) {
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    const T* s = static_cast<const T*>(in);
+    T* d = static_cast<T*>(out);
+    for (uint64_t i = 0; i < n; ++i) {
+      const T* v = s + i * (uint64_t)taps;
+      if (taps == 8)
+        d[i] = median ? Conv::filter8<T, true>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7])
+                      : Conv::filter8<T, false>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+      else if (taps == 4)
+        d[i] = median ? Conv::filter4<T, true>(v[0], v[1], v[2], v[3]) : Conv::filter4<T, false>(v[0], v[1], v[2], v[3]);
+      else
+        d[i] = median ? Conv::filter2<T, true>(v[0], v[1]) : Conv::filter2<T, false>(v[0], v[1]);
+    }
+  };
+  if (bpv == 1) run(uint8_t()); else run(uint16_t());
+}
+
+// iteration order of the dataset's brick table: (lod, 1D index) per brick
+uint64_t orc_ds_table_order(const orc_ds* ds, uint64_t* out, uint64_t cap) {
+  uint64_t i = 0;
+  for (auto it = ds->table.begin(); it != ds->table.end(); ++it, ++i)
+    if (i < cap) {
+      out[2 * i] = std::get<1>(it->first);
+      out[2 * i + 1] = std::get<2>(it->first);
+    }
+  return i;
+}
 
 }  // extern "C"
 

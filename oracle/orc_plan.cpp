@@ -177,19 +177,41 @@ extern "C" int orc_plan(const orc_ds* ds, const orc_view* view, const orc_params
   return (int)list.size();
 }
 
+extern "C" void orc_cull_planes(const orc_view* view, float planes[24]) {
+  Cull cull;
+  cull.update(modelview_of(view), from_arr(view->proj));
+  for (int i = 0; i < 6; ++i)
+    for (int k = 0; k < 4; ++k) planes[4 * i + k] = cull.planes[i][k];
+}
+
+extern "C" int orc_cull_visible(const orc_view* view, const float c[3], const float e[3]) {
+  Cull cull;
+  cull.update(modelview_of(view), from_arr(view->proj));
+  return cull.visible(V3(c[0], c[1], c[2]), V3(e[0], e[1], e[2])) ? 1 : 0;
+}
+
+extern "C" int orc_cull_lod_level(const orc_view* view, float sse, const float c[3], const float e[3],
+                                  const uint32_t vox[3]) {
+  // CullingLOD::SetScreenParams (CullingLOD.cpp:57-67) and GetLODLevel (:126-138)
+  float lodFactor = 2.0f * std::tan(view->fov * ((3.1416f / 180.0f) / 2.0f)) * sse / float(view->height);
+  V3 ext(e[0], e[1], e[2]);
+  float lzero = (ext / V3((float)vox[0], (float)vox[1], (float)vox[2])).maxVal();
+  float zc = xform_point(V3(c[0], c[1], c[2]), modelview_of(view)).z;
+  float fz = std::max(view->znear, -zc);
+  return (int)std::floor(std::log(lodFactor * fz / lzero) / std::log(2.0f));
+}
+
 extern "C" int orc_min_lod(const orc_ds* ds, const orc_view* view, float sse) {
-  // AbstrRenderer.cpp:789-803 and CullingLOD.cpp:57-67, :126-138
+  // AbstrRenderer.cpp:789-803: the level of the whole domain at the origin,
+  // clamped to [0, levels - 1]
   const Octree& t = ds->tree;
   V3 dom((float)t.dims[0], (float)t.dims[1], (float)t.dims[2]);
   V3 sc(ds->scale[0], ds->scale[1], ds->scale[2]);
   V3 ext = dom * sc;
   ext = ext / ext.maxVal();
-  float lodFactor = 2.0f * std::tan(view->fov * ((3.1416f / 180.0f) / 2.0f)) * sse / float(view->height);
-  M4 mv = modelview_of(view);
-  float lzero = (ext / dom).maxVal();
-  float zc = xform_point(V3(0, 0, 0), mv).z;
-  float fz = std::max(view->znear, -zc);
-  int lvl = (int)std::floor(std::log(lodFactor * fz / lzero) / std::log(2.0f));
+  const float c0[3] = {0, 0, 0}, e0[3] = {ext.x, ext.y, ext.z};
+  const uint32_t vox[3] = {(uint32_t)t.dims[0], (uint32_t)t.dims[1], (uint32_t)t.dims[2]};
+  int lvl = orc_cull_lod_level(view, sse, c0, e0, vox);
   int maxl = (int)t.lods.size() - 1;
   return std::min(std::max(lvl, 0), maxl);
 }

@@ -129,18 +129,29 @@ def octree_meta(dims, brick, overlap, maxl=64):
 class Dataset:
     """Oracle dataset: ExtendedOctree conversion of a linear volume."""
 
-    def __init__(self, vol, brick, overlap, clamp=False, median=False, scale=(1, 1, 1), lod0_only=False):
+    def __init__(self, vol, brick, overlap, clamp=False, median=False, scale=(1, 1, 1), lod0_only=False,
+                 dims=None):
+        """vol: (z, y, x) uint8/uint16 array; or None with dims (x, y, z) for the
+        metadata and brick table alone (no voxels converted)."""
         L = lib()
-        vol = np.ascontiguousarray(vol)
-        assert vol.dtype in (np.uint8, np.uint16)
-        self.vol = vol
-        self.bpv = vol.dtype.itemsize
-        nz, ny, nx = vol.shape
-        self.dims = (nx, ny, nz)
+        if vol is None:
+            self.vol = None
+            self.bpv = 1
+            self.dims = tuple(int(v) for v in dims)
+        else:
+            vol = np.ascontiguousarray(vol)
+            assert vol.dtype in (np.uint8, np.uint16)
+            self.vol = vol
+            self.bpv = vol.dtype.itemsize
+            nz, ny, nx = vol.shape
+            self.dims = (nx, ny, nz)
         d = np.asarray(self.dims, np.uint64)
         b = np.asarray(brick, np.uint32)
         sc = np.asarray(scale, np.float32)
-        if lod0_only:
+        if vol is None:
+            self.h = L.orc_ds_create_lod0(None, 1, _p(d, C.c_uint64), _p(b, C.c_uint32), overlap, 0,
+                                          _p(sc, C.c_float))
+        elif lod0_only:
             self.h = L.orc_ds_create_lod0(vol.ctypes.data, self.bpv, _p(d, C.c_uint64), _p(b, C.c_uint32),
                                           overlap, int(clamp), _p(sc, C.c_float))
         else:
@@ -204,6 +215,20 @@ class Dataset:
 
     def range_max(self):
         return float(lib().orc_ds_range_max(self.h))
+
+    def _meta(self):
+        if not hasattr(self, "_m"):
+            self._m = octree_meta(self.dims, self.brick, self.overlap)
+        return self._m
+
+    def lod_bricks(self, lod):
+        return tuple(int(v) for v in self._meta()["lod_bricks"][lod])
+
+    def lod_size(self, lod):
+        return tuple(int(v) for v in self._meta()["lod_size"][lod])
+
+    def lod_offset(self, lod):
+        return int(self._meta()["lod_offset"][lod])
 
 
 def tf1d_bytes(rgba, used_range=255):
@@ -431,3 +456,56 @@ def leap_pass(ds, camera, params, eye, resident=None, first=True, resume_in=None
                          C.byref(ns))
     assert rc == 0, "brick grid of the pool differs from the octree's"
     return out, res, miss, ns.value
+
+
+# ---- culling, filter and brick-table order (checked against oracle/_ref) ------------
+def cull_planes(camera):
+    L = lib()
+    L.orc_cull_planes.argtypes = [C.POINTER(OrcView), C.POINTER(C.c_float)]
+    v = make_view(camera)
+    out = np.zeros(24, np.float32)
+    L.orc_cull_planes(C.byref(v), _p(out, C.c_float))
+    return out.reshape(6, 4)
+
+
+def cull_visible(camera, centers, extents):
+    L = lib()
+    L.orc_cull_visible.argtypes = [C.POINTER(OrcView), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_cull_visible.restype = C.c_int
+    v = make_view(camera)
+    c = np.ascontiguousarray(centers, np.float32)
+    e = np.ascontiguousarray(extents, np.float32)
+    return np.array([bool(L.orc_cull_visible(C.byref(v), _p(c[i], C.c_float), _p(e[i], C.c_float)))
+                     for i in range(c.shape[0])])
+
+
+def cull_lod_level(camera, centers, extents, voxels, sse=1.0):
+    L = lib()
+    L.orc_cull_lod_level.argtypes = [C.POINTER(OrcView), C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_uint32)]
+    L.orc_cull_lod_level.restype = C.c_int
+    v = make_view(camera)
+    c = np.ascontiguousarray(centers, np.float32)
+    e = np.ascontiguousarray(extents, np.float32)
+    n = np.ascontiguousarray(voxels, np.uint32)
+    return np.array([L.orc_cull_lod_level(C.byref(v), sse, _p(c[i], C.c_float), _p(e[i], C.c_float),
+                                          _p(n[i], C.c_uint32)) for i in range(c.shape[0])], np.int32)
+
+
+def filter_groups(taps, median, vals):
+    L = lib()
+    L.orc_filter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p]
+    vals = np.ascontiguousarray(vals)
+    out = np.zeros(vals.shape[0], vals.dtype)
+    L.orc_filter(taps, int(median), vals.dtype.itemsize, vals.ctypes.data, vals.shape[0], out.ctypes.data)
+    return out
+
+
+def table_order(ds):
+    L = lib()
+    L.orc_ds_table_order.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64]
+    L.orc_ds_table_order.restype = C.c_uint64
+    n = int(L.orc_ds_table_order(ds.h, None, 0))
+    out = np.zeros(2 * n, np.uint64)
+    L.orc_ds_table_order(ds.h, _p(out, C.c_uint64), n)
+    return out.reshape(-1, 2)

@@ -193,3 +193,61 @@ def test_isosurface_sphere_coverage():
     assert not hit[outside].any()
     # facing the camera the normal is ~(0,0,1): diffuse term saturates with the default lights
     assert img[W // 2, W // 2, 0] == pytest.approx(1.0, abs=1e-3)
+
+
+@pytest.mark.parametrize("mode,theta,light", [(0, 0.6, (0.8, -0.2, 0.56)), (0, -1.1, (-0.5, 0.4, -0.77)),
+                                              (1, 0.9, (0.6, 0.1, -0.79))])
+def test_lit_ramp_closed_form(mode, theta, light):
+    """Lit shading on a linear ramp along x (one brick, edge-clamped overlap and
+    CLAMP_TO_EDGE sampling): the gradient
+    points along -x everywhere, so every sample's normal is the rotated x axis and
+    its colour follows lighting.glsl in closed form: ambient + diffuse * tf.rgb *
+    |N.L| (+ specular * max(R.L, 0)^8 with R = reflect(normalize(-P), N)).  With no
+    specular term every pixel's rgb is that colour times its alpha; on the centre
+    ray of an odd-sized frame P lies on the view axis, so the specular term is a
+    constant too and the pixel is colour * (1 - (1 - a)^N)
+    (VRender1DLit.glsl:49-72, GLRaycaster-2D-light-FS.glsl:61-117, Volume3D.glsl:43-60)."""
+    n = 46
+    x = np.arange(n)
+    vol = np.broadcast_to((30 + 4 * x).astype(np.uint8), (n, n, n)).copy()
+    ds = O.Dataset(vol, (n + 2,) * 3, 1, clamp=True)   # overlap voxels repeat the edge
+    W = H = 33
+    rot = O.rotation_y(theta)
+    cam = O.camera(W, H, eye=(0.0, 0.0, 2.0), rot=rot)
+    rgb, a_byte = np.array([180, 120, 60]), 40
+    L = np.asarray(light, np.float64) / np.linalg.norm(light)
+    amb, dif, spc = np.array([0.1, 0.12, 0.08]), np.array([0.9, 0.8, 1.0]), np.array([0.5, 0.7, 0.6])
+    if mode == 0:
+        tf = np.zeros((256, 4), np.uint8)
+        tf[:, :3] = rgb
+        tf[:, 3] = a_byte
+        kw = dict(tf1d=tf)
+    else:
+        tf = np.zeros((256, 256, 4), np.uint8)
+        tf[..., :3] = rgb
+        tf[..., 3] = a_byte
+        kw = dict(tf2d=tf, grad_scale=3.0)
+    base = params(mode=mode, lighting=True, clamp_to_edge=1, light_dir=tuple(L), light_ambient=tuple(amb),
+                  light_diffuse=tuple(dif), **kw)
+    mv = (cam["rot"] @ cam["trans"] @ cam["view"]).astype(np.float64)
+    N = np.array([-1.0, 0.0, 0.0]) @ mv[:3, :3]
+    N /= np.linalg.norm(N)
+    tfc = rgb / 255.0
+    a = a_byte / 255.0
+    diffuse = amb + dif * tfc * abs(N @ L)
+    # whole frame, no specular term
+    img, _ = O.render(ds, cam, dict(base, light_specular=(0.0, 0.0, 0.0)))
+    alpha = img[..., 3]
+    hit = alpha > 0
+    assert hit.sum() > 200
+    want = np.clip(diffuse, 0, 1)[None, :] * alpha[hit][:, None]
+    assert np.abs(img[hit][:, :3] - want).max() < 2e-6
+    # centre ray with the specular term
+    I = np.array([0.0, 0.0, 1.0])
+    Rf = I - 2 * (N @ I) * N
+    col = np.clip(diffuse + spc * max(Rf @ L, 0.0) ** 8, 0, 1)
+    c, ns = O.render(ds, cam, dict(base, light_specular=tuple(spc)), roi=(W // 2, H // 2, W // 2 + 1, H // 2 + 1))
+    px = c[0, 0].astype(np.float64)
+    assert px[3] == pytest.approx(1 - (1 - a) ** ns, rel=2e-5)
+    assert np.abs(px[:3] - col * px[3]).max() < 2e-6
+    assert max(Rf @ L, 0.0) ** 8 > 0.05   # the specular term is exercised

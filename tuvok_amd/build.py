@@ -64,12 +64,6 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # copy kernel of known size used to calibrate the HBM byte counters in profiles
-    probe = os.path.join(PKG, "copy_probe")
-    r = subprocess.run([HIPCC, os.path.join(CSRC, "tools", "copy_probe.hip"), "-o", probe] + COMMON,
-                       capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"copy_probe build failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB
