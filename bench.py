@@ -29,7 +29,10 @@ ERT_EPS = 2.0 ** -11
 
 def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
     """The oracle (CPU restatement of the reference raycaster) on bands of rows of
-    the first view, from the middle outwards, until about budget_s of host time."""
+    the first view, from the middle outwards, until about budget_s of host time.
+    The oracle's rows are kept and compared with the HIP frame of the same view
+    (reference semantics, no early termination): max and mean per-channel error,
+    and, when every row was rendered, the composited-sample counts."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O
     from helpers import oracle_camera_from, oracle_params_from
@@ -45,19 +48,38 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
     # bands of rows ordered from the middle of the frame outwards
     starts = sorted(range(0, H, band), key=lambda r: abs(r + band // 2 - H // 2))
     ns_total, dt_total, rows = 0, 0.0, []
+    ref = np.zeros((H, W, 4), np.float32)
     for r0 in starts:
         if dt_total >= budget_s:
             break
         roi = (0, r0, W, min(H, r0 + band))
         t0 = time.perf_counter()
-        _, ns = O.render(ds, cam, p, roi=roi, threads=threads)
+        img, ns = O.render(ds, cam, p, roi=roi, threads=threads)
         dt_total += time.perf_counter() - t0
         ns_total += ns
+        ref[roi[1]:roi[3]] = img
         rows.append(roi)
     lo, hi = min(r[1] for r in rows), max(r[3] for r in rows)
+    # the HIP frame of view 0 with the reference semantics, sample-counted
+    eps = ren.GetParams().ert_eps
+    ren.SetEarlyRayTermination(0.0)
+    ren.SetCountSamples(True)
+    ren.SetRotation(views[0])
+    hip = ren.Paint()
+    hip_ns = int(ren.last_stats.samples)
+    ren.SetCountSamples(False)
+    ren.SetEarlyRayTermination(eps)
+    d = np.abs(hip[lo:hi].astype(np.float64) - ref[lo:hi].astype(np.float64))
+    parity = dict(rows=[lo, hi - 1], max_err=float(d.max()), mean_err=float(d.mean()),
+                  max_err_x255=round(float(d.max()) * 255, 5),
+                  samples_oracle=ns_total if (lo, hi) == (0, H) else None,
+                  samples_hip=hip_ns if (lo, hi) == (0, H) else None,
+                  samples_equal=(ns_total == hip_ns) if (lo, hi) == (0, H) else None)
+    print(f"[bench] view 0, HIP vs oracle: {parity}", file=sys.stderr, flush=True)
     return dict(value=ns_total / dt_total / 1e9, unit="Gsamples/s", cores=threads, kind="port",
                 sample=f"oracle (GL-faithful FP32 restatement, no early termination) on rows {lo}..{hi - 1} of view 0 "
-                       f"({W}x{hi - lo} rays, {ns_total} samples, {dt_total:.1f} s, {threads} threads)")
+                       f"({W}x{hi - lo} rays, {ns_total} samples, {dt_total:.1f} s, {threads} threads)",
+                view0_parity=parity)
 
 
 def other_workloads(learned=True):

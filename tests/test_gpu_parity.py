@@ -174,6 +174,56 @@ def test_plan_matches_oracle(ren):
     ren.SetRotation(np.eye(4, dtype=np.float32))
 
 
+def test_plan_of_a_culling_view_matches_oracle(ren):
+    """A close, off-axis view whose frustum cuts the volume: the culled list (fewer
+    bricks than the level holds) equals the oracle's item by item, order included."""
+    vol, tf = setup_blobs(ren)
+    ren.SetRendermode(RM_1DTRANS)
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    nlod0 = len(lod0_bricks(ds))
+    seen_culled = 0
+    for eye, at in (((0.35, 0.25, 0.45), (0.3, 0.2, 0.0)), ((-0.6, 0.1, 0.3), (-0.2, -0.3, 0.1)),
+                    ((0.1, 0.7, 0.2), (0.25, 0.0, -0.3))):
+        ren.SetViewPosition(eye, at=at)
+        got = ren.PlanFrame()
+        want = O.plan(ds, oracle_camera_from(ren), oracle_params_from(ren, tf1d=tf))
+        assert len(got) == len(want)
+        seen_culled += nlod0 - len(got)
+        for g, w in zip(got, want):
+            assert (g.key_index, g.toc_index, g.is_empty, g.distance) == (w.key_index, w.toc_index, w.is_empty,
+                                                                         w.distance)
+            assert list(g.center) == list(w.center) and list(g.tex_min) == list(w.tex_min)
+    assert seen_culled > 0
+    ren.SetViewPosition((0.2, -0.1, 2.0))
+
+
+def test_current_lod_matches_oracle_over_distances(ren):
+    """ComputeMinLODForCurrentView of the product (tvk_current_lod with lod = -1)
+    equals the oracle's as the eye moves from 1.5 to 12 volume widths, at two
+    viewport heights and screen-space errors; several levels are chosen."""
+    dims = (256, 256, 256)
+    ren.LoadVolume(np.zeros(dims[::-1], np.uint8), (66, 66, 66), 1, max_lods=0)
+    ds = O.Dataset(np.zeros(dims[::-1], np.uint8), (66, 66, 66), 1, lod0_only=True)
+    ren.Set1DTrans(tf1d_blue_white_red().GetByteArray())
+    ren.SetViewParameters(45.0, 0.01, 1000.0)
+    ren.SetLOD(-1)
+    seen = set()
+    try:
+        for sse in (1.0, 2.5):
+            ren.SetParams(screen_space_error=sse)
+            for dist in np.linspace(1.5, 12.0, 22):
+                for h in (270, 1080):
+                    ren.Resize(h * 16 // 9, h)
+                    ren.SetViewPosition((0.0, 0.0, float(dist)))
+                    got = ren.CurrentLOD()
+                    assert got == O.min_lod(ds, oracle_camera_from(ren), sse), (sse, dist, h)
+                    seen.add(got)
+    finally:
+        ren.SetLOD(0)
+        ren.SetParams(screen_space_error=1.0)
+    assert len(seen) >= 3
+
+
 @pytest.mark.parametrize("mode,lit", [(RM_1DTRANS, False), (RM_1DTRANS, True), (RM_2DTRANS, False),
                                       (RM_2DTRANS, True), (RM_ISOSURFACE, False)])
 def test_render_modes_match_oracle(ren, mode, lit):
