@@ -197,6 +197,23 @@ int tvk_set_count_samples(tvk_ctx* ctx, int on);
  * brick once). */
 int tvk_touched_bytes(tvk_ctx* ctx, uint64_t* bytes, uint64_t* bricks, uint64_t* unique_bytes,
                       uint64_t* unique_bricks);
+/* Allocation-free variant of tvk_render_views.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory while they
+ * render (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or allocating routine: the
+ * planned brick list, view constants and sample counter go to one static
+ * 8 MiB device array per GPU, and the frames (n x W x H RGBA32F, premultiplied,
+ * GL row order) to d_out, device memory of out_bytes bytes that the caller
+ * owns.  rots may be NULL when n == 1 (the context's rotation).  Limits: 302
+ * views and 52,019 visible non-empty bricks summed over the views; beyond
+ * them, or with a too small d_out, TVK_ERR_ARG.  Fixed step only (a context
+ * with a step proposal attached gets TVK_ERR_UNSUPPORTED); the dataset, TF and
+ * view must have been set up by the allocating calls beforehand.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU race.
+ * tvk_display and tvk_frame_device_ptr keep referring to the frames of
+ * tvk_render / tvk_render_views. */
+int tvk_render_views_noalloc(tvk_ctx* ctx, int n, const float* rots, float* d_out, size_t out_bytes,
+                             tvk_stats* stats);
 /* Device pointer of the last rendered frame(s) (for zero-copy consumers). */
 int tvk_frame_device_ptr(tvk_ctx* ctx, void** dptr);
 /* The last rendered frame(s) as the window shows them: drawn over the

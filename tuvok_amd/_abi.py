@@ -17,7 +17,7 @@ EXPORTS = [
     "tvk_brick_md", "tvk_brick_minmax", "tvk_get_brick", "tvk_get_pool", "tvk_max_gradient",
     "tvk_set_tf1d", "tvk_set_tf2d", "tvk_set_view_matrices", "tvk_set_camera", "tvk_set_rotation",
     "tvk_get_view_matrices", "tvk_set_params", "tvk_get_params", "tvk_current_lod", "tvk_plan",
-    "tvk_render", "tvk_render_views", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_cell_grid", "tvk_frame_device_ptr", "tvk_display",
+    "tvk_render", "tvk_render_views", "tvk_render_views_noalloc", "tvk_set_count_samples", "tvk_touched_bytes", "tvk_cell_grid", "tvk_frame_device_ptr", "tvk_display",
     "tvk_event_record", "tvk_event_elapsed", "tvk_synchronize",
     "tvk_default_tuning", "tvk_batch_create", "tvk_batch_destroy", "tvk_batch_last_error", "tvk_batch_clear",
     "tvk_batch_add", "tvk_batch_set_tuning", "tvk_batch_get_tuning", "tvk_batch_set_count_samples",
@@ -115,6 +115,7 @@ def load(path=LIB_PATH):
         "tvk_plan": (C.c_int, [vp, C.POINTER(BrickRec), C.c_int, C.POINTER(C.c_int)]),
         "tvk_render": (C.c_int, [vp, fp, C.POINTER(Stats)]),
         "tvk_render_views": (C.c_int, [vp, C.c_int, fp, fp, C.POINTER(Stats)]),
+        "tvk_render_views_noalloc": (C.c_int, [vp, C.c_int, fp, vp, C.c_size_t, C.POINTER(Stats)]),
         "tvk_set_count_samples": (C.c_int, [vp, C.c_int]),
         "tvk_touched_bytes": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
         "tvk_cell_grid": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_uint64, u64p]),

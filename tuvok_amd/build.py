@@ -30,6 +30,7 @@ SOURCES = [
     ("batch.cpp", ["-ffp-contract=off"]),
     ("stepnet_api.cpp", []),
     ("leap_api.cpp", ["-ffp-contract=off"]),
+    ("noalloc.hip", ["-ffp-contract=off"]),
 ]
 
 COMMON = [

@@ -216,6 +216,11 @@ int tvk_create(tvk_ctx** out, int device) {
     return TVK_ERR_HIP;
   }
   for (auto& ev : c->ev) (void)hipEventCreate(&ev);
+  // this device's address of the allocation-free render's scratch array
+  {
+    const hipError_t se = scratch_address(device, &c->scratch);
+    if (se != hipSuccess) c->scratch_err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(se);
+  }
   if (hipMalloc(&c->d_counter, sizeof(unsigned long long)) != hipSuccess) {
     delete c;
     return TVK_ERR_NOMEM;
@@ -895,16 +900,74 @@ void fill_shade(const tvk_ctx* c, ShadeConst& S) {
   S.overlap = (int)c->L.overlap;
 }
 
-// learn: 0 fixed step, 1 step proposal of c->net, 2 fixed step recording segment samples into *seg
-int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st, int learn = -1,
-                const SegOut* seg = nullptr) {
+// The launch of an allocation-free render: the planned lists go to the
+// caller's buffers (fixed), checked against their capacities.
+static int launch_fixed(tvk_ctx* c, int nviews, uint64_t pixels, uint32_t groups, uint32_t in_list,
+                        const FrameBuffers& fx, tvk_stats* st) {
+  if (c->h_list.size() > fx.list_cap)
+    return fail(c, TVK_ERR_ARG, "visible-brick list (" + std::to_string(c->h_list.size()) +
+                                    " entries) exceeds the allocation-free render's limit of " +
+                                    std::to_string(fx.list_cap));
+  if ((size_t)nviews > fx.views_cap)
+    return fail(c, TVK_ERR_ARG, std::to_string(nviews) + " views exceed the allocation-free render's limit of " +
+                                    std::to_string(fx.views_cap));
+  if ((size_t)pixels * 4 > fx.out_floats) return fail(c, TVK_ERR_ARG, "output buffer too small");
+  if (!c->h_list.empty())
+    HIPCHK(c, hipMemcpyAsync(fx.list, c->h_list.data(), c->h_list.size() * sizeof(FrameBrick), hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(fx.views, c->h_views.data(), (size_t)nviews * sizeof(ViewConst), hipMemcpyHostToDevice,
+                           c->stream));
+  MarchArgs a{};
+  fill_shade(c, a.S);
+  a.list = fx.list;
+  a.views = fx.views;
+  a.mode = c->P.mode;
+  a.out = fx.out;
+  a.counter = c->count ? fx.counter : nullptr;
+  a.ntoc = (uint32_t)c->L.toc.size();
+  a.touched = nullptr;
+  a.nviews = nviews;
+  a.lighting = c->P.lighting;
+  a.groups_total = groups;
+  a.groups_padded = (groups + 7u) & ~7u;
+  // a kernel, not hipMemsetAsync, clears the counter (capturable into a graph)
+  if (c->count) HIPCHK(c, launch_fill_u32((unsigned int*)fx.counter, 2u, 0u, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+  if (a.groups_total > 0) HIPCHK(c, launch_march(a, c->L.bpv, c->stream, 0));
+  HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    unsigned long long ns = 0;
+    if (c->count) HIPCHK(c, hipMemcpyAsync(&ns, fx.counter, sizeof(ns), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    st->samples = ns;
+    st->bricks_in_list = in_list;
+    st->bricks_nonempty = (uint32_t)c->h_list.size();
+    st->ms = ms;
+  }
+  return TVK_OK;
+}
+
+// learn: 0 fixed step, 1 step proposal of c->net, 2 fixed step recording segment samples into *seg.
+// fixed: device buffers the caller provides (allocation-free render): nothing
+// is allocated, the uploads come from the context's host vectors, the sample
+// counter is cleared by a kernel and the frames go to fixed->out.
+int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st, int learn,
+                const SegOut* seg, const FrameBuffers* fixed) {
   if (learn < 0) learn = (c->learn_on && c->P.mode != TVK_RM_ISOSURFACE) ? 1 : 0;
   if (learn > 0 && c->P.mode == TVK_RM_ISOSURFACE)
     return fail(c, TVK_ERR_UNSUPPORTED, "step proposal is not available in isosurface mode");
+  if (fixed && learn > 0)
+    return fail(c, TVK_ERR_UNSUPPORTED, "the allocation-free render marches at the fixed step (detach the proposal)");
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
   if (!c->has_view) return fail(c, TVK_ERR_STATE, "no view");
   if (nviews < 1) return fail(c, TVK_ERR_ARG, "nviews must be >= 1");
   HIPCHK(c, hipSetDevice(c->device));
+  // the host vectors are the upload source of an allocation-free render: the
+  // previous call's copies must have consumed them
+  if (fixed) HIPCHK(c, hipStreamSynchronize(c->stream));
   if (learn > 0) {
     int rc = learn_prepare(c);
     if (rc) return rc;
@@ -939,6 +1002,7 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
     c->last_lod = lod;
   }
   if (pixels >= (1ull << 32)) return fail(c, TVK_ERR_ARG, "too many pixels in one call");
+  if (fixed) return launch_fixed(c, nviews, pixels, groups, in_list, *fixed, st);
   int rc = ensure(c, &c->d_list, &c->list_cap, std::max<size_t>(c->h_list.size(), 1));
   if (!rc) rc = ensure(c, &c->d_views, &c->views_cap, (size_t)nviews);
   if (!rc) rc = ensure(c, &c->d_out, &c->out_cap, (size_t)pixels * 4);
@@ -1053,12 +1117,12 @@ int tvk_plan(tvk_ctx* c, tvk_brick_rec* out, int cap, int* n) {
 
 int tvk_render(tvk_ctx* c, float* host_out, tvk_stats* st) {
   if (!c) return TVK_ERR_ARG;
-  return render_impl(c, 1, nullptr, host_out, st);
+  return render_impl(c, 1, nullptr, host_out, st, -1, nullptr, nullptr);
 }
 
 int tvk_render_views(tvk_ctx* c, int n, const float* rots, float* host_out, tvk_stats* st) {
   if (!c || !rots) return TVK_ERR_ARG;
-  return render_impl(c, n, rots, host_out, st);
+  return render_impl(c, n, rots, host_out, st, -1, nullptr, nullptr);
 }
 
 int tvk_set_step_proposal(tvk_ctx* c, tvk_stepnet* net, const tvk_proposal_params* p) {
@@ -1104,7 +1168,7 @@ int tvk_generate_segments(tvk_ctx* c, int nviews, const float* rots, uint32_t ca
     so.seed = seed;
     const float eps = c->P.ert_eps;
     c->P.ert_eps = 0.0f;   // the recorded marches run without early termination
-    rc = render_impl(c, nviews, rots, nullptr, nullptr, 2, &so);
+    rc = render_impl(c, nviews, rots, nullptr, nullptr, 2, &so, nullptr);
     c->P.ert_eps = eps;
   }
   unsigned int cnt = 0;

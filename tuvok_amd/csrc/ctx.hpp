@@ -139,6 +139,10 @@ struct tvk_ctx {
   int last_views = 0;
   uint32_t last_w = 0, last_h = 0;
   int last_lod = 0;
+  // device address of this GPU's copy of the allocation-free render's scratch
+  // array (looked up once per device at tvk_create); null if the lookup failed
+  void* scratch = nullptr;
+  std::string scratch_err;
 };
 
 
@@ -196,5 +200,26 @@ int ensure_cells(tvk_ctx* c);
 bool stepnet_device(const struct tvk_stepnet* s, StepNetDev* out);
 // shading constants of the context's dataset, TF and parameters
 void fill_shade(const tvk_ctx* c, ShadeConst& S);
+
+// Device buffers of an allocation-free render (capacities in entries).
+struct FrameBuffers {
+  FrameBrick* list;
+  size_t list_cap;
+  ViewConst* views;
+  size_t views_cap;
+  unsigned long long* counter;
+  float* out;
+  size_t out_floats;
+};
+// Plans and renders nviews views (rots: 16 floats each, or null for the
+// context's rotation); learn < 0 takes the context's step proposal.  With
+// fixed set nothing is allocated and the frames go to fixed->out.
+int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st, int learn,
+                const SegOut* seg, const FrameBuffers* fixed);
+// sets n words at p to v with a kernel (capturable, unlike hipMemsetAsync)
+hipError_t launch_fill_u32(unsigned int* p, unsigned int n, unsigned int v, hipStream_t st);
+// this device's address of the allocation-free render's scratch array (call
+// with the device current; looked up once per device)
+hipError_t scratch_address(int device, void** out);
 
 }  // namespace tvk

@@ -521,8 +521,9 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
       ok = ok && tf > tn && tf > 0.0f;
       const float te = fmaxf(tn, 1.0f);  // near-plane entry when the front face is clipped
       const bool active = !done && ok && tf > te;
-      if (COUNT) {
-        // every ray that reaches a brick takes at least one sample there
+      if (COUNT && touched) {
+        // every ray that reaches a brick takes at least one sample there (the
+        // record is optional: the allocation-free render counts samples only)
         if (__ballot(active) != 0 && lane == __builtin_amdgcn_readfirstlane(lane)) touched[fb->toc] = 1u;
       }
       if (LEARN == 0 && !active) continue;
@@ -916,7 +917,7 @@ __device__ __forceinline__ void march_group(const MarchArgs& A) {
   const int wave = threadIdx.x >> 6;
   const int tx = (int)gx * 2 + (wave & 1), ty = (int)gy * 2 + (wave >> 1);
   march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
-                                         A.counter, COUNT ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
+                                         A.counter, (COUNT && A.touched) ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
                                          &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v,
                                          s_cols + (MODE == 1 ? wave * 64 : 0));
 }
@@ -972,7 +973,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
       const uint32_t ty = lt / R->tiles_x, tx = lt - ty * R->tiles_x;
       march_tile<MODE, LIT, T, COUNT, LEARN>(R->S, R->S.tf1, &R->V, B.list + R->V.list_offset, B.out,
                                              (int)(tx * B.tile_w), (int)(ty * B.tile_h), (int)B.tile_w,
-                                             (int)B.tile_h, B.counter, COUNT ? B.touched + R->touched_offset : nullptr,
+                                             (int)B.tile_h, B.counter, (COUNT && B.touched) ? B.touched + R->touched_offset : nullptr,
                                              &B.learn, nullptr, s_sn + (LEARN > 0 ? (threadIdx.x >> 6) * SN_WAVE_LDS : 0),
                                              0u, s_cols + (MODE == 1 ? (threadIdx.x >> 6) * 64 : 0));
     }
@@ -982,6 +983,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void b
 __global__ void fill_u32(unsigned int* p, unsigned int n, unsigned int v) {
   const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(unsigned int* p, unsigned int n, unsigned int v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32, dim3((n + 255u) / 256u), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
 }
 
 // host-side launchers (extern "C++" within the library)
