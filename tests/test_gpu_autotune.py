@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle_ffi as O
-from tuvok_amd import HIPRaycaster, workloads
+from tuvok_amd import HIPRaycaster, tf1d_blue_white_red, workloads
 from tuvok_amd.autotune import tune_brick_size
 
 pytestmark = pytest.mark.gpu
@@ -48,7 +48,18 @@ def test_oversized_bricks_are_rejected():
     vol = np.random.default_rng(22).integers(0, 256, size=(24, 20, 16)).astype(np.uint8)
     r = HIPRaycaster(0)
     r.LoadVolume(vol, (10, 10, 10), 1, max_lods=1, keep_linear=True)
-    for brick in ((4097, 8, 8), (2048, 2048, 2048)):
+    r.Set1DTrans(tf1d_blue_white_red().GetByteArray())
+    r.SetViewParameters(45.0, 0.01, 1000.0)
+    r.SetViewPosition((0.1, 0.2, 2.0))
+    r.Resize(40, 32)
+    before = r.Paint()
+    pool = r.GetPool()
+    # a rejected size, and an odd core with a pyramid (unsupported after the
+    # layout was accepted), both leave the dataset in place
+    for brick, lods in (((4097, 8, 8), 0), ((2048, 2048, 2048), 0), ((11, 11, 11), 0)):
         with pytest.raises(Exception, match="brick"):
-            r.Rebrick(brick, 1, max_lods=0)
+            r.Rebrick(brick, 1, max_lods=lods)
+        assert np.array_equal(r.Paint(), before)
+        assert np.array_equal(r.DownloadVolume(), vol)
+        assert np.array_equal(r.GetPool(), pool)
     r.close()

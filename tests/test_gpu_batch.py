@@ -277,3 +277,32 @@ def test_lod64g_learned_batch():
     assert res["chunks"] > 1000 and res["loss_last"] < res["loss_first"]
     assert res["max_err"] <= TOL_MAX
     owner.close()
+
+
+def test_batch_frames_follow_the_last_successful_render():
+    """A frame is returned only for a request of the last successful render: after
+    Clear and re-adding a larger request (no render yet) Frame is refused instead
+    of reading past the output buffer; after the render it is the new frame."""
+    a = HIPRaycaster(0)
+    a.GenerateVolume(GEN_FBM_SHELLS, 4, np.uint16, (64, 64, 64), (34, 34, 34), 1, max_lods=1, keep_linear=True)
+    a.Set2DTrans(tf2d_gaussian_lobes().GetByteArray())
+    a.SetRendermode(RM_2DTRANS)
+    a.SetGradientScale(1.0 / a.MaxGradientMagnitude())
+    a.SetViewParameters(45.0, 0.01, 1000.0)
+    a.SetViewPosition((0.0, 0.0, 2.0))
+    a.Resize(32, 24)
+    b = RenderBatch(0)
+    try:
+        b.Add(a, np.eye(4, dtype=np.float32), 32, 24)
+        b.Render()
+        assert b.Frame(0).shape == (24, 32, 4)
+        b.Clear()
+        b.Add(a, np.eye(4, dtype=np.float32), 300, 200)
+        with pytest.raises(Exception, match="no such request"):
+            b.Frame(0)
+        b.Render()
+        a.Resize(300, 200)
+        assert np.array_equal(b.Frame(0), a.Paint())
+    finally:
+        b.close()
+        a.close()

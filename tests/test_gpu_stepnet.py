@@ -99,7 +99,7 @@ def test_proposal_of_multiplier_one_equals_fixed_step(net):
 
 def test_segment_samples_are_consistent():
     r = fbm_renderer()
-    rec = r.GenerateSegments([rotation_y(a) for a in (0.0, 1.0, 2.0)], cap=200000, keep=0.5, seed=1)
+    rec = r.GenerateSegments([rotation_y(a) for a in (0.0, 1.0, 2.0)], cap=200000, keep=0.5, seed=1, fit=True)
     f, tb, steps, c = split_samples(rec)
     print(f"segments: {len(rec)}", flush=True)
     assert len(rec) > 1000

@@ -159,15 +159,56 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
   return TVK_OK;
 }
 
+// Validates the new layout first: a rejected size leaves the current dataset as it was.
 int begin_dataset(tvk_ctx* c, int dtype, const uint64_t dims[3], const uint32_t brick[3], uint32_t overlap,
                   int clamp, int max_lods) {
   if (dtype != TVK_U8 && dtype != TVK_U16) return fail(c, TVK_ERR_ARG, "dtype must be TVK_U8 or TVK_U16");
   if (!dims || !brick) return fail(c, TVK_ERR_ARG, "null dims/brick");
-  free_dataset(c);
+  Layout fresh;
   std::string e;
-  if (!c->L.build(dims, brick, overlap, dtype, max_lods, &e)) return fail(c, TVK_ERR_ARG, e);
+  if (!fresh.build(dims, brick, overlap, dtype, max_lods, &e)) return fail(c, TVK_ERR_ARG, e);
+  free_dataset(c);
+  c->L = std::move(fresh);
   c->clamp = clamp ? 1 : 0;
   return TVK_OK;
+}
+
+// The dataset fields of a context (the pool and linear volume are held, not copied).
+struct DatasetState {
+  Layout L;
+  int clamp;
+  void* d_pool;
+  std::shared_ptr<void> pool_hold, lin_hold;
+  std::shared_ptr<CellGrid> cells;
+  size_t pool_bytes;
+  void* d_lin;
+  size_t lin_bytes;
+  std::vector<double> minmax;
+  double range_min, range_max;
+  float build_ms;
+  bool has_ds;
+  std::vector<uint8_t> leap_resident;
+};
+DatasetState save_dataset(const tvk_ctx* c) {
+  return DatasetState{c->L, c->clamp, c->d_pool, c->pool_hold, c->lin_hold, c->cells, c->pool_bytes, c->d_lin,
+                      c->lin_bytes, c->minmax, c->range_min, c->range_max, c->build_ms, c->has_ds, c->leap_resident};
+}
+void restore_dataset(tvk_ctx* c, DatasetState&& s) {
+  c->L = std::move(s.L);
+  c->clamp = s.clamp;
+  c->d_pool = s.d_pool;
+  c->pool_hold = std::move(s.pool_hold);
+  c->lin_hold = std::move(s.lin_hold);
+  c->cells = std::move(s.cells);
+  c->pool_bytes = s.pool_bytes;
+  c->d_lin = s.d_lin;
+  c->lin_bytes = s.lin_bytes;
+  c->minmax = std::move(s.minmax);
+  c->range_min = s.range_min;
+  c->range_max = s.range_max;
+  c->build_ms = s.build_ms;
+  c->has_ds = s.has_ds;
+  c->leap_resident = std::move(s.leap_resident);
 }
 
 }  // namespace
@@ -371,18 +412,24 @@ int tvk_rebrick(tvk_ctx* c, const uint32_t brick[3], uint32_t overlap, int max_l
   if (!c->has_ds || !c->d_lin) return fail(c, TVK_ERR_STATE, "rebricking needs the linear volume (keep_linear)");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  // the linear volume outlives the old pool (contexts sharing the old pool keep it)
-  std::shared_ptr<void> lin = c->lin_hold;
-  void* d_lin = c->d_lin;
-  const size_t lin_bytes = c->lin_bytes;
+  // The current dataset is held until the new pool is complete: on any failure
+  // (a rejected size, an allocation) the context keeps rendering it.  The two
+  // pools coexist while the new one is built.
+  DatasetState old = save_dataset(c);
   const uint64_t dims[3] = {c->L.dims[0], c->L.dims[1], c->L.dims[2]};
   int rc = begin_dataset(c, c->L.bpv, dims, brick, overlap, c->clamp, max_lods);
-  c->lin_hold = lin;
-  c->d_lin = d_lin;
-  c->lin_bytes = lin_bytes;
+  if (rc) return rc;   // nothing was changed
+  c->lin_hold = old.lin_hold;   // the linear volume outlives the old pool
+  c->d_lin = old.d_lin;
+  c->lin_bytes = old.lin_bytes;
   c->leap_resident.clear();
-  if (!rc) rc = build_pool(c, c->d_lin);
-  if (rc) free_dataset(c);
+  rc = build_pool(c, c->d_lin);
+  if (rc) {
+    const std::string msg = c->err;
+    free_dataset(c);
+    restore_dataset(c, std::move(old));
+    c->err = msg;
+  }
   return rc;
 }
 

@@ -11,7 +11,7 @@
 using namespace tvk;
 
 namespace tvk {
-int batch_occupancy(int mode, bool lit, int bpv, int* groups_per_cu);
+int batch_occupancy(int mode, bool lit, int bpv, bool count, int learn, int* groups_per_cu);
 }
 
 struct tvk_batch {
@@ -28,8 +28,14 @@ struct tvk_batch {
   // step proposal for the TF-mode requests (nullptr: fixed step)
   struct tvk_stepnet* net = nullptr;
   LearnConst learn{};
-  // per-request results of the last render
-  std::vector<uint64_t> out_offset;   // first pixel of the request in d_out
+  // per-request results of the last successful render: first pixel in d_out
+  // and the image size it was rendered at (empty after clear, add or a failure)
+  struct Frame {
+    uint64_t offset;
+    uint32_t W, H;
+  };
+  std::vector<Frame> frames;
+  std::vector<Frame> planned_frames;   // of the render in progress
   // device buffers
   BatchRequest* d_req = nullptr;
   size_t req_cap = 0;
@@ -125,7 +131,8 @@ int plan_batch(tvk_batch* b, uint32_t* bricks_total) {
     std::vector<FrameBrick> list;
   };
   std::vector<Planned> P(b->reqs.size());
-  b->out_offset.assign(b->reqs.size(), 0);
+  b->frames.clear();
+  b->planned_frames.assign(b->reqs.size(), tvk_batch::Frame{0, 0, 0});
   uint64_t pixels = 0;
   std::vector<PlannedBrick> planned;
   for (size_t i = 0; i < b->reqs.size(); ++i) {
@@ -175,7 +182,7 @@ int plan_batch(tvk_batch* b, uint32_t* bricks_total) {
       return bfail(b, TVK_ERR_STATE, "request " + std::to_string(i) + ": no 1D transfer function");
     if (P[i].mode == TVK_RM_2DTRANS && !c->d_tf2f)
       return bfail(b, TVK_ERR_STATE, "request " + std::to_string(i) + ": no 2D transfer function");
-    b->out_offset[i] = pixels;
+    b->planned_frames[i] = tvk_batch::Frame{pixels, r.W, r.H};
     pixels += (uint64_t)r.W * r.H;
   }
   if (pixels >= (1ull << 32)) return bfail(b, TVK_ERR_ARG, "too many pixels in one batch");
@@ -235,7 +242,7 @@ int plan_batch(tvk_batch* b, uint32_t* bricks_total) {
     a.tile_h = th;
     int per_cu = b->tune.groups_per_cu;
     if (per_cu <= 0) {
-      hipError_t e = (hipError_t)batch_occupancy(L.mode, L.lit, L.bpv, &per_cu);
+      hipError_t e = (hipError_t)batch_occupancy(L.mode, L.lit, L.bpv, b->count, L.learn, &per_cu);
       if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     }
     L.groups = ((b->cu_count * per_cu) + 7) & ~7;
@@ -327,6 +334,7 @@ const char* tvk_batch_last_error(const tvk_batch* b) { return b ? b->err.c_str()
 int tvk_batch_clear(tvk_batch* b) {
   if (!b) return TVK_ERR_ARG;
   b->reqs.clear();
+  b->frames.clear();
   return TVK_OK;
 }
 
@@ -341,6 +349,7 @@ int tvk_batch_add(tvk_batch* b, tvk_ctx* c, const float rot[16], uint32_t w, uin
   r.W = w;
   r.H = h;
   b->reqs.push_back(r);
+  b->frames.clear();   // the last render's frames no longer match the request list
   if (id) *id = (int)b->reqs.size() - 1;
   return TVK_OK;
 }
@@ -420,6 +429,7 @@ int tvk_batch_render(tvk_batch* b, tvk_stats* st) {
   b->last_samples = ns;
   b->last_bricks = bricks;
   b->last_ms = ms;
+  b->frames = b->planned_frames;
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->samples = ns;
@@ -432,11 +442,11 @@ int tvk_batch_render(tvk_batch* b, tvk_stats* st) {
 
 int tvk_batch_frame(tvk_batch* b, int id, float* host_out) {
   if (!b || !host_out) return TVK_ERR_ARG;
-  if (id < 0 || (size_t)id >= b->out_offset.size() || b->out_offset.size() != b->reqs.size())
-    return bfail(b, TVK_ERR_RANGE, "no such request in the last render");
-  const tvk_batch::Req& r = b->reqs[(size_t)id];
-  BCHK(b, hipMemcpyAsync(host_out, b->d_out + 4 * b->out_offset[(size_t)id], (size_t)r.W * r.H * 4 * sizeof(float),
-                         hipMemcpyDeviceToHost, b->stream));
+  if (id < 0 || (size_t)id >= b->frames.size()) return bfail(b, TVK_ERR_RANGE, "no such request in the last render");
+  const tvk_batch::Frame& f = b->frames[(size_t)id];
+  const size_t floats = (size_t)f.W * f.H * 4;
+  if (!b->d_out || 4 * f.offset + floats > b->out_cap) return bfail(b, TVK_ERR_STATE, "frame outside the output buffer");
+  BCHK(b, hipMemcpyAsync(host_out, b->d_out + 4 * f.offset, floats * sizeof(float), hipMemcpyDeviceToHost, b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
   return TVK_OK;
 }
@@ -484,8 +494,8 @@ int tvk_batch_autotune(tvk_batch* b, int reps, tvk_tuning* best, float* best_ms)
 
 int tvk_batch_frame_device_ptr(tvk_batch* b, int id, void** p) {
   if (!b || !p) return TVK_ERR_ARG;
-  if (id < 0 || (size_t)id >= b->out_offset.size()) return bfail(b, TVK_ERR_RANGE, "no such request");
-  *p = b->d_out + 4 * b->out_offset[(size_t)id];
+  if (id < 0 || (size_t)id >= b->frames.size()) return bfail(b, TVK_ERR_RANGE, "no such request in the last render");
+  *p = b->d_out + 4 * b->frames[(size_t)id].offset;
   return TVK_OK;
 }
 

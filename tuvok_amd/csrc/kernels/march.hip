@@ -1045,20 +1045,26 @@ static void launch_batch_one(const BatchArgs& b, int groups, hipStream_t st, int
     hipLaunchKernelGGL((batch_kernel<M, L, T, false, 0>), dim3(groups), dim3(256), 0, st, b, b.req);
 }
 
+// occupancy of the exact instance launch_batch_one picks
 template <int M, bool L, typename T>
-static hipError_t occ_one(int* n) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, batch_kernel<M, L, T, false, 0>, 256, 0);
+static hipError_t occ_one(int* n, bool count, int learn) {
+  if (learn == 1 && M != 2)
+    return count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, batch_kernel<M, L, T, true, (M != 2 ? 1 : 0)>, 256, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, batch_kernel<M, L, T, false, (M != 2 ? 1 : 0)>, 256, 0);
+  return count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, batch_kernel<M, L, T, true, 0>, 256, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, batch_kernel<M, L, T, false, 0>, 256, 0);
 }
 
 // workgroups of the persistent kernel that fit on one CU at once
-int batch_occupancy(int mode, bool lit, int bpv, int* n) {
+int batch_occupancy(int mode, bool lit, int bpv, bool c, int l, int* n) {
   const bool u16 = bpv == 2;
   lit = lit && mode != 2;
-  if (mode == 0) return lit ? (u16 ? occ_one<0, true, uint16_t>(n) : occ_one<0, true, uint8_t>(n))
-                            : (u16 ? occ_one<0, false, uint16_t>(n) : occ_one<0, false, uint8_t>(n));
-  if (mode == 1) return lit ? (u16 ? occ_one<1, true, uint16_t>(n) : occ_one<1, true, uint8_t>(n))
-                            : (u16 ? occ_one<1, false, uint16_t>(n) : occ_one<1, false, uint8_t>(n));
-  return u16 ? occ_one<2, false, uint16_t>(n) : occ_one<2, false, uint8_t>(n);
+  if (mode == 2) l = 0;
+  if (mode == 0) return lit ? (u16 ? occ_one<0, true, uint16_t>(n, c, l) : occ_one<0, true, uint8_t>(n, c, l))
+                            : (u16 ? occ_one<0, false, uint16_t>(n, c, l) : occ_one<0, false, uint8_t>(n, c, l));
+  if (mode == 1) return lit ? (u16 ? occ_one<1, true, uint16_t>(n, c, l) : occ_one<1, true, uint8_t>(n, c, l))
+                            : (u16 ? occ_one<1, false, uint16_t>(n, c, l) : occ_one<1, false, uint8_t>(n, c, l));
+  return u16 ? occ_one<2, false, uint16_t>(n, c, 0) : occ_one<2, false, uint8_t>(n, c, 0);
 }
 
 // Clears the 8 queue counters with a kernel (graph-capturable) and launches the

@@ -382,10 +382,19 @@ class HIPRaycaster:
         n, total = C.c_uint32(), C.c_uint32()
         self._chk(self._L.tvk_generate_segments(self._h, rots.shape[0], _fp(rots), int(cap), float(keep), int(seed),
                                                 _fp(out), C.byref(n), C.byref(total)))
-        if fit and total.value > cap:
+        # When more chunks are selected than cap, the device keeps the first cap in
+        # the order its waves finish, which varies from run to run.  fit=True
+        # lowers the keep fraction until the whole selection fits; without fit
+        # the kept set is not repeatable, and that is an error.
+        tries = 0
+        while fit and total.value > cap and tries < 8:
             keep = float(keep) * 0.97 * cap / total.value
             self._chk(self._L.tvk_generate_segments(self._h, rots.shape[0], _fp(rots), int(cap), keep, int(seed),
                                                     _fp(out), C.byref(n), C.byref(total)))
+            tries += 1
+        if total.value > cap:
+            raise ValueError(f"{total.value} chunks selected for {cap} records: lower keep or pass fit=True "
+                             f"(the first {cap} in wave-finish order would not be repeatable)")
         self.last_segments_total = int(total.value)
         out = out[:n.value]
         w = out.view(np.uint32)
