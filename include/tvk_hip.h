@@ -157,7 +157,52 @@ int tvk_brick_minmax(tvk_ctx* ctx, uint64_t toc_index, double minmax[2]);
  * tvk_brick_info's pool_offset is the byte offset of a brick in it. */
 int tvk_get_brick(tvk_ctx* ctx, uint64_t toc_index, void* host_out, size_t bytes);
 int tvk_get_pool(tvk_ctx* ctx, void* host_out, size_t bytes);
-/* MaxGradientMagnitude over LOD0 core voxels (Histogram2DDataBlock.cpp:349-390) */
+/* ---- brick-level ingest and the budgeted pool -------------------------------
+ * A dataset handed over brick by brick, as a BrickedDataset holds it: the
+ * ExtendedOctree layout (dims, stored brick size incl. 2*overlap, overlap,
+ * lod_count levels, 0 = every level down to 1x1x1) and, per brick, its key
+ * and MaxMinForKey range (BrickedDataset::GetBrickMetadata / MaxMinForKey,
+ * IO/BrickedDataset.cpp:52-67, IO/uvfDataset.cpp:1183-1199).  Every brick of
+ * the layout is listed once (TVK_ERR_RANGE for a key outside it, the
+ * BrickedDataset domain_error).  budget_bytes = 0 (or a budget holding the
+ * whole pool): every brick resident in HBM, and rendering waits until every
+ * payload was uploaded.  Otherwise the bricks stay in pinned host memory and
+ * the HBM pool holds as many brick slots as the budget allows (at least 2):
+ * tvk_render pages the bricks of a frame in, least recently used slot first
+ * (GPUMemMan::GetVolume, Renderer/GPUMemMan/GPUMemMan.cpp:846-996), and renders
+ * a frame whose bricks exceed the slots in parts, in compositing order, with
+ * the same result; the traversal pages with GLVolumePool's policy.  Batches and
+ * the learned step need every brick resident. */
+typedef struct {
+  uint32_t lod;          /* BrickKey LOD                                        */
+  uint64_t index;        /* BrickKey 1D index within the LOD (x + y*bx + z*bx*by) */
+  double minmax[2];      /* MaxMinForKey: min and max over the stored voxels     */
+} tvk_brick_key;
+int tvk_set_bricks(tvk_ctx* ctx, int dtype, const uint64_t dims[3], const uint32_t brick[3], uint32_t overlap,
+                   int lod_count, const tvk_brick_key* keys, size_t n, uint64_t budget_bytes);
+/* One brick's payload, Dataset::GetBrick (IO/Dataset.h:93-100): the reference
+ * layout, x fastest, stored size incl. overlap (bytes must match). */
+int tvk_upload_brick(tvk_ctx* ctx, uint32_t lod, uint64_t index, const void* data, size_t bytes);
+/* Holds the current dataset within budget_bytes of HBM (as above; its bricks are
+ * copied to pinned host memory first), or, with 0, makes every brick resident
+ * again.  Contexts that shared the dataset before keep their own pool. */
+int tvk_set_pool_budget(tvk_ctx* ctx, uint64_t budget_bytes);
+typedef struct {
+  uint64_t pool_bytes;   /* HBM bytes of the pool                                    */
+  uint64_t slots;        /* brick slots (every brick when the pool is not budgeted)   */
+  uint64_t slot_bytes;
+  uint64_t resident;     /* slots holding a brick                                    */
+  uint64_t uploads, evictions, upload_bytes;   /* since the budget was set             */
+  float upload_ms;       /* host time spent in uploads                                */
+  uint32_t parts;        /* launches of the last budgeted frame                       */
+} tvk_pool_stats;
+int tvk_get_pool_stats(tvk_ctx* ctx, tvk_pool_stats* out);
+
+/* MaxGradientMagnitude of the dataset's 2D histogram, as RAWConverter computes it at
+ * level 0 (Histogram2DDataBlock.cpp:349-395, RAWConverter.cpp:655-660): the largest
+ * |(f(x-1)-f(x+1))/(2*(2^bits-1)), ...| over the core voxels of the LOD-0 bricks, with
+ * the neighbours from the bricks' overlap (a dataset without overlap uses the
+ * kept linear volume, zero outside).  fGradientScale is its inverse. */
 int tvk_max_gradient(tvk_ctx* ctx, float* out);
 
 /* ---- transfer functions --------------------------------------------------- */
@@ -280,16 +325,32 @@ typedef struct {
   uint64_t unfinished;   /* rays that left the requested resolution (resume later)   */
   float ms;              /* device time of the pass                                   */
 } tvk_leap_stats;
-/* residency per TOC brick (NULL: every brick resident); the coarsest level is always resident */
+/* residency per TOC brick (NULL: every brick resident, or, for a budgeted pool,
+ * only the coarsest brick); the coarsest level is always resident.  For a
+ * budgeted pool the flagged bricks are paged in (at most the slots less one). */
 int tvk_leap_set_resident(tvk_ctx* ctx, const uint8_t* flags, size_t n);
 /* one pass: first = 1 starts every ray at the volume box, 0 resumes the unfinished
  * rays of the previous pass (GLGridLeaper::Continue3DDraw) */
 int tvk_leap_render(tvk_ctx* ctx, int first, float* host_out, tvk_leap_stats* stats);
-/* bricks reported missing by the last pass (flags per TOC brick, may be NULL) */
+/* Missing bricks are requested through GLHashTable (Renderer/GL/GLHashTable.cpp:90-170):
+ * a table of `size` entries (0: sized from the largest brick as
+ * GLGridLeaper::InitHashTable does) holding the brick's 4D index + 1 at its value
+ * modulo size or one of the next `rehash` entries; requests that find no free
+ * entry are dropped for the pass.  Defaults 509 and 10 (MasterController.h:70-74).
+ * Coarser missing bricks met while falling back are requested too (BS_RequestAll). */
+int tvk_leap_set_request_table(tvk_ctx* ctx, uint32_t size, uint32_t rehash);
+/* the last pass's table (GLHashTable::GetData before condensing); *size = entries */
+int tvk_leap_requests(tvk_ctx* ctx, uint32_t* entries, size_t n, size_t* size);
+/* bricks requested by the last pass (flags per TOC brick, may be NULL) */
 int tvk_leap_missing(tvk_ctx* ctx, uint8_t* flags, size_t n, uint64_t* count);
-/* makes up to max_bricks reported bricks resident (the bricks already sit in the
- * HBM pool: paging is a metadata update, GLVolumePool::UploadBricks) */
+/* Pages up to max_bricks requested bricks in, in table order
+ * (GLVolumePool::UploadBricks, GLVolumePool.cpp:673-717, :955-959, :1720-1789).  With
+ * every brick in HBM this marks them resident; with a budgeted pool it copies them
+ * into the slots of the oldest uploads (at most all slots but the coarsest
+ * brick's per call), and the bricks replaced become missing. */
 int tvk_leap_page_in(tvk_ctx* ctx, uint64_t max_bricks, uint64_t* paged);
+/* residency per TOC brick: in a slot (budgeted pool), or flagged resident */
+int tvk_pool_resident(tvk_ctx* ctx, uint8_t* flags, size_t n);
 /* resume state of the last pass: per pixel 8 floats (normalised xyz, eye z or 1000 when finished, RGBA) */
 int tvk_leap_resume_state(tvk_ctx* ctx, float* host, size_t floats);
 

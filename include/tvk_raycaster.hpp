@@ -37,6 +37,29 @@ class Raycaster {
     chk(tvk_load_volume(ctx_, voxels, dtype, dims, brick, overlap, clamp_to_edge, max_lods, 0));
   }
 
+  // a BrickedDataset handed over brick by brick: the layout and every brick's
+  // key and MaxMinForKey range, then each Dataset::GetBrick payload
+  void SetBricks(int dtype, const uint64_t dims[3], const uint32_t brick[3], uint32_t overlap,
+                 const std::vector<tvk_brick_key>& keys, int lod_count = 0, uint64_t budget_bytes = 0) {
+    chk(tvk_set_bricks(ctx_, dtype, dims, brick, overlap, lod_count, keys.data(), keys.size(), budget_bytes));
+  }
+  void UploadBrick(uint32_t lod, uint64_t index, const void* data, size_t bytes) {
+    chk(tvk_upload_brick(ctx_, lod, index, data, bytes));
+  }
+  // GPUMemMan's memory budget for this dataset's bricks (0: all resident)
+  void SetPoolBudget(uint64_t bytes) { chk(tvk_set_pool_budget(ctx_, bytes)); }
+  tvk_pool_stats PoolStats() {
+    tvk_pool_stats s{};
+    chk(tvk_get_pool_stats(ctx_, &s));
+    return s;
+  }
+  // UVFDataset::MaxMinForKey by TOC index
+  std::array<double, 2> MaxMinForKey(uint64_t toc) {
+    std::array<double, 2> mm{};
+    chk(tvk_brick_minmax(ctx_, toc, mm.data()));
+    return mm;
+  }
+
   // AbstrRenderer::Set1DTrans / Changed2DTrans
   void Set1DTrans(const std::vector<unsigned char>& rgba) {
     chk(tvk_set_tf1d(ctx_, rgba.data(), (int)(rgba.size() / 4)));

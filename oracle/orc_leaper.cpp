@@ -369,6 +369,10 @@ void pass_pixel(const Leaper& L, int px, int py, const Ray& r, float out[4], flo
           ++lod;
           coords(lod);
           info = L.meta[L.id(bc[0], bc[1], bc[2], (int)lod)];
+          // RendererState::BS_RequestAll, the default strategy (MasterController.h:70-74,
+          // GLVolumePool.cpp:570-586): the coarser missing bricks are requested too
+          if (info == BI_MISSING)
+            __atomic_store_n(&missing[L.id(bc[0], bc[1], bc[2], (int)lod)], (uint8_t)1, __ATOMIC_RELAXED);
         } while (info == BI_MISSING);
       }
       const bool bEmpty = info <= BI_EMPTY;

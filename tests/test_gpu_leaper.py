@@ -148,3 +148,85 @@ def test_isosurface_passes_match_oracle():
     assert k > 0 and miss.sum() == 0
     r.LeapSetResident(None)
     r.close()
+
+
+def brick_serial(ds, t):
+    x, y, z, lod = ds.brick_info(t)["coords"]
+    bx, by, bz = ds.lod_bricks(0)
+    return 1 + x + y * bx + z * bx * by + lod * bx * by * bz
+
+
+@pytest.mark.parametrize("size,rehash", [(509, 10), (16, 2), (0, 10)])
+def test_request_table_matches_oracle_requests(size, rehash):
+    """GLHashTable semantics: every table entry is a brick the oracle's pass
+    requests (coarser missing bricks met while falling back included); a request
+    missing from the table found every one of its probe positions taken by
+    other bricks; a table as large as the default holds every request."""
+    r, ds, p, eye = setup("blobs", eye=(0.0, 0.0, 1.6))
+    r.SetUseLighting(False)
+    n, nl = ds.brick_count, ds.lod_count
+    coarse = np.array([ds.brick_info(i)["coords"][3] >= nl - 2 for i in range(n)], np.uint8)
+    r.LeapSetResident(coarse)
+    r.LeapSetRequestTable(size, rehash)
+    img, st = r.LeapRender(first=True)
+    table = r.LeapRequests()
+    miss = r.LeapMissing()
+    cam = oracle_camera_from(r)
+    ref, rres, rmiss, ns = O.leap_pass(ds, cam, oracle_params_from(r, **p), eye, coarse, True, None)
+    want = set(np.nonzero(rmiss)[0].tolist())
+    got = set(np.nonzero(miss)[0].tolist())
+    tsize = len(table)
+    print(f"  table {size}->{tsize} rehash {rehash}: {len(got)}/{len(want)} requests held", flush=True)
+    if size == 0:
+        # the size GLGridLeaper derives from the largest brick (18^3 stored voxels)
+        assert tsize == max(15, int(509 / (18 ** 3 / 32768.0)))
+    else:
+        assert tsize == size
+    assert got <= want and len(want) >= 18
+    serial = {brick_serial(ds, t): t for t in want}
+    entries = [int(e) for e in table if e]
+    assert len(entries) == len(set(entries)) and all(e in serial for e in entries)
+    for t in want - got:
+        v = brick_serial(ds, t)
+        probes = [(v + k) % tsize for k in range(max(rehash, 1))]
+        assert all(table[h] != 0 and table[h] != v for h in probes), t
+    if size == 509:
+        assert got == want
+    if size == 16:
+        assert len(got) < len(want)
+    mx, mean = err_stats(img, ref)
+    assert mx <= 1.0 / 255 and mean < 1e-3
+    r.close()
+
+
+def test_budgeted_pool_paging_matches_oracle_and_converges():
+    """The pool holds fewer slots than the dataset has bricks: each pass matches
+    the oracle for the bricks resident at that time, paging copies requested
+    bricks into the oldest slots (the coarsest brick stays), and the passes
+    converge to the all-resident frame."""
+    r, ds, p, eye = setup("blobs", eye=(0.0, 0.0, 1.6))
+    r.SetUseLighting(False)
+    full, _ = r.LeapRender(first=True)
+    nb = ds.brick_count
+    slot = 24 * 20 * 20 * 2          # 18^3 uint8 brick in the blocked layout, doubled rows
+    r.SetPoolBudget((nb - 3) * slot)
+    st = r.GetPoolStats()
+    assert st.slots == nb - 3
+    r.LeapSetResident(None)          # only the coarsest brick
+    resident = r.PoolResident()
+    assert resident.sum() == 1 and resident[-1] == 1
+    resume, first = None, True
+    for k in range(16):
+        resident = r.PoolResident()
+        img, res, miss = check_pass(r, ds, p, eye, first, resident, resume)
+        if miss.sum() == 0:
+            break
+        r.LeapPageIn()
+        assert r.PoolResident()[-1] == 1
+        resume, first = res, False
+    assert k > 0 and miss.sum() == 0
+    mx, mean = err_stats(img, full)
+    print(f"  budgeted paging converged after {k + 1} passes: max {mx:.3g} mean {mean:.3g}, "
+          f"uploads {r.GetPoolStats().uploads}", flush=True)
+    assert mx <= 2.0 / 255 and mean < 1e-3
+    r.close()

@@ -27,6 +27,8 @@ EXPORTS = [
     "tvk_stepnet_grad", "tvk_stepnet_train", "tvk_set_step_proposal", "tvk_generate_segments",
     "tvk_batch_set_step_proposal", "tvk_leap_set_resident", "tvk_leap_render", "tvk_leap_missing",
     "tvk_leap_page_in", "tvk_leap_resume_state",
+    "tvk_set_bricks", "tvk_upload_brick", "tvk_set_pool_budget", "tvk_get_pool_stats",
+    "tvk_leap_set_request_table", "tvk_leap_requests", "tvk_pool_resident",
 ]
 
 
@@ -64,6 +66,16 @@ class ProposalParams(C.Structure):
 
 class LeapStats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("missing", C.c_uint64), ("unfinished", C.c_uint64), ("ms", C.c_float)]
+
+
+class BrickKey(C.Structure):
+    _fields_ = [("lod", C.c_uint32), ("index", C.c_uint64), ("minmax", C.c_double * 2)]
+
+
+class PoolStats(C.Structure):
+    _fields_ = [("pool_bytes", C.c_uint64), ("slots", C.c_uint64), ("slot_bytes", C.c_uint64),
+                ("resident", C.c_uint64), ("uploads", C.c_uint64), ("evictions", C.c_uint64),
+                ("upload_bytes", C.c_uint64), ("upload_ms", C.c_float), ("parts", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -155,6 +167,14 @@ def load(path=LIB_PATH):
         "tvk_leap_missing": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_size_t, u64p]),
         "tvk_leap_page_in": (C.c_int, [vp, C.c_uint64, u64p]),
         "tvk_leap_resume_state": (C.c_int, [vp, fp, C.c_size_t]),
+        "tvk_set_bricks": (C.c_int, [vp, C.c_int, u64p, u32p, C.c_uint32, C.c_int, C.POINTER(BrickKey), C.c_size_t,
+                                     C.c_uint64]),
+        "tvk_upload_brick": (C.c_int, [vp, C.c_uint32, C.c_uint64, vp, C.c_size_t]),
+        "tvk_set_pool_budget": (C.c_int, [vp, C.c_uint64]),
+        "tvk_leap_set_request_table": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+        "tvk_leap_requests": (C.c_int, [vp, u32p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "tvk_pool_resident": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_size_t]),
+        "tvk_get_pool_stats": (C.c_int, [vp, C.POINTER(PoolStats)]),
         "tvk_generate_segments": (C.c_int, [vp, C.c_int, fp, C.c_uint32, C.c_float, C.c_uint32, fp,
                                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     }

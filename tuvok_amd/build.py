@@ -31,6 +31,7 @@ SOURCES = [
     ("stepnet_api.cpp", []),
     ("leap_api.cpp", ["-ffp-contract=off"]),
     ("noalloc.hip", ["-ffp-contract=off"]),
+    ("pool.cpp", ["-ffp-contract=off"]),
 ]
 
 COMMON = [

@@ -17,6 +17,9 @@ void free_dataset(tvk_ctx* c) {
   c->pool_hold.reset();   // frees the pool once no context shares it
   c->lin_hold.reset();
   c->cells.reset();
+  c->pst.reset();
+  c->ingest_missing.clear();
+  c->ingest_pending = 0;
   c->d_pool = c->d_lin = nullptr;
   c->pool_bytes = c->lin_bytes = 0;
   c->has_ds = false;
@@ -180,6 +183,7 @@ struct DatasetState {
   void* d_pool;
   std::shared_ptr<void> pool_hold, lin_hold;
   std::shared_ptr<CellGrid> cells;
+  std::shared_ptr<PoolState> pst;
   size_t pool_bytes;
   void* d_lin;
   size_t lin_bytes;
@@ -190,7 +194,7 @@ struct DatasetState {
   std::vector<uint8_t> leap_resident;
 };
 DatasetState save_dataset(const tvk_ctx* c) {
-  return DatasetState{c->L, c->clamp, c->d_pool, c->pool_hold, c->lin_hold, c->cells, c->pool_bytes, c->d_lin,
+  return DatasetState{c->L, c->clamp, c->d_pool, c->pool_hold, c->lin_hold, c->cells, c->pst, c->pool_bytes, c->d_lin,
                       c->lin_bytes, c->minmax, c->range_min, c->range_max, c->build_ms, c->has_ds, c->leap_resident};
 }
 void restore_dataset(tvk_ctx* c, DatasetState&& s) {
@@ -200,6 +204,7 @@ void restore_dataset(tvk_ctx* c, DatasetState&& s) {
   c->pool_hold = std::move(s.pool_hold);
   c->lin_hold = std::move(s.lin_hold);
   c->cells = std::move(s.cells);
+  c->pst = std::move(s.pst);
   c->pool_bytes = s.pool_bytes;
   c->d_lin = s.d_lin;
   c->lin_bytes = s.lin_bytes;
@@ -278,7 +283,7 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_tf1d) (void)hipFree(c->d_tf1d);
   if (c->d_tf2d) (void)hipFree(c->d_tf2d);
   if (c->d_tf2f) (void)hipFree(c->d_tf2f);
-  for (void* q : {(void*)c->d_leap_meta, (void*)c->d_leap_missing, (void*)c->d_leap_bricks, (void*)c->d_leap_resume})
+  for (void* q : {(void*)c->d_leap_meta, (void*)c->d_leap_table, (void*)c->d_leap_bricks, (void*)c->d_leap_resume})
     if (q) (void)hipFree(q);
   if (c->d_tf1f) (void)hipFree(c->d_tf1f);
   if (c->d_tf2cols) (void)hipFree(c->d_tf2cols);
@@ -287,6 +292,7 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->d_counter) (void)hipFree(c->d_counter);
   if (c->d_touched) (void)hipFree(c->d_touched);
+  if (c->d_depth) (void)hipFree(c->d_depth);
   if (c->h_list_pinned) (void)hipHostFree(c->h_list_pinned);
   if (c->h_views_pinned) (void)hipHostFree(c->h_views_pinned);
   for (auto& ev : c->ev) if (ev) (void)hipEventDestroy(ev);
@@ -447,6 +453,9 @@ int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src) {
   dst->d_lin = src->d_lin;
   dst->lin_hold = src->lin_hold;
   dst->cells = src->cells;
+  dst->pst = src->pst;
+  dst->ingest_missing = src->ingest_missing;
+  dst->ingest_pending = src->ingest_pending;
   dst->lin_bytes = src->lin_bytes;
   dst->minmax = src->minmax;
   dst->range_min = src->range_min;
@@ -573,18 +582,65 @@ int tvk_get_brick(tvk_ctx* c, uint64_t i, void* host, size_t bytes) {
   if (!c || !host) return TVK_ERR_ARG;
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
   if (i >= c->L.toc.size()) return fail(c, TVK_ERR_RANGE, "unknown brick");
+  if (budgeted(c)) return pool_download(c, (size_t)i, 1, host, bytes);   // the host store holds every brick
+  if (c->ingest_pending) return fail(c, TVK_ERR_STATE, "not every brick was uploaded");
   return download_ref(c, (size_t)i, 1, host, bytes);
 }
 
 int tvk_get_pool(tvk_ctx* c, void* host, size_t bytes) {
   if (!c || !host) return TVK_ERR_ARG;
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (budgeted(c)) return pool_download(c, 0, c->L.toc.size(), host, bytes);
+  if (c->ingest_pending) return fail(c, TVK_ERR_STATE, "not every brick was uploaded");
   return download_ref(c, 0, c->L.toc.size(), host, bytes);
 }
 
 int tvk_max_gradient(tvk_ctx* c, float* out) {
   if (!c || !out) return TVK_ERR_ARG;
-  if (!c->d_lin) return fail(c, TVK_ERR_STATE, "linear volume was not kept");
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (c->ingest_pending) return fail(c, TVK_ERR_STATE, "not every brick was uploaded");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->L.overlap >= 1) {
+    // Histogram2DDataBlock::ComputeTemplate at level 0 (what RAWConverter builds,
+    // RAWConverter.cpp:655-660): the LOD-0 bricks' core voxels, neighbours from the
+    // bricks' own overlap; a budgeted pool pages the bricks through its slots
+    const LodInfo& l0 = c->L.lods[0];
+    const uint32_t nb = (uint32_t)(l0.bricks[0] * l0.bricks[1] * l0.bricks[2]);
+    const uint32_t gx = 16;
+    const uint32_t group = budgeted(c) ? c->pst->nslots : nb;
+    GatherBrick* d_gb = nullptr;
+    double* d_part = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_gb, (size_t)group * sizeof(GatherBrick)));
+    std::shared_ptr<void> gb_hold(d_gb, [](void* p) { (void)hipFree(p); });
+    HIPCHK(c, hipMalloc((void**)&d_part, (size_t)group * gx * sizeof(double)));
+    std::shared_ptr<void> part_hold(d_part, [](void* p) { (void)hipFree(p); });
+    double m = 0.0;
+    std::vector<double> h((size_t)group * gx);
+    for (uint32_t first = 0; first < nb; first += group) {
+      const uint32_t cnt = std::min(group, nb - first);
+      std::vector<GatherBrick> gb(cnt);
+      std::vector<uint32_t> need(cnt);
+      for (uint32_t k = 0; k < cnt; ++k) need[k] = (uint32_t)(l0.offset + first + k);
+      std::vector<int32_t> slot;
+      if (budgeted(c)) {
+        const int rc = pool_acquire_lru(c, need, slot);
+        if (rc) return rc;
+      }
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const TocBrick& t = c->L.toc[need[k]];
+        for (int a = 0; a < 3; ++a) { gb[k].c[a] = (uint32_t)t.coords[a]; gb[k].n[a] = t.size[a]; }
+        gb[k].offset = budgeted(c) ? (uint64_t)slot[k] * c->pst->slot_elems : t.offset;
+      }
+      HIPCHK(c, hipMemcpyAsync(d_gb, gb.data(), cnt * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, launch_brick_max_gradient(c->d_pool, c->L.bpv, d_gb, cnt, c->L.overlap, gx, d_part, c->stream));
+      HIPCHK(c, hipMemcpyAsync(h.data(), d_part, (size_t)cnt * gx * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (size_t k = 0; k < (size_t)cnt * gx; ++k) m = std::max(m, h[k]);
+    }
+    *out = (float)m;
+    return TVK_OK;
+  }
+  if (!c->d_lin) return fail(c, TVK_ERR_STATE, "without overlap the gradient needs the kept linear volume");
   const uint32_t blocks = 4096;
   double* d = nullptr;
   HIPCHK(c, hipMalloc(&d, blocks * sizeof(double)));
@@ -839,6 +895,7 @@ float range_max(const std::vector<std::vector<float>>& t, int lo, int hi) {   //
 }
 
 int learn_prepare(tvk_ctx* c) {
+  if (budgeted(c)) return fail(c, TVK_ERR_UNSUPPORTED, "the learned step needs every brick resident (pool budget 0)");
   if (c->tfst_dirty) {
     build_tf_stats(c);
     const std::vector<float>& F = c->tfst.flat;
@@ -854,6 +911,8 @@ int learn_prepare(tvk_ctx* c) {
 }
 
 int ensure_cells(tvk_ctx* c) {
+  if (budgeted(c)) return fail(c, TVK_ERR_UNSUPPORTED, "the cell grid needs every brick resident (pool budget 0)");
+  if (c->ingest_pending) return fail(c, TVK_ERR_STATE, "not every brick was uploaded");
   if (!c->cells) return fail(c, TVK_ERR_STATE, "no dataset");
   CellGrid& G = *c->cells;
   if (G.d) return TVK_OK;
@@ -1011,7 +1070,14 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
   if (!c->has_view) return fail(c, TVK_ERR_STATE, "no view");
   if (nviews < 1) return fail(c, TVK_ERR_ARG, "nviews must be >= 1");
+  if (c->ingest_pending)
+    return fail(c, TVK_ERR_STATE, std::to_string(c->ingest_pending) + " bricks were declared but not uploaded");
   HIPCHK(c, hipSetDevice(c->device));
+  if (budgeted(c)) {
+    if (fixed) return fail(c, TVK_ERR_UNSUPPORTED, "the allocation-free render needs every brick resident");
+    if (learn > 0) return fail(c, TVK_ERR_UNSUPPORTED, "the learned step needs every brick resident (pool budget 0)");
+    return render_budget_impl(c, nviews, rots, host_out, st);
+  }
   // the host vectors are the upload source of an allocation-free render: the
   // previous call's copies must have consumed them
   if (fixed) HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -140,6 +140,9 @@ int plan_batch(tvk_batch* b, uint32_t* bricks_total) {
     tvk_ctx* c = r.ctx;
     if (!c->has_ds) return bfail(b, TVK_ERR_STATE, "request " + std::to_string(i) + ": context has no dataset");
     if (!c->has_view) return bfail(b, TVK_ERR_STATE, "request " + std::to_string(i) + ": context has no view");
+    if (budgeted(c) || c->ingest_pending)
+      return bfail(b, TVK_ERR_UNSUPPORTED, "request " + std::to_string(i) +
+                                               ": a batch needs every brick of its datasets resident (pool budget 0)");
     ViewState V = c->V;
     V.rot = r.rot;
     if (r.W != V.W || r.H != V.H) {

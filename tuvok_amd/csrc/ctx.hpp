@@ -42,6 +42,41 @@ hipError_t launch_display(const float* frames, uint32_t W, uint32_t H, uint32_t 
                           const float bottom[3], uint8_t* out, hipStream_t st);
 hipError_t launch_cells(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t* d_base, uint32_t nb,
                         uint32_t max_cells, uint32_t* out, hipStream_t st);
+hipError_t launch_brick_max_gradient(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb,
+                                     uint32_t ov, uint32_t gx, double* out, hipStream_t st);
+hipError_t launch_swizzle(const void* src, int bpv, const GatherBrick* d_bricks, const uint64_t* d_src_off, uint32_t nb,
+                          uint64_t max_elems, void* pool, hipStream_t st);
+
+// A dataset held within a device-memory budget (GPUMemMan's budget and
+// texture reuse, GLVolumePool's slots): every brick lives in pinned host
+// memory in the reference layout, and the HBM pool holds nslots slots of
+// slot_elems elements (the largest brick's blocked layout), paged on demand.
+// Shared by the contexts that share the dataset; all calls that page must
+// come from one thread.
+struct PoolState {
+  int bpv = 1;
+  uint64_t slot_elems = 0;
+  uint32_t nslots = 0;
+  std::vector<int64_t> slot_brick;     // per slot: TOC brick or -1
+  std::vector<uint64_t> slot_time;     // per slot: last use (raycaster) / creation time (traversal)
+  std::vector<int32_t> brick_slot;     // per TOC brick: slot or -1
+  uint64_t clock = 2;                  // 0: never used, 1: flagged empty (GLVolumePool)
+  // host store (pinned): bricks in TOC order, reference layout
+  std::vector<void*> chunks;
+  std::vector<uint8_t*> brick_host;    // per TOC brick
+  std::vector<uint8_t> present;        // per TOC brick: payload handed over
+  // device staging for the uploads
+  uint8_t* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  GatherBrick* d_desc = nullptr;
+  uint64_t* d_srcoff = nullptr;
+  size_t desc_cap = 0;
+  // statistics
+  uint64_t uploads = 0, evictions = 0, upload_bytes = 0;
+  double upload_ms = 0.0;
+  ~PoolState();
+};
+
 // Cell grid of a dataset (learned mode): per brick, the (min, max) of its cells
 // (tvk_types.hpp), built on first use and shared by the contexts that share
 // the dataset.
@@ -71,6 +106,11 @@ struct tvk_ctx {
   std::shared_ptr<void> pool_hold;   // owns d_pool; shared by contexts that share the dataset
   std::shared_ptr<void> lin_hold;    // owns d_lin
   std::shared_ptr<tvk::CellGrid> cells;   // created with the pool, filled on first learned use
+  std::shared_ptr<tvk::PoolState> pst;    // null: every brick resident at its TOC offset
+  // bricks declared by tvk_set_bricks (resident pool) whose payload has not
+  // arrived yet: rendering waits for all of them
+  std::vector<uint8_t> ingest_missing;
+  uint64_t ingest_pending = 0;
   size_t pool_bytes = 0;
   void* d_lin = nullptr;
   size_t lin_bytes = 0;
@@ -119,8 +159,10 @@ struct tvk_ctx {
   std::vector<uint8_t> leap_resident;   // per TOC brick (empty: all resident)
   uint32_t* d_leap_meta = nullptr;
   size_t leap_meta_cap = 0;
-  uint32_t* d_leap_missing = nullptr;
-  size_t leap_missing_cap = 0;
+  uint32_t* d_leap_table = nullptr;    // GLHashTable of requested bricks
+  size_t leap_table_cap = 0;
+  uint32_t leap_table_size = 509, leap_rehash = 10;   // RendererState defaults (MasterController.h:70-74)
+  uint32_t leap_table_used = 0;       // entries of the last pass's table
   tvk::GatherBrick* d_leap_bricks = nullptr;
   size_t leap_bricks_cap = 0;
   float4* d_leap_resume = nullptr;
@@ -129,6 +171,9 @@ struct tvk_ctx {
   tvk::F3 leap_eye = tvk::F3(0.0f, 0.0f, 1.6f);   // m_vEye
   uint32_t* d_touched = nullptr;
   size_t touched_cap = 0, touched_n = 0;
+  float* d_depth = nullptr;          // isosurface depth of a frame rendered in parts
+  size_t depth_cap = 0;
+  uint32_t last_parts = 0;           // launches of the last budgeted frame
   hipEvent_t ev[8] = {};
   std::vector<FrameBrick> h_list;
   std::vector<ViewConst> h_views;
@@ -200,6 +245,22 @@ int ensure_cells(tvk_ctx* c);
 bool stepnet_device(const struct tvk_stepnet* s, StepNetDev* out);
 // shading constants of the context's dataset, TF and parameters
 void fill_shade(const tvk_ctx* c, ShadeConst& S);
+
+// Budgeted pools (pool.cpp): slots for the bricks of `need` (TOC indices, at most
+// nslots distinct) by least recent use, paging in the ones not resident;
+// slot_of[i] receives the slot of need[i].  Stream-ordered: the uploads run on
+// the context stream after the work already queued there.
+int pool_acquire_lru(tvk_ctx* c, const std::vector<uint32_t>& need, std::vector<int32_t>& slot_of);
+// Pages the given TOC bricks into the given slots (no policy).
+int pool_upload(tvk_ctx* c, const std::vector<uint32_t>& toc, const std::vector<int32_t>& slots);
+// Converts the current dataset to a budgeted one (budget 0: back to all resident
+// needs the payloads: TVK_ERR_UNSUPPORTED unless it is already resident).
+int pool_set_budget(tvk_ctx* c, uint64_t budget_bytes);
+// Copies bricks [first, first+n) in the reference layout to host memory.
+int pool_download(tvk_ctx* c, size_t first, size_t n, void* host, size_t bytes);
+inline bool budgeted(const tvk_ctx* c) { return c->pst != nullptr; }
+// a frame of a budgeted dataset (fixed step, any mode)
+int render_budget_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st);
 
 // Device buffers of an allocation-free render (capacities in entries).
 struct FrameBuffers {

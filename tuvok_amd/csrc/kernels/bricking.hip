@@ -174,6 +174,25 @@ __global__ __launch_bounds__(256) void unswizzle_bricks(const T* __restrict__ po
   }
 }
 
+// reference layout (x fastest, unpadded, at src_off[brick]) -> blocked pool at
+// bricks[].offset: every element of the padded layout, the repeated half-rows
+// included, is the brick voxel its index decodes to (clamped to the brick)
+template <typename T>
+__global__ __launch_bounds__(256) void swizzle_bricks(const T* __restrict__ src, const GatherBrick* __restrict__ bricks,
+                                                      const uint64_t* __restrict__ src_off, T* __restrict__ pool) {
+  const GatherBrick b = bricks[blockIdx.y];
+  const int lx = lx_of<T>();
+  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2);
+  const uint64_t n = padded_count(b.n, lx);
+  const uint64_t so = src_off[blockIdx.y];
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    uint32_t x, y, z;
+    unswz(i, nbx, nby, lx, x, y, z);
+    x = min(x, b.n[0] - 1); y = min(y, b.n[1] - 1); z = min(z, b.n[2] - 1);
+    pool[b.offset + i] = src[so + x + (uint64_t)b.n[0] * (y + (uint64_t)b.n[1] * z)];
+  }
+}
+
 // 2x box filter with the reference's mean (double accumulation, truncation);
 // an axis of size 1 is not halved (ExtendedOctree.cpp:201-213).
 template <typename T>
@@ -302,6 +321,47 @@ __global__ __launch_bounds__(256) void max_gradient(const T* __restrict__ lin, u
   if (threadIdx.x == 0) {
     for (int k = 1; k < 4; ++k) best = sb[k] > best ? sb[k] : best;
     out[blockIdx.x] = best;
+  }
+}
+
+// Histogram2DDataBlock::ComputeTemplate's first pass (Histogram2DDataBlock.cpp:349-395,
+// ComputeGradient :326-346): over the core voxels of each brick (overlap ov
+// <= x < size - ov), the gradient (f(x-1)-f(x+1))/(2 norm) per axis from the
+// brick's own stored neighbours, and the largest magnitude.  grid: (gx, bricks);
+// out[brick * gx + block] receives the block's maximum.
+template <typename T>
+__global__ __launch_bounds__(256) void brick_max_gradient(const T* __restrict__ pool,
+                                                          const GatherBrick* __restrict__ bricks, uint32_t ov,
+                                                          double norm, double* __restrict__ out) {
+  const GatherBrick b = bricks[blockIdx.y];
+  const int lx = lx_of<T>();
+  const uint32_t nbx = nblocks(b.n[0], lx), nby = nblocks(b.n[1], 2);
+  const uint32_t cx = b.n[0] - 2 * ov, cy = b.n[1] - 2 * ov, cz = b.n[2] - 2 * ov;
+  const uint64_t n = (uint64_t)cx * cy * cz;
+  const T* __restrict__ p = pool + b.offset;
+  auto at = [&](uint32_t x, uint32_t y, uint32_t z) -> double { return (double)p[swz(x, y, z, nbx, nby, lx)]; };
+  double best = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint32_t x = ov + (uint32_t)(i % cx);
+    const uint64_t r = i / cx;
+    const uint32_t y = ov + (uint32_t)(r % cy), z = ov + (uint32_t)(r / cy);
+    const double gx = (at(x - 1, y, z) - at(x + 1, y, z)) / (norm * 2);
+    const double gy = (at(x, y - 1, z) - at(x, y + 1, z)) / (norm * 2);
+    const double gz = (at(x, y, z - 1) - at(x, y, z + 1)) / (norm * 2);
+    const double l = sqrt(gx * gx + gy * gy + gz * gz);
+    best = l > best ? l : best;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(best, o, 64);
+    best = t > best ? t : best;
+  }
+  __shared__ double sb[4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sb[w] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) best = sb[k] > best ? sb[k] : best;
+    out[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = best;
   }
 }
 
@@ -469,6 +529,23 @@ hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bric
   return hipGetLastError();
 }
 
+hipError_t launch_swizzle(const void* src, int bpv, const GatherBrick* d_bricks, const uint64_t* d_src_off, uint32_t nb,
+                          uint64_t max_elems, void* pool, hipStream_t st) {
+  uint64_t gx = (max_elems + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  if (gx == 0) gx = 1;
+  for (uint32_t first = 0; first < nb; first += 65535) {
+    const uint32_t cnt = nb - first < 65535 ? nb - first : 65535;
+    if (bpv == 1)
+      hipLaunchKernelGGL(swizzle_bricks<uint8_t>, dim3((uint32_t)gx, cnt), dim3(256), 0, st, (const uint8_t*)src,
+                         d_bricks + first, d_src_off + first, (uint8_t*)pool);
+    else
+      hipLaunchKernelGGL(swizzle_bricks<uint16_t>, dim3((uint32_t)gx, cnt), dim3(256), 0, st, (const uint16_t*)src,
+                         d_bricks + first, d_src_off + first, (uint16_t*)pool);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,
                          uint32_t* out, hipStream_t st) {
   uint32_t gx = (max_vox / (bpv == 1 ? 16 : 8) + 255) / 256;
@@ -513,6 +590,20 @@ hipError_t launch_cells(const void* pool, int bpv, const GatherBrick* d_bricks, 
     else
       hipLaunchKernelGGL(cell_minmax<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint16_t*)pool,
                          d_bricks + first, d_base + first, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_brick_max_gradient(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb,
+                                     uint32_t ov, uint32_t gx, double* out, hipStream_t st) {
+  for (uint32_t first = 0; first < nb; first += 65535u) {
+    const uint32_t cnt = std::min(nb - first, 65535u);
+    if (bpv == 1)
+      hipLaunchKernelGGL(brick_max_gradient<uint8_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint8_t*)pool,
+                         d_bricks + first, ov, 255.0, out + (size_t)first * gx);
+    else
+      hipLaunchKernelGGL(brick_max_gradient<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (const uint16_t*)pool,
+                         d_bricks + first, ov, 65535.0, out + (size_t)first * gx);
   }
   return hipGetLastError();
 }

@@ -232,6 +232,23 @@ __device__ __forceinline__ uint32_t brick_id(const LeapArgs& A, uint32_t x, uint
   return A.lodoff[l] + x + y * A.ilayout[l][0] + z * A.ilayout[l][0] * A.ilayout[l][1];
 }
 
+// ReportMissingBrick: GLHashTable's insertion (GLHashTable.cpp:140-181).  The
+// brick's 4D index over the finest level's brick layout, plus one, is stored at
+// its value modulo the table size, or, when another brick holds that entry, at
+// the next of `rehash` consecutive positions; a request that finds no free
+// entry is dropped (the host pages what the table holds, and rays needing the
+// dropped brick report it again in the next pass).
+__device__ __forceinline__ void report_missing(const LeapArgs& A, const uint32_t bc[4]) {
+  const uint32_t lx = A.ilayout[0][0], ly = A.ilayout[0][1], lz = A.ilayout[0][2];
+  const uint32_t v = 1u + bc[0] + bc[1] * lx + bc[2] * lx * ly + bc[3] * lx * ly * lz;
+  uint32_t r = 0;
+  do {   // at least one probe, as the reference's do-while
+    const uint32_t h = (v + r) % A.table_size;
+    const uint32_t old = atomicCAS(&A.table[h], 0u, v);
+    if (old == 0u || old == v) return;
+  } while (++r < A.rehash);
+}
+
 // GLVolumePool.cpp:653-656, the floor of the logarithm read from the exponent
 __device__ __forceinline__ uint32_t compute_lod(const LeapArgs& A, float depth) {
   const float x = A.lod_factor * (-depth) / A.l0err;
@@ -343,12 +360,14 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
         coords(lod);
         uint32_t info = A.meta[brick_id(A, bc[0], bc[1], bc[2], lod)];
         if (info == BI_MISSING) {
-          A.missing[brick_id(A, bc[0], bc[1], bc[2], lod)] = 1u;   // ReportMissingBrick
+          report_missing(A, bc);
           found_req = false;
           do {
             ++lod;
             coords(lod);
             info = A.meta[brick_id(A, bc[0], bc[1], bc[2], lod)];
+            // BS_RequestAll, the reference's default strategy: coarser missing bricks too
+            if (info == BI_MISSING) report_missing(A, bc);
           } while (info == BI_MISSING && lod + 1 < (uint32_t)A.nlod);
         }
         const bool bEmpty = info <= BI_EMPTY;

@@ -442,7 +442,8 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
                                            const FrameBrick* __restrict__ list, float* __restrict__ out, int tx0,
                                            int ty0, int tw, int th, unsigned long long* counter, uint32_t* touched,
                                            const LearnConst* LC, const SegOut* SO, uint16_t* sn_lds,
-                                           uint32_t view_id, uint64_t* cols) {
+                                           uint32_t view_id, uint64_t* cols, bool acc = false,
+                                           float* __restrict__ depth_buf = nullptr) {
   const int lane = threadIdx.x & 63;
   const int px = tx0 + lane % tw, py = ty0 + lane / tw;
   const int W = (int)VC->W, H = (int)VC->H;
@@ -475,11 +476,20 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
   const float eps = S.ert_eps;
   const bool edge = S.clamp_edge != 0;
 
+  // acc: this launch continues the frame of an earlier launch over the bricks
+  // before these in compositing order (a budgeted pool renders a frame in
+  // parts): the ray state is the pixel's value so far (and, for the
+  // isosurface, its depth), so the parts compose to the one-launch frame
+  const size_t pix = (size_t)VC->out_offset + (size_t)py * (size_t)W + (size_t)px;
   float dst0 = 0.f, dst1 = 0.f, dst2 = 0.f, dst3 = 0.f;
+  if (MODE != 2 && acc && inside) {
+    const float4 p0 = reinterpret_cast<const float4*>(out)[pix];
+    dst0 = p0.x; dst1 = p0.y; dst2 = p0.z; dst3 = p0.w;
+  }
   bool done = !inside;
   uint32_t ns = 0;
   // isosurface state
-  float best_depth = 1.0f, hit_param = 0.0f;
+  float best_depth = (MODE == 2 && acc && inside) ? depth_buf[pix] : 1.0f, hit_param = 0.0f;
   bool have_hit = false;
   V3 hit_pos = mk(0, 0, 0), hit_tex = mk(0, 0, 0);
   int hit_brick = 0;
@@ -865,6 +875,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
   }
   if (!inside) return;
   float4 res;
+  if (MODE == 2 && acc && !have_hit) return;   // an earlier part's hit (or none) stands
   if (MODE == 2) {
     // Normal of the winning hit: per-lane brick, computed once all bricks are
     // done (inside the brick loop the per-brick values of lanes that take
@@ -887,7 +898,8 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
   } else {
     res = make_float4(dst0, dst1, dst2, dst3);
   }
-  reinterpret_cast<float4*>(out)[(size_t)VC->out_offset + (size_t)py * (size_t)W + (size_t)px] = res;
+  reinterpret_cast<float4*>(out)[pix] = res;
+  if (MODE == 2 && depth_buf) depth_buf[pix] = best_depth;
 }
 
 // Multi-view launch: one 256-thread workgroup = 16x16 pixels = 4 waves of 8x8.
@@ -919,7 +931,7 @@ __device__ __forceinline__ void march_group(const MarchArgs& A) {
   march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
                                          A.counter, (COUNT && A.touched) ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
                                          &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v,
-                                         s_cols + (MODE == 1 ? wave * 64 : 0));
+                                         s_cols + (MODE == 1 ? wave * 64 : 0), A.accumulate != 0, A.depth);
 }
 
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>

@@ -2,10 +2,12 @@
 //
 // Host half of GLGridLeaper + GLVolumePool: brick IDs and layouts
 // (GLVolumePool.cpp:89-117, :205-223, :288-291), the brick metadata with the
-// visibility of the current TF (:961-1100), residency and paging of requested
-// bricks (:673-717, :1720-1789: here the bricks already sit in the HBM pool and
-// paging marks them resident), and the shader uniforms (GLGridLeaper.cpp:560-574,
-// :700-745; GLVolumePool.cpp:790-803).
+// visibility of the current TF (:961-1100), the table of requested bricks
+// (GLHashTable.cpp:90-170), residency and paging of requested bricks (:673-717,
+// :955-959, :1720-1789), and the shader uniforms (GLGridLeaper.cpp:560-574,
+// :700-745; GLVolumePool.cpp:790-803).  With every brick in HBM paging marks
+// bricks resident; with a budgeted pool it copies them into slots, the oldest
+// upload replaced first, the coarsest brick never.
 #include "ctx.hpp"
 
 using namespace tvk;
@@ -89,7 +91,8 @@ int make_meta(tvk_ctx* c, const Grid& G, std::vector<uint32_t>& meta) {
           const uint64_t i = G.id(x, y, z, l);
           const bool empty = !(fmax >= c->minmax[2 * i] && fmin <= c->minmax[2 * i + 1]);
           if (!empty) {
-            const bool res = c->leap_resident.empty() || c->leap_resident[i] || l == G.nlod - 1;
+            const bool res = budgeted(c) ? c->pst->brick_slot[i] >= 0
+                                         : (c->leap_resident.empty() || c->leap_resident[i] || l == G.nlod - 1);
             meta[i] = res ? (uint32_t)(BI_FLAG_COUNT + i) : (uint32_t)BI_MISSING;
             continue;
           }
@@ -105,15 +108,147 @@ int make_meta(tvk_ctx* c, const Grid& G, std::vector<uint32_t>& meta) {
   return TVK_OK;
 }
 
+// TOC bricks of the request table's entries, in table order (GLHashTable::GetData)
+std::vector<uint32_t> decode_requests(const tvk_ctx* c, const Grid& G, const std::vector<uint32_t>& table) {
+  std::vector<uint32_t> out;
+  const uint64_t lx = G.ilayout[0][0], ly = G.ilayout[0][1], lz = G.ilayout[0][2];
+  const uint64_t vol = lx * ly * lz;
+  for (uint32_t e : table) {
+    if (!e) continue;
+    uint64_t i = (uint64_t)e - 1;   // GLHashTable::Int2Vector
+    const uint64_t w = i / vol;
+    i -= w * vol;
+    const uint64_t z = i / (lx * ly);
+    i -= z * lx * ly;
+    const uint64_t y = i / lx, x = i - y * lx;
+    if ((int)w < G.nlod && x < G.ilayout[w][0] && y < G.ilayout[w][1] && z < G.ilayout[w][2])
+      out.push_back((uint32_t)G.id(x, y, z, (int)w));
+  }
+  (void)c;
+  return out;
+}
+
+int fetch_table(tvk_ctx* c, std::vector<uint32_t>& table) {
+  if (!c->d_leap_table || !c->leap_table_used) return fail(c, TVK_ERR_STATE, "no pass yet");
+  table.assign(c->leap_table_used, 0u);
+  HIPCHK(c, hipMemcpyAsync(table.data(), c->d_leap_table, table.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TVK_OK;
+}
+
+// The coarsest brick of a budgeted pool sits in a slot that is never replaced
+// (GLVolumePool::UploadFirstBrick, GLVolumePool.cpp:763-766).
+int leap_pin_coarsest(tvk_ctx* c) {
+  PoolState& P = *c->pst;
+  const uint32_t last = (uint32_t)c->L.toc.size() - 1;
+  if (P.brick_slot[last] >= 0) {
+    P.slot_time[(size_t)P.brick_slot[last]] = std::numeric_limits<uint64_t>::max();
+    return TVK_OK;
+  }
+  // the oldest slot (or a free one)
+  int32_t best = 0;
+  for (uint32_t s = 1; s < P.nslots; ++s)
+    if (P.slot_time[s] < P.slot_time[(size_t)best]) best = (int32_t)s;
+  const int rc = pool_upload(c, std::vector<uint32_t>{last}, std::vector<int32_t>{best});
+  if (rc) return rc;
+  P.slot_time[(size_t)best] = std::numeric_limits<uint64_t>::max();
+  return TVK_OK;
+}
+
+// GLVolumePool::UploadBricks with a budgeted pool: slots sorted by upload time
+// (PrepareForPaging), requested bricks written to the oldest slots in request
+// order, at most every slot but the coarsest brick's, each stamped with the
+// next upload time; a replaced brick becomes missing.
+int leap_page_budgeted(tvk_ctx* c, const std::vector<uint32_t>& req, uint64_t max_bricks, uint64_t* paged) {
+  PoolState& P = *c->pst;
+  int rc = leap_pin_coarsest(c);
+  if (rc) return rc;
+  std::vector<uint32_t> want;
+  for (uint32_t t : req)
+    if (P.brick_slot[t] < 0 && std::find(want.begin(), want.end(), t) == want.end()) want.push_back(t);
+  std::vector<int32_t> order(P.nslots);
+  for (uint32_t s = 0; s < P.nslots; ++s) order[s] = (int32_t)s;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return P.slot_time[(size_t)a] < P.slot_time[(size_t)b]; });
+  const size_t room = P.nslots - 1;
+  const size_t n = (size_t)std::min<uint64_t>(std::min<uint64_t>(want.size(), room), max_bricks);
+  want.resize(n);
+  order.resize(n);
+  if (n) {
+    rc = pool_upload(c, want, order);
+    if (rc) return rc;
+    for (int32_t s : order) P.slot_time[(size_t)s] = P.clock++;
+  }
+  if (paged) *paged = n;
+  return TVK_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
+int tvk_leap_set_request_table(tvk_ctx* c, uint32_t size, uint32_t rehash) {
+  if (!c) return TVK_ERR_ARG;
+  if (size > (1u << 26)) return fail(c, TVK_ERR_ARG, "request table too large");
+  c->leap_table_size = size;
+  c->leap_rehash = rehash;
+  return TVK_OK;
+}
+
+int tvk_leap_requests(tvk_ctx* c, uint32_t* entries, size_t n, size_t* size) {
+  if (!c) return TVK_ERR_ARG;
+  std::vector<uint32_t> t;
+  const int rc = fetch_table(c, t);
+  if (rc) return rc;
+  if (size) *size = t.size();
+  if (entries) {
+    if (n < t.size()) return fail(c, TVK_ERR_ARG, "output too small");
+    std::copy(t.begin(), t.end(), entries);
+  }
+  return TVK_OK;
+}
+
+int tvk_pool_resident(tvk_ctx* c, uint8_t* flags, size_t n) {
+  if (!c || !flags) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  const size_t nb = c->L.toc.size();
+  if (n != nb) return fail(c, TVK_ERR_ARG, "one flag per brick expected");
+  for (size_t i = 0; i < nb; ++i) {
+    if (budgeted(c)) flags[i] = c->pst->brick_slot[i] >= 0 ? 1 : 0;
+    else if (!c->ingest_missing.empty() && c->ingest_missing[i]) flags[i] = 0;
+    else flags[i] = (c->leap_resident.empty() || c->leap_resident[i] || i + 1 == nb) ? 1 : 0;
+  }
+  return TVK_OK;
+}
+
 int tvk_leap_set_resident(tvk_ctx* c, const uint8_t* flags, size_t n) {
   if (!c) return TVK_ERR_ARG;
   if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (flags && n != c->L.toc.size()) return fail(c, TVK_ERR_ARG, "one flag per brick expected");
+  if (budgeted(c)) {
+    // evict everything but the coarsest brick, then page the flagged bricks in
+    PoolState& P = *c->pst;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint32_t s = 0; s < P.nslots; ++s) {
+      if (P.slot_brick[s] >= 0 && (size_t)P.slot_brick[s] + 1 != c->L.toc.size()) {
+        P.brick_slot[(size_t)P.slot_brick[s]] = -1;
+        P.slot_brick[s] = -1;
+        P.slot_time[s] = 0;
+      }
+    }
+    // NULL: only the coarsest brick stays (a budgeted pool cannot hold every brick)
+    std::vector<uint32_t> req;
+    if (flags) {
+      for (size_t i = 0; i + 1 < n; ++i)
+        if (flags[i]) req.push_back((uint32_t)i);
+    }
+    if (req.size() + 1 > P.nslots) return fail(c, TVK_ERR_NOMEM, "more bricks flagged than the pool's slots hold");
+    uint64_t paged = 0;
+    return leap_page_budgeted(c, req, req.size(), &paged);
+  }
   if (!flags) { c->leap_resident.clear(); return TVK_OK; }
-  if (n != c->L.toc.size()) return fail(c, TVK_ERR_ARG, "one flag per brick expected");
   c->leap_resident.assign(flags, flags + n);
   return TVK_OK;
 }
@@ -125,14 +260,31 @@ int tvk_leap_render(tvk_ctx* c, int first, float* host_out, tvk_leap_stats* st) 
   Grid G;
   int rc = make_grid(c, G);
   if (rc) return rc;
+  if (c->ingest_pending) return fail(c, TVK_ERR_STATE, "not every brick was uploaded");
+  if (budgeted(c)) {
+    rc = leap_pin_coarsest(c);   // before the metadata: the coarsest level is always resident
+    if (rc) return rc;
+  }
   std::vector<uint32_t> meta;
   rc = make_meta(c, G, meta);
   if (rc) return rc;
   const size_t nb = c->L.toc.size();
   const size_t pixels = (size_t)c->V.W * c->V.H;
   if (!first && c->leap_pixels != pixels) return fail(c, TVK_ERR_STATE, "no earlier pass at this image size");
+  // the request table: HashTableSize, or 0 for the size from the largest brick
+  // (GLGridLeaper::InitHashTable, GLGridLeaper.cpp:266-279)
+  uint32_t tsize = c->leap_table_size;
+  if (tsize == 0) {
+    const double rmax = (double)c->L.brick[0] * c->L.brick[1] * c->L.brick[2] / 32768.0;
+    tsize = std::max<uint32_t>(15u, (uint32_t)(509.0 / rmax));
+  }
+  {
+    const uint64_t l0 = (uint64_t)G.ilayout[0][0] * G.ilayout[0][1] * G.ilayout[0][2];
+    if (l0 * (uint64_t)G.nlod + 1 >= (1ull << 32))
+      return fail(c, TVK_ERR_UNSUPPORTED, "brick indices exceed the request table's 32-bit entries");
+  }
   rc = ensure(c, &c->d_leap_meta, &c->leap_meta_cap, nb);
-  if (!rc) rc = ensure(c, &c->d_leap_missing, &c->leap_missing_cap, nb);
+  if (!rc) rc = ensure(c, &c->d_leap_table, &c->leap_table_cap, tsize);
   if (!rc) rc = ensure(c, &c->d_leap_bricks, &c->leap_bricks_cap, nb);
   if (!rc) rc = ensure(c, &c->d_leap_resume, &c->leap_resume_cap, 2 * pixels);
   if (!rc) rc = ensure(c, &c->d_out, &c->out_cap, 4 * pixels);
@@ -140,16 +292,21 @@ int tvk_leap_render(tvk_ctx* c, int first, float* host_out, tvk_leap_stats* st) 
   std::vector<GatherBrick> gb(nb);
   for (size_t i = 0; i < nb; ++i) {
     for (int k = 0; k < 3; ++k) { gb[i].c[k] = (uint32_t)c->L.toc[i].coords[k]; gb[i].n[k] = c->L.toc[i].size[k]; }
-    gb[i].offset = c->L.toc[i].offset;
+    // a budgeted pool: the brick's slot (bricks without one are missing in the metadata)
+    gb[i].offset = budgeted(c) ? (uint64_t)std::max(c->pst->brick_slot[i], 0) * c->pst->slot_elems : c->L.toc[i].offset;
   }
+  // host sources of these copies live until the synchronisation below
   HIPCHK(c, hipMemcpyAsync(c->d_leap_meta, meta.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_leap_bricks, gb.data(), nb * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->d_leap_missing, 0, nb * sizeof(uint32_t), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_leap_table, 0, tsize * sizeof(uint32_t), c->stream));
+  c->leap_table_used = tsize;
   LeapArgs a{};
   fill_shade(c, a.S);
   a.meta = c->d_leap_meta;
   a.bricks = c->d_leap_bricks;
-  a.missing = c->d_leap_missing;
+  a.table = c->d_leap_table;
+  a.table_size = tsize;
+  a.rehash = c->leap_rehash;
   a.resume = c->d_leap_resume;
   a.out = c->d_out;
   a.counter = c->count ? c->d_counter : nullptr;
@@ -205,10 +362,10 @@ int tvk_leap_render(tvk_ctx* c, int first, float* host_out, tvk_leap_stats* st) 
     HIPCHK(c, hipMemcpyAsync(host_out, c->d_out, pixels * 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   if (st) {
     std::memset(st, 0, sizeof(*st));
-    std::vector<uint32_t> miss(nb);
+    std::vector<uint32_t> miss(tsize);
     std::vector<float4> res(2 * pixels);
     unsigned long long ns = 0;
-    HIPCHK(c, hipMemcpyAsync(miss.data(), c->d_leap_missing, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(miss.data(), c->d_leap_table, tsize * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(res.data(), c->d_leap_resume, res.size() * sizeof(float4), hipMemcpyDeviceToHost,
                              c->stream));
     if (c->count) HIPCHK(c, hipMemcpyAsync(&ns, c->d_counter, sizeof(ns), hipMemcpyDeviceToHost, c->stream));
@@ -227,31 +384,39 @@ int tvk_leap_render(tvk_ctx* c, int first, float* host_out, tvk_leap_stats* st) 
 
 int tvk_leap_missing(tvk_ctx* c, uint8_t* flags, size_t n, uint64_t* count) {
   if (!c) return TVK_ERR_ARG;
-  if (!c->d_leap_missing || c->leap_missing_cap < c->L.toc.size()) return fail(c, TVK_ERR_STATE, "no pass yet");
   const size_t nb = c->L.toc.size();
   if (flags && n != nb) return fail(c, TVK_ERR_ARG, "one flag per brick expected");
-  std::vector<uint32_t> miss(nb);
-  HIPCHK(c, hipMemcpyAsync(miss.data(), c->d_leap_missing, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  uint64_t k = 0;
-  for (size_t i = 0; i < nb; ++i) {
-    if (flags) flags[i] = miss[i] ? 1 : 0;
-    k += miss[i] ? 1 : 0;
+  std::vector<uint32_t> table;
+  int rc = fetch_table(c, table);
+  if (rc) return rc;
+  Grid G;
+  rc = make_grid(c, G);
+  if (rc) return rc;
+  const std::vector<uint32_t> req = decode_requests(c, G, table);
+  if (flags) {
+    std::fill(flags, flags + nb, (uint8_t)0);
+    for (uint32_t t : req) flags[t] = 1;
   }
-  if (count) *count = k;
+  if (count) *count = req.size();
   return TVK_OK;
 }
 
 int tvk_leap_page_in(tvk_ctx* c, uint64_t max_bricks, uint64_t* paged) {
   if (!c) return TVK_ERR_ARG;
   const size_t nb = c->L.toc.size();
-  std::vector<uint8_t> miss(nb);
-  int rc = tvk_leap_missing(c, miss.data(), nb, nullptr);
+  std::vector<uint32_t> table;
+  int rc = fetch_table(c, table);
   if (rc) return rc;
+  Grid G;
+  rc = make_grid(c, G);
+  if (rc) return rc;
+  const std::vector<uint32_t> req = decode_requests(c, G, table);   // table order
+  HIPCHK(c, hipSetDevice(c->device));
+  if (budgeted(c)) return leap_page_budgeted(c, req, max_bricks, paged);
   if (c->leap_resident.empty()) c->leap_resident.assign(nb, 1);   // everything is resident already
   uint64_t k = 0;
-  for (size_t i = 0; i < nb && k < max_bricks; ++i)
-    if (miss[i] && !c->leap_resident[i]) { c->leap_resident[i] = 1; ++k; }
+  for (size_t j = 0; j < req.size() && k < max_bricks; ++j)
+    if (!c->leap_resident[req[j]]) { c->leap_resident[req[j]] = 1; ++k; }
   if (paged) *paged = k;
   return TVK_OK;
 }

@@ -163,6 +163,8 @@ struct MarchArgs {
   int lighting, mode;
   uint32_t groups_total;        // sum of groups over the views
   uint32_t groups_padded;       // rounded up to a multiple of 8 (XCD count)
+  int accumulate;               // continue the pixels' values (and depths) of an earlier launch
+  float* depth;                 // isosurface: per-pixel depth of the best hit so far (or nullptr)
 };
 
 // One request of a batch: its shading constants, view, and place in the batch.
@@ -197,7 +199,8 @@ struct LeapArgs {
   ShadeConst S;                // pool, TF tables, scales, lights
   const uint32_t* meta;        // per brick ID: 0 missing, 1 child empty, 2 empty, 3 + TOC index resident
   const GatherBrick* bricks;   // per TOC brick: stored size and pool offset
-  uint32_t* missing;           // per brick ID: set to 1 when reported missing
+  uint32_t* table;             // GLHashTable of the requested bricks: serialised 4D brick index + 1, 0 = free
+  uint32_t table_size, rehash; // entries; probes per request (GLHashTable.cpp:140-181)
   float4* resume;              // 2 per pixel: (normalised position, eye z; w = 1000 finished), colour
   float* out;                  // RGBA32F accumulated colour
   unsigned long long* counter; // samples (or nullptr)

@@ -94,6 +94,56 @@ class HIPRaycaster:
                                               b.ctypes.data_as(C.POINTER(C.c_uint32)), overlap, int(clamp_to_edge),
                                               max_lods, int(keep_linear)))
 
+    def SetBricks(self, dtype, dims, brick, overlap, keys, lod_count=0, budget_bytes=0):
+        """A dataset handed over brick by brick (BrickedDataset): keys is a list of
+        (lod, index, (min, max)) covering the ExtendedOctree layout of dims (x, y, z),
+        brick (stored size) and overlap; payloads follow with UploadBrick.
+        budget_bytes > 0 keeps the bricks in host memory and pages them into
+        that much HBM."""
+        dt = {np.dtype(np.uint8): _abi.TVK_U8, np.dtype(np.uint16): _abi.TVK_U16}[np.dtype(dtype)]
+        arr = (_abi.BrickKey * max(len(keys), 1))()
+        for i, (lod, index, mm) in enumerate(keys):
+            arr[i].lod, arr[i].index = int(lod), int(index)
+            arr[i].minmax[0], arr[i].minmax[1] = float(mm[0]), float(mm[1])
+        d = np.array(dims, np.uint64)
+        b = np.array(brick, np.uint32)
+        self._chk(self._L.tvk_set_bricks(self._h, dt, d.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         b.ctypes.data_as(C.POINTER(C.c_uint32)), int(overlap), int(lod_count), arr,
+                                         len(keys), int(budget_bytes)))
+
+    def UploadBrick(self, lod, index, data):
+        """Dataset::GetBrick payload: (z, y, x) array of the stored brick."""
+        a = np.ascontiguousarray(data)
+        self._chk(self._L.tvk_upload_brick(self._h, int(lod), int(index), a.ctypes.data, a.nbytes))
+
+    def SetPoolBudget(self, budget_bytes):
+        """Hold the dataset within budget_bytes of HBM (0: every brick resident)."""
+        self._chk(self._L.tvk_set_pool_budget(self._h, int(budget_bytes)))
+
+    def PoolResident(self):
+        """Per TOC brick: 1 where the brick is resident (in a slot of a budgeted pool)."""
+        n = self.GetDatasetDesc().brick_count
+        f = np.zeros(n, np.uint8)
+        self._chk(self._L.tvk_pool_resident(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8)), n))
+        return f
+
+    def LeapSetRequestTable(self, size=509, rehash=10):
+        """GLHashTable size and rehash count of the traversal's brick requests."""
+        self._chk(self._L.tvk_leap_set_request_table(self._h, int(size), int(rehash)))
+
+    def LeapRequests(self):
+        """The last pass's request table (entry = 4D brick index + 1, 0 = free)."""
+        n = C.c_size_t()
+        self._chk(self._L.tvk_leap_requests(self._h, None, 0, C.byref(n)))
+        t = np.zeros(n.value, np.uint32)
+        self._chk(self._L.tvk_leap_requests(self._h, t.ctypes.data_as(C.POINTER(C.c_uint32)), n.value, C.byref(n)))
+        return t
+
+    def GetPoolStats(self):
+        s = _abi.PoolStats()
+        self._chk(self._L.tvk_get_pool_stats(self._h, C.byref(s)))
+        return s
+
     def Rebrick(self, brick, overlap=1, max_lods=0):
         """Re-brick the dataset from its kept linear volume (keep_linear=True)."""
         b = np.array(brick, np.uint32)
