@@ -209,6 +209,34 @@ int tvk_max_gradient(tvk_ctx* ctx, float* out);
 int tvk_set_tf1d(tvk_ctx* ctx, const uint8_t* rgba8, int n);
 int tvk_set_tf2d(tvk_ctx* ctx, const uint8_t* rgba8, int w, int h);
 
+/* 2D TF from swatches, TransferFunction2D::RenderTransferFunction8Bit + GetByteArray
+ * (IO/TransferFunction2D.cpp:222-241, :294-361) without Qt: the 1D TF (n1d RGBA8
+ * entries, may be 0) stretched over a canvas of GetRenderSize(), every swatch -- a
+ * polygon of n_points normalised (x, y) points filled with the odd-even rule, coloured
+ * by a linear gradient from (gradient[0], gradient[1]) to (gradient[2], gradient[3]) or
+ * a radial one centred at the first point, stops at positions 0..1 -- painted source-over
+ * in order, then the canvas scaled to w x h (nearest).  out: w*h RGBA8, row 0 first.
+ * Host code, no device needed.  Qt's own raster rules (pixel coverage, gradient
+ * rounding) are not reproduced bit for bit: the rules used are stated in
+ * csrc/host/tf2d.cpp. */
+typedef struct {
+  float pos;             /* GradientStop::first                                   */
+  float rgba[4];         /* GradientStop::second                                  */
+} tvk_gradient_stop;
+typedef struct {
+  int radial;            /* TFPolygon::bRadial                                     */
+  uint32_t first_point, n_points;   /* into points[] (pairs)                      */
+  float gradient[4];     /* TFPolygon::pGradientCoords[0].xy, [1].xy (normalised)  */
+  uint32_t first_stop, n_stops;     /* into stops[]                               */
+} tvk_swatch;
+int tvk_rasterize_tf2d(int w, int h, const uint8_t* tf1d_rgba8, int n1d, const tvk_swatch* swatches, int nswatch,
+                       const float* points, size_t npoints, const tvk_gradient_stop* stops, size_t nstops,
+                       uint8_t* out);
+/* Changed2DTrans with swatches: rasterizes as above and sets the table (tvk_set_tf2d) */
+int tvk_set_tf2d_swatches(tvk_ctx* ctx, int w, int h, const uint8_t* tf1d_rgba8, int n1d, const tvk_swatch* swatches,
+                          int nswatch, const float* points, size_t npoints, const tvk_gradient_stop* stops,
+                          size_t nstops);
+
 /* ---- view ------------------------------------------------------------------ */
 /* Row-vector matrices in Basics/Vectors.h layout (array[0]=m11, array[12..14] = translation);
  * modelView = rot * trans * view (GLRenderer.cpp:627). */

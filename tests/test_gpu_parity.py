@@ -415,3 +415,29 @@ def test_display_over_background_closed_form():
     flat = r.Display((0.5, 0.5, 0.5))
     assert np.array_equal(flat[..., :3][frames[..., 3] == 0], np.full(((frames[..., 3] == 0).sum(), 3), 128, np.uint8))
     r.close()
+
+
+def test_swatch_2d_tf_renders_like_its_table(ren):
+    """Changed2DTrans with swatches (rasterised by the library) renders exactly as
+    the same table set as bytes, and as the oracle with that table."""
+    from tuvok_amd import TFPolygon, TransferFunction2D
+    vol, _ = setup_blobs(ren)
+    t = TransferFunction2D(256, 256)
+    t.Update1DTrans(tf1d_blue_white_red())
+    t.SwatchPushBack(TFPolygon([(0.45, 0.0), (0.95, 0.0), (0.95, 1.0), (0.45, 1.0)], gradient=((0.45, 0.5), (0.95, 0.5)),
+                               stops=((0.0, (0.2, 0.4, 1.0, 0.1)), (1.0, (1.0, 0.9, 0.2, 0.8)))))
+    t.SwatchPushBack(TFPolygon([(0.6, 0.2), (0.8, 0.5), (0.6, 0.8)], gradient=((0.7, 0.5), (0.8, 0.5)),
+                               stops=((0.0, (1, 0, 0, 1)), (1.0, (1, 0, 0, 0))), radial=True))
+    table = t.GetByteArray()
+    ren.SetRendermode(RM_2DTRANS)
+    ren.SetUseLighting(True)
+    ren.SetGradientScale(1.0 / ren.MaxGradientMagnitude())
+    ren.Changed2DTrans(t)
+    a = ren.Paint()
+    ren.Set2DTrans(table)
+    b = ren.Paint()
+    assert np.array_equal(a, b) and a[..., 3].max() > 0.05
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    ref, _ = O.render(ds, oracle_camera_from(ren), oracle_params_from(ren, tf2d=table))
+    mx, mean = err_stats(a, ref)
+    assert mx <= TOL_MAX and mean < TOL_MEAN

@@ -9,4 +9,5 @@ from ._abi import (GEN_BLOBS, GEN_ELLIPSOID_SDF, GEN_FBM_SHELLS, GEN_PHANTOM, RM
                    RM_ISOSURFACE, TvkError)
 from .batch import RenderBatch  # noqa: F401
 from .renderer import HIPRaycaster, rotation_y  # noqa: F401
-from .transfer import TransferFunction1D, TransferFunction2D, tf1d_blue_white_red, tf2d_gaussian_lobes  # noqa: F401
+from .transfer import (TFPolygon, TransferFunction1D, TransferFunction2D, tf1d_blue_white_red,  # noqa: F401
+                       tf2d_gaussian_lobes)

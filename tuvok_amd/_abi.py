@@ -29,6 +29,7 @@ EXPORTS = [
     "tvk_leap_page_in", "tvk_leap_resume_state",
     "tvk_set_bricks", "tvk_upload_brick", "tvk_set_pool_budget", "tvk_get_pool_stats",
     "tvk_leap_set_request_table", "tvk_leap_requests", "tvk_pool_resident",
+    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches",
 ]
 
 
@@ -76,6 +77,15 @@ class PoolStats(C.Structure):
     _fields_ = [("pool_bytes", C.c_uint64), ("slots", C.c_uint64), ("slot_bytes", C.c_uint64),
                 ("resident", C.c_uint64), ("uploads", C.c_uint64), ("evictions", C.c_uint64),
                 ("upload_bytes", C.c_uint64), ("upload_ms", C.c_float), ("parts", C.c_uint32)]
+
+
+class GradientStop(C.Structure):
+    _fields_ = [("pos", C.c_float), ("rgba", C.c_float * 4)]
+
+
+class Swatch(C.Structure):
+    _fields_ = [("radial", C.c_int), ("first_point", C.c_uint32), ("n_points", C.c_uint32),
+                ("gradient", C.c_float * 4), ("first_stop", C.c_uint32), ("n_stops", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -174,6 +184,10 @@ def load(path=LIB_PATH):
         "tvk_leap_set_request_table": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
         "tvk_leap_requests": (C.c_int, [vp, u32p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "tvk_pool_resident": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_size_t]),
+        "tvk_rasterize_tf2d": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int, C.POINTER(Swatch), C.c_int,
+                                         fp, C.c_size_t, C.POINTER(GradientStop), C.c_size_t, C.POINTER(C.c_uint8)]),
+        "tvk_set_tf2d_swatches": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int, C.POINTER(Swatch),
+                                            C.c_int, fp, C.c_size_t, C.POINTER(GradientStop), C.c_size_t]),
         "tvk_get_pool_stats": (C.c_int, [vp, C.POINTER(PoolStats)]),
         "tvk_generate_segments": (C.c_int, [vp, C.c_int, fp, C.c_uint32, C.c_float, C.c_uint32, fp,
                                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),

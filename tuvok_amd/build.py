@@ -32,6 +32,7 @@ SOURCES = [
     ("leap_api.cpp", ["-ffp-contract=off"]),
     ("noalloc.hip", ["-ffp-contract=off"]),
     ("pool.cpp", ["-ffp-contract=off"]),
+    ("host/tf2d.cpp", ["-ffp-contract=off"]),
 ]
 
 COMMON = [

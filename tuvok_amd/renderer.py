@@ -225,6 +225,21 @@ class HIPRaycaster:
         h, w = t.shape[:2]
         self._chk(self._L.tvk_set_tf2d(self._h, t.ctypes.data_as(C.POINTER(C.c_uint8)), w, h))
 
+    def Changed2DTrans(self, tf2d):
+        """AbstrRenderer::Changed2DTrans: a TransferFunction2D with swatches (and the
+        1D TF under them) is painted by the library's rasteriser and set; a plain
+        float table is set as its bytes."""
+        from .transfer import _swatch_arrays
+        if not tf2d.swatches and tf2d.trans1d is None:
+            self.Set2DTrans(tf2d.GetByteArray())
+            return
+        w, h = tf2d.GetSize()
+        sw, pa, st, npts, nst = _swatch_arrays(tf2d.swatches)
+        t1 = tf2d.trans1d if tf2d.trans1d is not None else np.zeros((1, 4), np.uint8)
+        self._chk(self._L.tvk_set_tf2d_swatches(self._h, w, h, t1.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                0 if tf2d.trans1d is None else t1.shape[0], sw, len(tf2d.swatches),
+                                                _fp(pa), npts, st, nst))
+
     # -- state (AbstrRenderer setters) -----------------------------------------
     def _push(self):
         self._chk(self._L.tvk_set_params(self._h, C.byref(self._p)))

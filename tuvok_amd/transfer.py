@@ -61,14 +61,87 @@ class TransferFunction1D:
         return (int(nz[0]), int(nz[-1]))
 
 
+class TFPolygon:
+    """A swatch of the 2D TF (IO/TransferFunction2D.h, TFPolygon): a polygon in
+    normalised table coordinates filled with a linear (point 0 -> point 1) or
+    radial (centre point 0, radius |point 1 - point 0|) gradient; stops are
+    (position, (r, g, b, a)) with floats in 0..1."""
+
+    def __init__(self, points, gradient=((0.0, 0.5), (1.0, 0.5)), stops=((0.0, (0, 0, 0, 1)), (1.0, (1, 1, 1, 1))),
+                 radial=False):
+        self.pPoints = [tuple(map(float, p)) for p in points]
+        self.pGradientCoords = [tuple(map(float, gradient[0])), tuple(map(float, gradient[1]))]
+        self.pGradientStops = [(float(t), tuple(map(float, c))) for t, c in stops]
+        self.bRadial = bool(radial)
+
+
+def _swatch_arrays(swatches):
+    """The C ABI's swatch, point and stop arrays of a list of TFPolygon."""
+    import ctypes as C
+    from . import _abi
+    sw = (_abi.Swatch * max(1, len(swatches)))()
+    pts, stops = [], []
+    for i, s in enumerate(swatches):
+        sw[i].radial = int(s.bRadial)
+        sw[i].first_point, sw[i].n_points = len(pts) // 2, len(s.pPoints)
+        for p in s.pPoints:
+            pts.extend(p)
+        g = list(s.pGradientCoords[0]) + list(s.pGradientCoords[1])
+        for k in range(4):
+            sw[i].gradient[k] = g[k]
+        sw[i].first_stop, sw[i].n_stops = len(stops), len(s.pGradientStops)
+        stops.extend(s.pGradientStops)
+    pa = np.ascontiguousarray(pts if pts else [0.0], np.float32)
+    st = (_abi.GradientStop * max(1, len(stops)))()
+    for i, (t, c) in enumerate(stops):
+        st[i].pos = t
+        for k in range(4):
+            st[i].rgba[k] = c[k]
+    return sw, pa, st, len(pts) // 2, len(stops)
+
+
 class TransferFunction2D:
-    """2D TF over (value, 1 - |gradient| * scale); rows index the gradient axis."""
+    """2D TF over (value, 1 - |gradient| * scale); rows index the gradient axis.
+
+    Either a float table (`color`) or, once swatches or a 1D TF are set, the
+    table painted from them (TransferFunction2D::RenderTransferFunction8Bit,
+    rasterised by libtvkhip.so's tvk_rasterize_tf2d without Qt)."""
 
     def __init__(self, width=256, height=256):
         self.color = np.zeros((height, width, 4), np.float32)
+        self.swatches = []
+        self.trans1d = None   # RGBA8 bytes of the 1D TF painted under the swatches
+
+    def SwatchPushBack(self, swatch):
+        self.swatches.append(swatch)
+
+    def Update1DTrans(self, tf1d):
+        """TransferFunction2D::Update1DTrans: the 1D TF (TransferFunction1D or RGBA8 bytes)."""
+        b = tf1d.GetByteArray() if hasattr(tf1d, "GetByteArray") else tf1d
+        self.trans1d = np.ascontiguousarray(b, np.uint8).reshape(-1, 4)
+
+    def GetSize(self):
+        return (self.color.shape[1], self.color.shape[0])
 
     def GetByteArray(self, used_range=255):
-        return _to_bytes(self.color, used_range)
+        if not self.swatches and self.trans1d is None:
+            return _to_bytes(self.color, used_range)
+        import ctypes as C
+        from . import _abi
+        L = _abi.load()
+        w, h = self.GetSize()
+        out = np.zeros((h, w, 4), np.uint8)
+        sw, pa, st, npts, nst = _swatch_arrays(self.swatches)
+        t1 = self.trans1d if self.trans1d is not None else np.zeros((1, 4), np.uint8)
+        rc = L.tvk_rasterize_tf2d(w, h, t1.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  0 if self.trans1d is None else t1.shape[0], sw, len(self.swatches),
+                                  pa.ctypes.data_as(C.POINTER(C.c_float)), npts, st, nst,
+                                  out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            raise _abi.TvkError(rc, "tvk_rasterize_tf2d failed")
+        if used_range != 255:   # GetByteArray(pcData, cUsedRange): byte * 255 / range, truncated
+            out = (out.astype(np.float32) * np.float32(255.0 / used_range)).astype(np.uint8)
+        return out
 
     def GetNonZeroLimits(self):
         a = self.GetByteArray()[:, :, 3]
