@@ -144,6 +144,15 @@ int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src);
  * parameters, on the device); used by the brick-size autotuner. */
 int tvk_rebrick(tvk_ctx* ctx, const uint32_t brick[3], uint32_t overlap, int max_lods);
 int tvk_download_volume(tvk_ctx* ctx, void* host_out, size_t bytes);
+/* AbstrRenderer::CropDataset (GLRenderer.cpp:2953-2971, UVFDataset::Crop,
+ * uvfDataset.cpp:1470-1580): the clip plane without the viewing transformation,
+ * made scale invariant, zeroes every LOD-0 voxel whose normalised coordinate
+ * (x/X - 0.5, ...) satisfies (n.p) + w >= 0 (CropData, uvfDataset.h:252-297); the
+ * volume is then bricked again with the dataset's brick size, overlap and level
+ * count (the kept linear volume, or LOD 0 flattened from the resident bricks).  No
+ * file is written (keep_old has no effect).  plane_used (may be NULL) receives the
+ * normalised-coordinate plane that was applied. */
+int tvk_crop_dataset(tvk_ctx* ctx, int keep_old, float plane_used[4]);
 int tvk_dataset_info(tvk_ctx* ctx, tvk_dataset_desc* out);
 int tvk_lod_info(tvk_ctx* ctx, int lod, uint64_t size[3], uint64_t bricks[3], uint64_t* offset,
                  double aspect[3]);
@@ -249,6 +258,15 @@ int tvk_set_camera(tvk_ctx* ctx, const float eye[3], const float at[3], const fl
 /* RenderRegion rotation (e.g. an orbit); translation stays as set */
 int tvk_set_rotation(tvk_ctx* ctx, const float rot[16]);
 int tvk_get_view_matrices(tvk_ctx* ctx, float view[16], float proj[16]);
+/* AbstrRenderer::SetClipPlane / EnableClipPlane / DisableClipPlane
+ * (Renderer/AbstrRenderer.h:215-219): plane = ExtendedPlane::Plane() in world space
+ * (x, y, z, w; NULL keeps the current one), enabled = m_bClipPlaneOn.  Bricks whose
+ * eight corners are all clipped ((n.p) + w >= 0 after rotation * translation) leave
+ * the visible-brick list (AbstrRenderer::Clipped, AbstrRenderer.cpp:923-948), and
+ * every brick's ray segment keeps the part with n.p + d <= 0 of the plane carried
+ * to model space (GLRaycaster::RenderBox + Clipper::BoxPlane, GLRaycaster.cpp:328-336).
+ * The ray-guided traversal has no clip plane (as GLGridLeaper). */
+int tvk_set_clip_plane(tvk_ctx* ctx, const float plane[4], int enabled);
 
 /* ---- parameters, planning, rendering -------------------------------------- */
 int tvk_set_params(tvk_ctx* ctx, const tvk_params* p);
@@ -277,7 +295,7 @@ int tvk_touched_bytes(tvk_ctx* ctx, uint64_t* bytes, uint64_t* bricks, uint64_t*
  * planned brick list, view constants and sample counter go to one static
  * 8 MiB device array per GPU, and the frames (n x W x H RGBA32F, premultiplied,
  * GL row order) to d_out, device memory of out_bytes bytes that the caller
- * owns.  rots may be NULL when n == 1 (the context's rotation).  Limits: 302
+ * owns.  rots may be NULL when n == 1 (the context's rotation).  Limits: 281
  * views and 52,019 visible non-empty bricks summed over the views; beyond
  * them, or with a too small d_out, TVK_ERR_ARG.  Fixed step only (a context
  * with a step proposal attached gets TVK_ERR_UNSUPPORTED); the dataset, TF and
@@ -377,6 +395,11 @@ int tvk_leap_missing(tvk_ctx* ctx, uint8_t* flags, size_t n, uint64_t* count);
  * into the slots of the oldest uploads (at most all slots but the coarsest
  * brick's per call), and the bricks replaced become missing. */
 int tvk_leap_page_in(tvk_ctx* ctx, uint64_t max_bricks, uint64_t* paged);
+/* GLRenderer::Pick (GLRenderer.cpp:2856-2872): the eye-space isosurface hit of the
+ * last tvk_render / tvk_render_views frame (view 0) at window position (x, y), y from
+ * the top (read at row H - y).  TVK_ERR_STATE outside isosurface mode, TVK_ERR_RANGE
+ * when the ray hit nothing ("No intersection."). */
+int tvk_pick(tvk_ctx* ctx, uint32_t x, uint32_t y, float out[3]);
 /* residency per TOC brick: in a slot (budgeted pool), or flagged resident */
 int tvk_pool_resident(tvk_ctx* ctx, uint8_t* flags, size_t n);
 /* resume state of the last pass: per pixel 8 floats (normalised xyz, eye z or 1000 when finished, RGBA) */

@@ -88,6 +88,8 @@ typedef struct {
   int   clamp_to_edge;  /* volume wrap: 0 = GL_CLAMP (border 0), 1 = CLAMP_TO_EDGE */
   float ert_eps;        /* 0 = reference semantics; >0: stop a ray once its
                            total transmittance drops below ert_eps           */
+  int   clip_on;        /* m_bClipPlaneOn                                      */
+  float clip[4];        /* m_ClipPlane.Plane() (world space: x, y, z, w)      */
 } orc_params;
 
 typedef struct {

@@ -119,6 +119,26 @@ inline M4 inverse(const M4& M) {
   return R;
 }
 
+// The clip plane in model space as GLRaycaster::RenderBox forms it
+// (GLRaycaster.cpp:328-336): m_ClipPlane.Plane() * inverse(rotation * translation),
+// where PLANE * M is PLANE::transform (Vectors.h:1460-1485): the row vector times
+// the transposed inverse of M, divided by the length of its xyz; then the
+// normal normalised again and d kept.  Points with n.p + d <= 0 stay
+// (Clipper::TriPlane keeps fa <= 0, Clipper.cpp:78-104).
+inline void model_clip_plane(const float plane[4], const M4& rot, const M4& trans, float out[4]) {
+  const M4 inv = inverse(mul(rot, trans));
+  const M4 ii = inverse(inv);
+  M4 it;
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) it.m(r, c) = ii.m(c, r);
+  float t[4];
+  vmul4(plane, it, t);
+  const float l = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  for (float& v : t) v /= l;
+  const float l2 = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  out[0] = t[0] / l2; out[1] = t[1] / l2; out[2] = t[2] / l2; out[3] = t[3];
+}
+
 // Vectors.h:970-975
 inline M4 translation(float x, float y, float z) {
   M4 R; R.m(3, 0) = x; R.m(3, 1) = y; R.m(3, 2) = z; return R;

@@ -158,9 +158,13 @@ struct Frame {
   std::vector<Entry> bricks;
   Tf tf;
   V3 amb, dif, spc, ldir, dscale, iso_col;
+  bool clip_on = false;
+  float clip[4] = {0, 0, 0, 0};   // model space, GLRaycaster::RenderBox
 
   bool setup(const orc_ds* d, const orc_view* v, const orc_params* prm) {
     ds = d; p = prm;
+    clip_on = prm->clip_on != 0;
+    if (clip_on) model_clip_plane(prm->clip, from_arr(v->rot), from_arr(v->trans), clip);
     W = (int)v->width; H = (int)v->height;
     znear = v->znear; zfar = v->zfar;
     mv = mul(mul(from_arr(v->rot), from_arr(v->trans)), from_arr(v->view));
@@ -250,6 +254,15 @@ struct Frame {
       float t0 = (b.bmin[a] - o_m[a]) * inv, t1 = (b.bmax[a] - o_m[a]) * inv;
       tn = std::max(tn, std::min(t0, t1));
       tf = std::min(tf, std::max(t0, t1));
+    }
+    if (clip_on) {
+      // the brick box cut by the clip plane and closed by its cap (Clipper::BoxPlane,
+      // Clipper.cpp:106-131): the ray keeps the part with n . p + d <= 0
+      const float a = clip[0] * d_m.x + clip[1] * d_m.y + clip[2] * d_m.z;
+      const float c = clip[0] * o_m.x + clip[1] * o_m.y + clip[2] * o_m.z + clip[3];
+      if (a > 0.0f) tf = std::min(tf, -c / a);
+      else if (a < 0.0f) tn = std::max(tn, -c / a);
+      else if (c > 0.0f) return false;
     }
     return tf > tn && tf > 0.0f;
   }

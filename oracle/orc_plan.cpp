@@ -104,12 +104,28 @@ float brick_distance(const V3& center, const V3& ext, const M4& mv) {
   return d;
 }
 
+// AbstrRenderer::Clipped (AbstrRenderer.cpp:923-948): the brick's eight corners
+// through rotation * translation (dehomogenised); clipped when every corner
+// satisfies PLANE::clip, (n . p) + w >= 0 (Vectors.h:1455-1457).
+bool clipped(const V3& c, const V3& e, const M4& world, const float pl[4]) {
+  for (int i = 0; i < 8; ++i) {
+    const V3 s((i & 4) ? +e.x : -e.x, (i & 2) ? +e.y : -e.y, (i & 1) ? +e.z : -e.z);
+    const V3 q = c + s * 0.5f;
+    float v[4] = {q.x, q.y, q.z, 1.0f}, o[4];
+    vmul4(v, world, o);
+    const V3 w(o[0] / o[3], o[1] / o[3], o[2] / o[3]);
+    if (!((pl[0] * w.x + pl[1] * w.y + pl[2] * w.z) + pl[3] >= 0.0f)) return false;
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" int orc_plan(const orc_ds* ds, const orc_view* view, const orc_params* prm,
                         orc_brick* out, int cap) {
   const Octree& t = ds->tree;
   M4 mv = modelview_of(view);
+  const M4 world = mul(from_arr(view->rot), from_arr(view->trans));
   Cull cull;
   cull.update(mv, from_arr(view->proj));
 
@@ -151,6 +167,7 @@ extern "C" int orc_plan(const orc_ds* ds, const orc_view* view, const orc_params
     for (int i = 0; i < 3; ++i) { b.center[i] = cen[i]; b.extension[i] = ext[i]; b.voxel_count[i] = bmd.n_voxels[i]; }
     // RegionNeedsBrick (AbstrRenderer.cpp:874-920)
     if (!cull.visible(bmd.center * vScaleVis, bmd.extents * vScaleVis)) continue;
+    if (prm->clip_on && clipped(bmd.center * vScaleVis, bmd.extents * vScaleVis, world, prm->clip)) continue;
     // ContainsData (AbstrRenderer.cpp:953-997; uvfDataset.cpp:1201-1227)
     const double* mm = t.toc[b.toc_index].minmax;
     bool contains;

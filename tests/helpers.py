@@ -45,7 +45,8 @@ def oracle_params_from(ren, tf1d=None, tf2d=None, bit_width=None, tf_scale=None)
                 domain_scale=tuple(p.domain_scale), light_ambient=tuple(p.light_ambient),
                 light_diffuse=tuple(p.light_diffuse), light_specular=tuple(p.light_specular),
                 light_dir=tuple(p.light_dir), iso_color=tuple(p.iso_color), refine_steps=p.refine_steps,
-                clamp_to_edge=p.clamp_to_edge, ert_eps=p.ert_eps)
+                clamp_to_edge=p.clamp_to_edge, ert_eps=p.ert_eps,
+                clip=getattr(ren, "_clip", None) if getattr(ren, "_clip_on", False) else None)
 
 
 def err_stats(a, b):

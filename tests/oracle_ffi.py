@@ -31,7 +31,8 @@ class OrcParams(C.Structure):
                 ("light_ambient", C.c_float * 3), ("light_diffuse", C.c_float * 3),
                 ("light_specular", C.c_float * 3), ("light_dir", C.c_float * 3),
                 ("iso_color", C.c_float * 3), ("refine_steps", C.c_int),
-                ("clamp_to_edge", C.c_int), ("ert_eps", C.c_float)]
+                ("clamp_to_edge", C.c_int), ("ert_eps", C.c_float), ("clip_on", C.c_int),
+                ("clip", C.c_float * 4)]
 
 
 class OrcBrick(C.Structure):
@@ -295,6 +296,9 @@ def make_params(p, keep):
     o.refine_steps = p.get("refine_steps", 5)
     o.clamp_to_edge = int(p.get("clamp_to_edge", 0))
     o.ert_eps = p.get("ert_eps", 0.0)
+    if p.get("clip") is not None:
+        o.clip_on = 1
+        o.clip[:] = [float(v) for v in p["clip"]]
     return o
 
 

@@ -441,3 +441,134 @@ def test_swatch_2d_tf_renders_like_its_table(ren):
     ref, _ = O.render(ds, oracle_camera_from(ren), oracle_params_from(ren, tf2d=table))
     mx, mean = err_stats(a, ref)
     assert mx <= TOL_MAX and mean < TOL_MEAN
+
+
+@pytest.mark.parametrize("mode,lit", [(RM_1DTRANS, True), (RM_2DTRANS, False), (RM_ISOSURFACE, False)])
+def test_clip_plane_matches_oracle(ren, mode, lit):
+    """SetClipPlane + EnableClipPlane: the culled list (AbstrRenderer::Clipped) and
+    the clipped frames equal the oracle's; bricks and samples are removed."""
+    vol, tf = setup_blobs(ren, eye=(0.1, 0.2, 1.9))
+    ren.SetRotation(rotation_y(0.5))
+    ren.SetRendermode(mode)
+    ren.SetUseLighting(lit)
+    ren.SetIsoValue(100.0)
+    tf2 = tf2d_gaussian_lobes().GetByteArray()
+    ren.Set2DTrans(tf2)
+    ren.SetGradientScale(1.0 / ren.MaxGradientMagnitude())
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    ren.SetCountSamples(True)
+    full = ren.Paint()
+    full_n, full_list = ren.last_stats.samples, len(ren.PlanFrame())
+    try:
+        ren.SetClipPlane((0.6, -0.3, 0.74, 0.08))
+        ren.EnableClipPlane(True)
+        rot = rotation_y(0.5)
+        got = ren.PlanFrame()
+        want = O.plan(ds, oracle_camera_from(ren, rot), oracle_params_from(ren, tf1d=tf, tf2d=tf2))
+        assert [(g.toc_index, g.is_empty) for g in got] == [(w.toc_index, w.is_empty) for w in want]
+        assert len(got) < full_list
+        img, ref, mx, mean, ns, hs = compare(ren, ds, tf1d=tf, tf2d=tf2, rot=rot)
+        print(f"clip mode={mode} lit={lit}: max {mx:.3g} mean {mean:.3g} samples {hs}/{full_n}", flush=True)
+        assert mx <= TOL_MAX and mean < TOL_MEAN
+        assert hs == ns and hs < full_n
+        assert not np.array_equal(img, full)
+    finally:
+        ren.DisableClipPlane()
+        ren.SetRotation(np.eye(4, dtype=np.float32))
+
+
+def test_pick_returns_the_isosurface_hit(ren):
+    """GLRenderer::Pick on a sphere SDF: a window position covered by the sphere gives
+    an eye-space point on the sphere (within a voxel of its radius) that projects
+    back onto that pixel; the background raises "No intersection"; outside
+    isosurface mode Pick is refused."""
+    n = 64
+    g = (np.arange(n) + 0.5) / n - 0.5
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    rad = 0.3
+    sdf = np.clip(128 + 20 * (rad - np.sqrt(x * x + y * y + z * z)) * n, 0, 255).astype(np.uint8)
+    ren.LoadVolume(sdf, (18, 18, 18), 1, max_lods=1)
+    ren.SetRendermode(RM_ISOSURFACE)
+    ren.SetIsoValue(128.0)
+    ren.SetViewParameters(45.0, 0.01, 1000.0)
+    ren.SetViewPosition((0.0, 0.0, 2.0))
+    ren.SetRotation(np.eye(4, dtype=np.float32))
+    W, H = 96, 80
+    ren.Resize(W, H)
+    img = ren.Paint()
+    view, proj = ren.GetViewMatrices()
+    inv_view = np.linalg.inv(view.astype(np.float64))
+    checked = 0
+    for (px, py) in [(48, 40), (40, 35), (55, 46), (47, 30)]:
+        row = H - py
+        assert img[row, px, 3] == 1.0
+        p = ren.Pick(px, py).astype(np.float64)
+        m = np.append(p, 1.0) @ inv_view                 # eye -> model (rotation is identity)
+        assert abs(np.linalg.norm(m[:3]) - rad) < 1.5 / n, (px, py, np.linalg.norm(m[:3]))
+        c = np.append(p, 1.0) @ proj.astype(np.float64)
+        sx, sy = (c[0] / c[3] + 1) * 0.5 * W, (c[1] / c[3] + 1) * 0.5 * H
+        assert abs(sx - (px + 0.5)) < 0.01 and abs(sy - (row + 0.5)) < 0.01, (sx, sy, px, row)
+        checked += 1
+    assert checked == 4
+    with pytest.raises(Exception, match="No intersection"):
+        ren.Pick(2, 2)
+    ren.SetRendermode(RM_1DTRANS)
+    with pytest.raises(Exception, match="isosurface"):
+        ren.Pick(48, 40)
+
+
+def test_crop_dataset_matches_closed_form(ren):
+    """CropDataset: the LOD-0 voxels on the clipped side of the plane (taken out of
+    the viewing transformation and made scale invariant) become 0 and the volume
+    is bricked again: the kept volume equals the numpy crop with the plane used,
+    the plane is the float64 restatement's, and the pool is the oracle's bricking
+    of the cropped volume -- for a loaded volume and for bricks handed over key
+    by key (flattened on the device)."""
+    dims = (40, 32, 36)
+    vol = np.random.default_rng(71).integers(1, 256, size=dims[::-1]).astype(np.uint8)
+    plane = np.array([0.4, -0.5, 0.77, 0.05], np.float32)
+    rot = rotation_y(0.8)
+    for ingest_bricks in (False, True):
+        if ingest_bricks:
+            ds0 = O.Dataset(vol, (12, 12, 12), 1)
+            keys = []
+            for i in range(ds0.brick_count):
+                info = ds0.brick_info(i)
+                x, y, z, lod = info["coords"]
+                bx, by, _ = ds0.lod_bricks(lod)
+                keys.append((lod, x + bx * (y + by * z), info["minmax"]))
+            ren.SetBricks(np.uint8, dims, (12, 12, 12), 1, keys)
+            for i, (lod, index, _) in enumerate(keys):
+                ren.UploadBrick(lod, index, ds0.brick_data(i))
+        else:
+            ren.LoadVolume(vol, (12, 12, 12), 1, max_lods=0, keep_linear=True)
+        ren.SetViewParameters(45.0, 0.01, 1000.0)
+        ren.SetViewPosition((0.0, 0.0, 2.0))
+        ren.Resize(32, 32)
+        ren.SetRotation(rot)
+        ren.SetClipPlane(plane)
+        used = ren.CropDataset()
+        # the plane in float64: world plane * (world^-1)^-T, normalised; then times
+        # the scale-invariance scaling, normalised
+        world = rot.astype(np.float64)
+        pm = plane.astype(np.float64) @ world.T
+        pm /= np.linalg.norm(pm[:3])
+        sc = np.array(dims, np.float64) / max(dims)
+        pm = np.append(pm[:3] * sc, pm[3])
+        pm /= np.linalg.norm(pm[:3])
+        assert np.abs(used - pm).max() < 1e-5
+        X, Y, Z = dims
+        zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+        f32 = np.float32
+        vx = (xx.astype(f32) / f32(X)) - f32(0.5)
+        vy = (yy.astype(f32) / f32(Y)) - f32(0.5)
+        vz = (zz.astype(f32) / f32(Z)) - f32(0.5)
+        d = ((used[0] * vx + used[1] * vy) + used[2] * vz) + used[3]
+        want = np.where(d >= 0, 0, vol).astype(np.uint8)
+        assert 0.2 < (want == 0).mean() < 0.8
+        ds = O.Dataset(want, (12, 12, 12), 1)
+        assert np.array_equal(ren.GetPool(), ds.all_data())
+        if not ingest_bricks:
+            assert np.array_equal(ren.DownloadVolume(), want)
+    ren.SetRotation(np.eye(4, dtype=np.float32))
+    ren.DisableClipPlane()

@@ -202,6 +202,39 @@ def test_plan_closed_form(seed, close):
         assert culled > 0
 
 
+@needs_ref
+@pytest.mark.parametrize("seed", [5, 6, 7])
+def test_plan_with_clip_plane_closed_form(seed):
+    """With the clip plane on, the list is the frustum-visible bricks of which at
+    least one corner (scaled box, through rotation * translation) is not clipped,
+    (n . p) + w < 0 (AbstrRenderer::Clipped, AbstrRenderer.cpp:923-948)."""
+    rng = np.random.default_rng(seed)
+    dims = (64, 48, 56)
+    vol = np.full(dims[::-1], 100, np.uint8)
+    ds = O.Dataset(vol, (18, 18, 18), 1)
+    tf = np.zeros((256, 4), np.uint8)
+    tf[50:, 3] = 40
+    cam, *_ = random_camera(rng, False)
+    nrm = rng.normal(size=3)
+    plane = np.append(nrm / np.linalg.norm(nrm), rng.uniform(-0.2, 0.2)).astype(np.float32)
+    params = dict(mode=0, lighting=0, lod=0, bit_width=8, tf1d=tf)
+    full = {int(b.key_index) for b in O.plan(ds, cam, params)}
+    got = {int(b.key_index) for b in O.plan(ds, cam, dict(params, clip=plane))}
+    world = (cam["rot"] @ cam["trans"]).astype(np.float64)
+    sc = corrected_scale((1, 1, 1), ds.lod_size(0)).astype(np.float64)
+    want = set()
+    for k in full:
+        c, e, _ = ds.brick_md(0, k)
+        sgn = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64)
+        pts = c.astype(np.float64) * sc + sgn * e.astype(np.float64) * sc * 0.5
+        q = np.concatenate([pts, np.ones((8, 1))], 1) @ world
+        q = q[:, :3] / q[:, 3:]
+        if not np.all(q @ plane[:3].astype(np.float64) + plane[3] >= 0):
+            want.add(k)
+    assert got == want
+    assert 0 < len(got) < len(full)
+
+
 # ---- FillOverlap's copy order ----------------------------------------------------------
 def fill_overlap_unfilled(bricks, sizes, ov):
     """Which overlap voxels of each brick of a coarse level FillOverlap leaves without

@@ -47,8 +47,10 @@ def test_tf_bytes_and_limits_known_answer():
     assert O.tf2d_limits(b2) == tf2.GetNonZeroLimits()
 
 
-def segment_lengths_np(cam, ds_dims, brick, ov, px, py):
-    """Per-brick eye-space segment lengths of the pixel's ray (numpy, float64)."""
+def segment_lengths_np(cam, ds_dims, brick, ov, px, py, plane=None):
+    """Per-brick eye-space segment lengths of the pixel's ray (numpy, float64); with
+    a world-space clip plane, each segment keeps its part with n.p + d <= 0 in
+    model space (the plane through the inverse of rotation * translation)."""
     W, H = cam["width"], cam["height"]
     mv = (cam["rot"] @ cam["trans"] @ cam["view"]).astype(np.float64)
     imv = np.linalg.inv(mv)
@@ -70,6 +72,17 @@ def segment_lengths_np(cam, ds_dims, brick, ov, px, py):
                     t0 = (lo - o_m[:3]) / d_m[:3]
                     t1 = (hi - o_m[:3]) / d_m[:3]
                 tn, tf = np.max(np.minimum(t0, t1)), np.min(np.maximum(t0, t1))
+                if plane is not None:
+                    world = (cam["rot"] @ cam["trans"]).astype(np.float64)
+                    pm = np.asarray(plane, np.float64) @ world.T      # plane * (world^-1)^-T
+                    pm = pm / np.linalg.norm(pm[:3])
+                    a_, c_ = pm[:3] @ d_m[:3], pm[:3] @ o_m[:3] + pm[3]
+                    if a_ > 0:
+                        tf = min(tf, -c_ / a_)
+                    elif a_ < 0:
+                        tn = max(tn, -c_ / a_)
+                    elif c_ > 0:
+                        continue
                 if tf > tn and tf > 0:
                     entry = (d * tn).astype(np.float32).astype(np.float16).astype(np.float64)
                     out.append(np.linalg.norm(d * tf - entry))
@@ -93,6 +106,33 @@ def test_constant_field_compositing_closed_form():
         L = segment_lengths_np(cam, (n, n, n), brick, ov, px, py)
         expect = sum(int(l / step) + 1 for l in L)
         assert abs(ns - expect) <= 1, (ns, expect)
+
+
+@pytest.mark.parametrize("plane,rot", [((1.0, 0.0, 0.0, 0.05), 0.0), ((0.3, -0.2, 0.9, 0.1), 0.6),
+                                       ((-0.5, 0.7, 0.2, -0.15), 2.2)])
+def test_clip_plane_constant_field_closed_form(plane, rot):
+    """A clip plane cuts every brick's segment: on a constant field the pixel is
+    1-(1-a)^N with N from the numpy segments of the kept half-space, and some
+    pixels of the frame lose samples against the unclipped frame."""
+    n, brick, ov = 48, 18, 1
+    vol = np.full((n, n, n), 200, np.uint8)
+    ds = O.Dataset(vol, (brick,) * 3, ov)
+    cam = O.camera(64, 64, eye=(0.0, 0.0, 2.0), rot=O.rotation_y(rot))
+    tf = const_tf(3)
+    a = 3 / 255
+    step = (1.0 / n * core_frac(brick, ov)) * 0.5
+    p = params(tf1d=tf, clip=plane)
+    lost = 0
+    for (px, py) in [(32, 32), (20, 40), (45, 30), (10, 12), (52, 50)]:
+        img, ns = O.render(ds, cam, p, roi=(px, py, px + 1, py + 1))
+        _, ns_full = O.render(ds, cam, params(tf1d=tf), roi=(px, py, px + 1, py + 1))
+        rgba = img[0, 0]
+        assert rgba[3] == pytest.approx(1 - (1 - a) ** ns, rel=2e-5)
+        L = segment_lengths_np(cam, (n, n, n), brick, ov, px, py, plane=plane)
+        expect = sum(int(l / step) + 1 for l in L)
+        assert abs(ns - expect) <= 1, (ns, expect)
+        lost += ns_full - ns
+    assert lost > 0
 
 
 def core_frac(brick, ov):

@@ -82,7 +82,7 @@ def test_noalloc_equals_render_views(mode):
             rc, _, _ = _noalloc(ren, [np.eye(4)], 96, 80, dbuf, 96 * 80 * 16 - 4)
             assert rc == TVK_ERR_ARG
             ren.Resize(4, 4)
-            rc, _, _ = _noalloc(ren, [np.eye(4)] * 303, 4, 4, dbuf, cap)
+            rc, _, _ = _noalloc(ren, [np.eye(4)] * 282, 4, 4, dbuf, cap)
             assert rc == TVK_ERR_ARG
             assert b"limit" in ren._L.tvk_last_error(ren._h)
         finally:

@@ -293,6 +293,7 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_counter) (void)hipFree(c->d_counter);
   if (c->d_touched) (void)hipFree(c->d_touched);
   if (c->d_depth) (void)hipFree(c->d_depth);
+  if (c->d_hitpos) (void)hipFree(c->d_hitpos);
   if (c->h_list_pinned) (void)hipHostFree(c->h_list_pinned);
   if (c->h_views_pinned) (void)hipHostFree(c->h_views_pinned);
   for (auto& ev : c->ev) if (ev) (void)hipEventDestroy(ev);
@@ -436,6 +437,84 @@ int tvk_rebrick(tvk_ctx* c, const uint32_t brick[3], uint32_t overlap, int max_l
     restore_dataset(c, std::move(old));
     c->err = msg;
   }
+  return rc;
+}
+
+int tvk_crop_dataset(tvk_ctx* c, int keep_old, float plane_used[4]) {
+  (void)keep_old;   // no file is replaced: the cropped dataset replaces this context's
+  if (!c) return TVK_ERR_ARG;
+  if (!c->has_ds) return fail(c, TVK_ERR_STATE, "no dataset");
+  if (c->ingest_pending) return fail(c, TVK_ERR_STATE, "not every brick was uploaded");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const Layout& L = c->L;
+  // GLRenderer::CropDataset (GLRenderer.cpp:2953-2971): the plane without the
+  // viewing transformation, ExtendedPlane::Transform(inverse(rotation * translation))
+  // = PLANE * M (the row vector times inverse(M) transposed, divided by |xyz|)
+  float pm[4];
+  {
+    const Mat4 inv = inverse(c->V.rot * c->V.trans);
+    const Mat4 ii = inverse(inv);
+    Mat4 it;
+    for (int r = 0; r < 4; ++r)
+      for (int q = 0; q < 4; ++q) it.at(r, q) = ii.at(q, r);
+    row_mul(c->V.clip, it, pm);
+    const float l = std::sqrt(pm[0] * pm[0] + pm[1] * pm[1] + pm[2] * pm[2]);
+    for (float& v : pm) v /= l;
+  }
+  // UVFDataset::Crop (uvfDataset.cpp:1470-1486): scale invariance,
+  // transformIT(Scaling(scale / max scale * domain / max domain)) = the row vector
+  // times that matrix, divided by |xyz|
+  {
+    const double dmax = (double)std::max(L.dims[0], std::max(L.dims[1], L.dims[2]));
+    const float smax = std::max(L.scale[0], std::max(L.scale[1], L.scale[2]));
+    float sc[3];
+    for (int k = 0; k < 3; ++k) sc[k] = (L.scale[k] / smax) * (float)((double)L.dims[k] / dmax);
+    const float t[4] = {pm[0] * sc[0], pm[1] * sc[1], pm[2] * sc[2], pm[3]};
+    const float l = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    for (int k = 0; k < 4; ++k) pm[k] = t[k] / l;
+  }
+  if (plane_used)
+    for (int k = 0; k < 4; ++k) plane_used[k] = pm[k];
+  // the LOD-0 volume: the kept linear copy, or flattened from the resident bricks
+  std::shared_ptr<void> lin = c->lin_hold;
+  void* d_lin = c->d_lin;
+  const size_t bytes = (size_t)(L.dims[0] * L.dims[1] * L.dims[2]) * (size_t)L.bpv;
+  if (!d_lin) {
+    if (budgeted(c)) return fail(c, TVK_ERR_UNSUPPORTED, "cropping a budgeted dataset needs its linear volume");
+    HIPCHK(c, hipMalloc(&d_lin, bytes));
+    lin = std::shared_ptr<void>(d_lin, [](void* p) { (void)hipFree(p); });
+    const LodInfo& l0 = L.lods[0];
+    std::vector<GatherBrick> gb((size_t)(l0.bricks[0] * l0.bricks[1] * l0.bricks[2]));
+    for (size_t i = 0; i < gb.size(); ++i) {
+      const TocBrick& t = L.toc[l0.offset + i];
+      for (int k = 0; k < 3; ++k) { gb[i].c[k] = (uint32_t)t.coords[k]; gb[i].n[k] = t.size[k]; }
+      gb[i].offset = t.offset;
+    }
+    GatherBrick* d_gb = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_gb, gb.size() * sizeof(GatherBrick)));
+    std::shared_ptr<void> gb_hold(d_gb, [](void* p) { (void)hipFree(p); });
+    HIPCHK(c, hipMemcpyAsync(d_gb, gb.data(), gb.size() * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream));
+    const uint32_t nb[3] = {(uint32_t)l0.bricks[0], (uint32_t)l0.bricks[1], (uint32_t)l0.bricks[2]};
+    const uint32_t core[3] = {L.brick[0] - 2 * L.overlap, L.brick[1] - 2 * L.overlap, L.brick[2] - 2 * L.overlap};
+    HIPCHK(c, launch_flatten(c->d_pool, L.bpv, d_gb, nb, core, L.overlap, L.dims, d_lin, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  HIPCHK(c, launch_crop(d_lin, L.bpv, L.dims, pm, c->stream));
+  // RAWConverter::ConvertRAWDataset of the cropped volume with the same bricking
+  const uint32_t brick[3] = {L.brick[0], L.brick[1], L.brick[2]};
+  const uint32_t ov = L.overlap;
+  const int lods = (int)(L.toc.back().coords[3] + 1);
+  const uint64_t dims[3] = {L.dims[0], L.dims[1], L.dims[2]};
+  const int bpv = L.bpv, clamp = c->clamp;
+  int rc = begin_dataset(c, bpv, dims, brick, ov, clamp, lods);
+  if (rc) return rc;
+  c->lin_hold = lin;
+  c->d_lin = d_lin;
+  c->lin_bytes = bytes;
+  c->leap_resident.clear();
+  rc = build_pool(c, d_lin);
+  if (rc) free_dataset(c);
   return rc;
 }
 
@@ -735,6 +814,17 @@ int tvk_set_camera(tvk_ctx* c, const float eye[3], const float at[3], const floa
   c->has_view = true;
   c->proj_from_fov = true;
   c->leap_eye = F3(eye[0], eye[1], eye[2]);
+  return TVK_OK;
+}
+
+int tvk_set_clip_plane(tvk_ctx* c, const float plane[4], int enabled) {
+  if (!c) return TVK_ERR_ARG;
+  if (plane) {
+    if (!(plane[0] * plane[0] + plane[1] * plane[1] + plane[2] * plane[2] > 0.0f))
+      return fail(c, TVK_ERR_ARG, "clip plane normal is zero");
+    for (int i = 0; i < 4; ++i) c->V.clip[i] = plane[i];
+  }
+  c->V.clip_on = enabled != 0;
   return TVK_OK;
 }
 
@@ -1156,6 +1246,13 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
       return fail(c, TVK_ERR_STATE, "no step network");
   }
   if (learn == 2 && seg) a.seg = *seg;
+  c->hitpos_valid = false;
+  if (c->P.mode == TVK_RM_ISOSURFACE) {
+    rc = ensure(c, &c->d_hitpos, &c->hitpos_cap, (size_t)pixels);
+    if (rc) return rc;
+    a.hitpos = c->d_hitpos;
+    c->hitpos_valid = true;
+  }
   a.lighting = c->P.lighting;
   a.groups_total = groups;
   a.groups_padded = (groups + 7u) & ~7u;
@@ -1358,6 +1455,22 @@ int tvk_display(tvk_ctx* c, const float bg_top[3], const float bg_bottom[3], uin
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return hipfail(c, "display", e);
+  return TVK_OK;
+}
+
+int tvk_pick(tvk_ctx* c, uint32_t x, uint32_t y, float out[3]) {
+  if (!c || !out) return TVK_ERR_ARG;
+  if (c->P.mode != TVK_RM_ISOSURFACE || !c->hitpos_valid)
+    return fail(c, TVK_ERR_STATE, "Can only determine pick locations in isosurface rendering mode.");
+  // ReadBackPixels(mousePos.x, m_vWinSize.y - mousePos.y) of the first view
+  const uint32_t W = c->last_w, H = c->last_h;
+  if (x >= W || y > H || y == 0) return fail(c, TVK_ERR_RANGE, "pick position outside the frame");
+  const size_t pix = (size_t)(H - y) * W + x;
+  float4 v;
+  HIPCHK(c, hipMemcpyAsync(&v, c->d_hitpos + pix, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (v.w == 0.0f) return fail(c, TVK_ERR_RANGE, "No intersection.");
+  out[0] = v.x; out[1] = v.y; out[2] = v.z;
   return TVK_OK;
 }
 

@@ -44,6 +44,9 @@ hipError_t launch_cells(const void* pool, int bpv, const GatherBrick* d_bricks, 
                         uint32_t max_cells, uint32_t* out, hipStream_t st);
 hipError_t launch_brick_max_gradient(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb,
                                      uint32_t ov, uint32_t gx, double* out, hipStream_t st);
+hipError_t launch_flatten(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t nb[3],
+                          const uint32_t core[3], uint32_t ov, const uint64_t dims[3], void* out, hipStream_t st);
+hipError_t launch_crop(void* vol, int bpv, const uint64_t dims[3], const float pl[4], hipStream_t st);
 hipError_t launch_swizzle(const void* src, int bpv, const GatherBrick* d_bricks, const uint64_t* d_src_off, uint32_t nb,
                           uint64_t max_elems, void* pool, hipStream_t st);
 
@@ -173,6 +176,9 @@ struct tvk_ctx {
   size_t touched_cap = 0, touched_n = 0;
   float* d_depth = nullptr;          // isosurface depth of a frame rendered in parts
   size_t depth_cap = 0;
+  float4* d_hitpos = nullptr;        // isosurface hit positions of the last frame (Pick)
+  size_t hitpos_cap = 0;
+  bool hitpos_valid = false;
   uint32_t last_parts = 0;           // launches of the last budgeted frame
   hipEvent_t ev[8] = {};
   std::vector<FrameBrick> h_list;

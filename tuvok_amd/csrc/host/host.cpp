@@ -210,6 +210,21 @@ float brick_distance(F3 c, F3 e, const Mat4& mv) {
   return d;
 }
 
+// AbstrRenderer::Clipped (AbstrRenderer.cpp:923-948): the eight corners through
+// rotation * translation (dehomogenised); clipped when every corner satisfies
+// PLANE::clip, (n . p) + w >= 0 (Vectors.h:1455-1457)
+bool clipped(F3 c, F3 e, const Mat4& world, const float pl[4]) {
+  for (int i = 0; i < 8; ++i) {
+    const F3 s((i & 4) ? +e.x : -e.x, (i & 2) ? +e.y : -e.y, (i & 1) ? +e.z : -e.z);
+    const F3 q = c + s * 0.5f;
+    float v[4] = {q.x, q.y, q.z, 1.0f}, o[4];
+    row_mul(v, world, o);
+    const F3 w(o[0] / o[3], o[1] / o[3], o[2] / o[3]);
+    if (!((pl[0] * w.x + pl[1] * w.y + pl[2] * w.z) + pl[3] >= 0.0f)) return false;
+  }
+  return true;
+}
+
 }  // namespace
 
 void plan_frame(const Layout& L, const std::vector<double>& minmax, double range_max, const ViewState& V,
@@ -217,6 +232,7 @@ void plan_frame(const Layout& L, const std::vector<double>& minmax, double range
   out.clear();
   Mat4 mv = V.modelview();
   Frustum fr(mv * V.proj);
+  const Mat4 world = V.rot * V.trans;
   F3 vScale = corrected_scale(L, 0);
   F3 vScaleVis = corrected_scale(L, P.lod);
   double r = range_max / double(P.tf_n - 1);   // AbstrRenderer.cpp:960-963
@@ -233,6 +249,7 @@ void plan_frame(const Layout& L, const std::vector<double>& minmax, double range
     b.center = m.center * vScale;
     for (int i = 0; i < 3; ++i) b.voxel_count[i] = m.n_voxels[i];
     if (!fr.visible(m.center * vScaleVis, m.extents * vScaleVis)) continue;
+    if (V.clip_on && clipped(m.center * vScaleVis, m.extents * vScaleVis, world, V.clip)) continue;
     const double mn = minmax[2 * b.toc_index], mx = minmax[2 * b.toc_index + 1];
     bool contains;
     if (P.mode == 0 || P.mode == 1) {
@@ -279,6 +296,9 @@ void make_view_const(const ViewState& V, ViewConst& vc) {
     for (int c = 0; c < 3; ++c) vc.nm[r * 3 + c] = inv3.at(r, c);
   vc.znear = V.znear;
   vc.zfar = V.zfar;
+  vc.clip_on = V.clip_on ? 1u : 0u;
+  if (V.clip_on) model_clip_plane(V.clip, V.rot, V.trans, vc.clip);
+  else vc.clip[0] = vc.clip[1] = vc.clip[2] = vc.clip[3] = 0.0f;
   vc.W = V.W;
   vc.H = V.H;
   vc.tiles_x = (V.W + 7) / 8;

@@ -76,6 +76,8 @@ struct ViewState {
   Mat4 view, rot, trans, proj;
   uint32_t W = 0, H = 0;
   float fov = 50.f, znear = 0.01f, zfar = 1000.f;
+  bool clip_on = false;        // m_bClipPlaneOn
+  float clip[4] = {0, 0, 1, 0};   // m_ClipPlane.Plane(), world space
   Mat4 modelview() const { return rot * trans * view; }  // GLRenderer.cpp:627
 };
 

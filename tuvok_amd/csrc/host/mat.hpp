@@ -83,6 +83,26 @@ inline Mat4 inverse(const Mat4& M) {
   return R;
 }
 
+// The clip plane in model space as GLRaycaster::RenderBox forms it
+// (GLRaycaster.cpp:328-336): m_ClipPlane.Plane() * inverse(rotation * translation),
+// PLANE * M being PLANE::transform (Vectors.h:1460-1485) -- the row vector times
+// the transposed inverse of M, divided by the length of its xyz -- then the
+// normal normalised again and d kept.  Points with n.p + d <= 0 stay
+// (Clipper::TriPlane keeps fa <= 0, Clipper.cpp:78-104).
+inline void model_clip_plane(const float plane[4], const Mat4& rot, const Mat4& trans, float out[4]) {
+  const Mat4 inv = inverse(rot * trans);
+  const Mat4 ii = inverse(inv);
+  Mat4 it;
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) it.at(r, c) = ii.at(c, r);
+  float t[4];
+  row_mul(plane, it, t);
+  const float l = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  for (float& v : t) v /= l;
+  const float l2 = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  out[0] = t[0] / l2; out[1] = t[1] / l2; out[2] = t[2] / l2; out[3] = t[3];
+}
+
 inline Mat4 translation(float x, float y, float z) { Mat4 R; R.at(3, 0) = x; R.at(3, 1) = y; R.at(3, 2) = z; return R; }
 inline Mat4 scaling(float x, float y, float z) { Mat4 R; R.at(0, 0) = x; R.at(1, 1) = y; R.at(2, 2) = z; return R; }
 inline Mat4 rotation_y(double angle) {

@@ -193,6 +193,42 @@ __global__ __launch_bounds__(256) void swizzle_bricks(const T* __restrict__ src,
   }
 }
 
+// The LOD-0 volume from the pool (UVFDataset::Export of level 0): voxel (x,y,z)
+// is the core voxel of the brick that owns it.  bricks: the LOD-0 bricks in TOC
+// order (nbx x nby x nbz of them).
+template <typename T>
+__global__ __launch_bounds__(256) void flatten_lod0(const T* __restrict__ pool, const GatherBrick* __restrict__ bricks,
+                                                    uint32_t nbx, uint32_t nby, uint32_t cx, uint32_t cy, uint32_t cz,
+                                                    uint32_t ov, uint64_t X, uint64_t Y, uint64_t Z, T* __restrict__ out) {
+  const int lx = lx_of<T>();
+  const uint64_t n = X * Y * Z;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint32_t x = (uint32_t)(i % X);
+    const uint64_t r = i / X;
+    const uint32_t y = (uint32_t)(r % Y), z = (uint32_t)(r / Y);
+    const uint32_t bx = x / cx, by = y / cy, bz = z / cz;
+    const GatherBrick b = bricks[((uint64_t)bz * nby + by) * nbx + bx];
+    const uint32_t u = x - bx * cx + ov, v = y - by * cy + ov, w = z - bz * cz + ov;
+    out[i] = pool[b.offset + swz(u, v, w, nblocks(b.n[0], lx), nblocks(b.n[1], 2), lx)];
+  }
+}
+
+// UVFDataset::CropData (IO/uvfDataset.h:252-297): voxels whose normalised
+// coordinate (x/X - 0.5, y/Y - 0.5, z/Z - 0.5) satisfies PLANE::clip,
+// (n . p) + w >= 0, are set to 0.
+template <typename T>
+__global__ __launch_bounds__(256) void crop_plane(T* __restrict__ vol, uint64_t X, uint64_t Y, uint64_t Z, float px,
+                                                  float py, float pz, float pw) {
+  const uint64_t n = X * Y * Z;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t x = i % X, r = i / X, y = r % Y, z = r / Y;
+    const float vx = (float)x / (float)X - 0.5f, vy = (float)y / (float)Y - 0.5f, vz = (float)z / (float)Z - 0.5f;
+    // separate roundings, as the reference's float expression (no fused multiply-add)
+    const float d = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(px, vx), __fmul_rn(py, vy)), __fmul_rn(pz, vz)), pw);
+    if (d >= 0.0f) vol[i] = (T)0;
+  }
+}
+
 // 2x box filter with the reference's mean (double accumulation, truncation);
 // an axis of size 1 is not halved (ExtendedOctree.cpp:201-213).
 template <typename T>
@@ -543,6 +579,34 @@ hipError_t launch_swizzle(const void* src, int bpv, const GatherBrick* d_bricks,
       hipLaunchKernelGGL(swizzle_bricks<uint16_t>, dim3((uint32_t)gx, cnt), dim3(256), 0, st, (const uint16_t*)src,
                          d_bricks + first, d_src_off + first, (uint16_t*)pool);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_flatten(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t nb[3],
+                          const uint32_t core[3], uint32_t ov, const uint64_t dims[3], void* out, hipStream_t st) {
+  const uint64_t n = dims[0] * dims[1] * dims[2];
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (bpv == 1)
+    hipLaunchKernelGGL(flatten_lod0<uint8_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint8_t*)pool, d_bricks,
+                       nb[0], nb[1], core[0], core[1], core[2], ov, dims[0], dims[1], dims[2], (uint8_t*)out);
+  else
+    hipLaunchKernelGGL(flatten_lod0<uint16_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint16_t*)pool,
+                       d_bricks, nb[0], nb[1], core[0], core[1], core[2], ov, dims[0], dims[1], dims[2],
+                       (uint16_t*)out);
+  return hipGetLastError();
+}
+
+hipError_t launch_crop(void* vol, int bpv, const uint64_t dims[3], const float pl[4], hipStream_t st) {
+  const uint64_t n = dims[0] * dims[1] * dims[2];
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (bpv == 1)
+    hipLaunchKernelGGL(crop_plane<uint8_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (uint8_t*)vol, dims[0], dims[1],
+                       dims[2], pl[0], pl[1], pl[2], pl[3]);
+  else
+    hipLaunchKernelGGL(crop_plane<uint16_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (uint16_t*)vol, dims[0],
+                       dims[1], dims[2], pl[0], pl[1], pl[2], pl[3]);
   return hipGetLastError();
 }
 

@@ -443,7 +443,8 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
                                            int ty0, int tw, int th, unsigned long long* counter, uint32_t* touched,
                                            const LearnConst* LC, const SegOut* SO, uint16_t* sn_lds,
                                            uint32_t view_id, uint64_t* cols, bool acc = false,
-                                           float* __restrict__ depth_buf = nullptr) {
+                                           float* __restrict__ depth_buf = nullptr,
+                                           float4* __restrict__ hitpos_buf = nullptr) {
   const int lane = threadIdx.x & 63;
   const int px = tx0 + lane % tw, py = ty0 + lane / tw;
   const int W = (int)VC->W, H = (int)VC->H;
@@ -473,6 +474,12 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
     cols[lane] = S.tf2_cols[lane];
     sn_wave_sync();
   }
+  // the view's model-space clip plane (uniform: scalar registers)
+  const bool clip_on = VC->clip_on != 0u;
+  float clip[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    clip[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, VC->clip[i])));
   const float eps = S.ert_eps;
   const bool edge = S.clamp_edge != 0;
 
@@ -527,6 +534,15 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           tn = fmaxf(tn, fminf(t0, t1));
           tf = fminf(tf, fmaxf(t0, t1));
         }
+      }
+      if (clip_on) {
+        // the brick box cut by the clip plane and closed by its cap (Clipper::BoxPlane,
+        // Clipper.cpp:106-131): the ray keeps the part with n . p + d <= 0
+        const float a = clip[0] * d_m.x + clip[1] * d_m.y + clip[2] * d_m.z;
+        const float c = clip[0] * o_m.x + clip[1] * o_m.y + clip[2] * o_m.z + clip[3];
+        if (a > 0.0f) tf = fminf(tf, -c / a);
+        else if (a < 0.0f) tn = fmaxf(tn, -c / a);
+        else if (c > 0.0f) ok = false;
       }
       ok = ok && tf > tn && tf > 0.0f;
       const float te = fmaxf(tn, 1.0f);  // near-plane entry when the front face is clipped
@@ -900,6 +916,10 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
   }
   reinterpret_cast<float4*>(out)[pix] = res;
   if (MODE == 2 && depth_buf) depth_buf[pix] = best_depth;
+  // the iso hit target of GLRaycaster-ISO-FS.glsl:95-96 (eye-space hit, segment
+  // parameter; cleared to 0), read back by GLRenderer::Pick
+  if (MODE == 2 && hitpos_buf)
+    hitpos_buf[pix] = have_hit ? make_float4(hit_pos.x, hit_pos.y, hit_pos.z, hit_param) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Multi-view launch: one 256-thread workgroup = 16x16 pixels = 4 waves of 8x8.
@@ -931,7 +951,7 @@ __device__ __forceinline__ void march_group(const MarchArgs& A) {
   march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
                                          A.counter, (COUNT && A.touched) ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
                                          &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v,
-                                         s_cols + (MODE == 1 ? wave * 64 : 0), A.accumulate != 0, A.depth);
+                                         s_cols + (MODE == 1 ? wave * 64 : 0), A.accumulate != 0, A.depth, A.hitpos);
 }
 
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>

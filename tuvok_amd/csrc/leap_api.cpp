@@ -358,6 +358,7 @@ int tvk_leap_render(tvk_ctx* c, int first, float* host_out, tvk_leap_stats* st) 
   HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
   c->leap_pixels = pixels;
   c->last_views = 1;
+  c->hitpos_valid = false;
   if (host_out)
     HIPCHK(c, hipMemcpyAsync(host_out, c->d_out, pixels * 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   if (st) {

@@ -393,6 +393,7 @@ int render_budgeted(tvk_ctx* c, int nviews, const float* rots, float* host_out, 
   if (!rc) rc = ensure(c, &c->d_views, &c->views_cap, parts.size());
   if (!rc) rc = ensure(c, &c->d_out, &c->out_cap, (size_t)pixels * 4);
   if (!rc && iso) rc = ensure(c, &c->d_depth, &c->depth_cap, (size_t)pixels);
+  if (!rc && iso) rc = ensure(c, &c->d_hitpos, &c->hitpos_cap, (size_t)pixels);
   if (rc) return rc;
   if (c->count) {
     const size_t nt = (size_t)nviews * c->L.toc.size();
@@ -446,6 +447,7 @@ int render_budgeted(tvk_ctx* c, int nviews, const float* rots, float* host_out, 
     a.groups_padded = (vc.groups + 7u) & ~7u;
     a.accumulate = pt.cont ? 1 : 0;
     a.depth = iso ? c->d_depth : nullptr;
+    a.hitpos = iso ? c->d_hitpos : nullptr;
     if (a.groups_total > 0) HIPCHK(c, launch_march(a, c->L.bpv, c->stream, 0));
   }
   HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
@@ -467,6 +469,7 @@ int render_budgeted(tvk_ctx* c, int nviews, const float* rots, float* host_out, 
     st->ms = ms;
   }
   c->last_parts = (uint32_t)parts.size();
+  c->hitpos_valid = iso;
   return TVK_OK;
 }
 

@@ -66,7 +66,9 @@ struct ViewConst {
   uint32_t groups_x;    // 16x16 block groups per row
   uint32_t groups;      // 16x16 block groups of the view
   uint32_t group_offset;// first 16x16 group of this view in the launch
-  uint32_t pad[2];
+  uint32_t clip_on;     // the clip plane cuts every brick's ray segment
+  uint32_t pad;
+  float clip[4];        // model-space clip plane (n, d): the ray keeps n.p + d <= 0
 };
 
 // One brick of a LOD for the bricking / min-max kernels.
@@ -165,6 +167,7 @@ struct MarchArgs {
   uint32_t groups_padded;       // rounded up to a multiple of 8 (XCD count)
   int accumulate;               // continue the pixels' values (and depths) of an earlier launch
   float* depth;                 // isosurface: per-pixel depth of the best hit so far (or nullptr)
+  float4* hitpos;               // isosurface: per-pixel eye-space hit and its segment parameter (or nullptr)
 };
 
 // One request of a batch: its shading constants, view, and place in the batch.

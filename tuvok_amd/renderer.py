@@ -320,6 +320,32 @@ class HIPRaycaster:
         if self._w:
             self._chk(self._L.tvk_set_rotation(self._h, _fp(self._rot)))
 
+    def SetClipPlane(self, plane):
+        """AbstrRenderer::SetClipPlane: ExtendedPlane::Plane() in world space (x, y, z, w)."""
+        self._clip = np.ascontiguousarray(plane, np.float32).reshape(4)
+        self._chk(self._L.tvk_set_clip_plane(self._h, _fp(self._clip), int(getattr(self, "_clip_on", False))))
+
+    def EnableClipPlane(self, on=True):
+        """AbstrRenderer::EnableClipPlane / DisableClipPlane."""
+        self._clip_on = bool(on)
+        self._chk(self._L.tvk_set_clip_plane(self._h, None, int(self._clip_on)))
+
+    def DisableClipPlane(self):
+        self.EnableClipPlane(False)
+
+    def CropDataset(self, keep_old=False):
+        """AbstrRenderer::CropDataset: zero the voxels on the clipped side of the clip
+        plane and brick the volume again; returns the normalised-coordinate plane used."""
+        pl = np.zeros(4, np.float32)
+        self._chk(self._L.tvk_crop_dataset(self._h, int(bool(keep_old)), _fp(pl)))
+        return pl
+
+    def Pick(self, x, y):
+        """GLRenderer::Pick: eye-space isosurface hit at window position (x, y from the top)."""
+        v = np.zeros(3, np.float32)
+        self._chk(self._L.tvk_pick(self._h, int(x), int(y), _fp(v)))
+        return v
+
     def SetUserMatrices(self, view, proj, rot=None, trans=None):
         """AbstrRenderer::SetUserMatrices with explicit row-vector matrices."""
         v = np.ascontiguousarray(view, np.float32)
