@@ -567,6 +567,38 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
       const int nx = (int)fb->n[0], ny = (int)fb->n[1], nz = (int)fb->n[2];
       const V3 vstep = mk(fb->vstep[0], fb->vstep[1], fb->vstep[2]);
       const float sscale = fb->step_scale;
+      // Fixed step (LEARN 0): the samples whose 4x4x4 neighbourhood lies inside the
+      // brick, decided once per segment instead of per sample.  Sample i sits at
+      // u_a = (entryTex_a + i * incTex_a) * n_a - 0.5 up to the drift of the
+      // accumulated position: each addition of a coordinate below 1 rounds by at
+      // most 2^-24, so after i < steps additions the drift is below steps * 2^-24
+      // in texture space, steps * n_a * 2^-24 voxels; the margin is twice that
+      // plus 1/16 voxel.  The interior form needs 1 <= floor(u_a) and
+      // floor(u_a) + 2 < n_a on every axis.  The range only selects between two
+      // forms that give the same values, so a conservative one is enough.
+      int in_lo = 0, in_hi = -1;
+      if (LEARN == 0 && MODE != 2) {
+        float lo = 0.0f, hi = (float)(steps - 1);
+        const float ec[3] = {entryTex.x, entryTex.y, entryTex.z}, ic3[3] = {incTex.x, incTex.y, incTex.z};
+        const int nn3[3] = {nx, ny, nz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float na = (float)nn3[a];
+          const float u0 = ec[a] * na - 0.5f, du = ic3[a] * na;
+          const float margin = 0.0625f + (float)steps * na * 0x1p-23f;
+          const float ulo = 1.0f + margin, uhi = na - 2.0f - margin;   // floor(u) in [1, n - 3]
+          if (!(ulo <= uhi)) {
+            hi = -1.0f;   // no interior on this axis
+          } else if (du == 0.0f) {
+            if (!(u0 >= ulo && u0 <= uhi)) hi = -1.0f;
+          } else {
+            const float t1 = (ulo - u0) / du, t2 = (uhi - u0) / du;
+            lo = fmaxf(lo, fminf(t1, t2));
+            hi = fminf(hi, fmaxf(t1, t2));
+          }
+        }
+        if (hi >= lo) { in_lo = (int)ceilf(lo); in_hi = (int)floorf(hi); }
+      }
 
       if (MODE == 2) {
         // GLRaycaster-ISO-FS.glsl:56-105 + RefineIsosurface.glsl:37-52
@@ -669,7 +701,9 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           } else {
             // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
             bool inner = true;
-            {
+            if (LEARN == 0) {
+              inner = i >= in_lo && i <= in_hi;   // the segment's interior range
+            } else {
               const float pc[3] = {pos.x, pos.y, pos.z};
               const int nn3[3] = {nx, ny, nz};
 #pragma unroll
