@@ -587,8 +587,10 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           const float u0 = ec[a] * na - 0.5f, du = ic3[a] * na;
           const float margin = 0.0625f + (float)steps * na * 0x1p-23f;
           const float ulo = 1.0f + margin, uhi = na - 2.0f - margin;   // floor(u) in [1, n - 3]
-          if (!(ulo <= uhi)) {
-            hi = -1.0f;   // no interior on this axis
+          if (!(ulo <= uhi) || !(fabsf(u0) < 0x1p24f) || !(fabsf(du) < 0x1p24f)) {
+            // no interior on this axis, or a position or increment that is not
+            // finite (a zero-length segment divides by zero): the general form
+            hi = -1.0f;
           } else if (du == 0.0f) {
             if (!(u0 >= ulo && u0 <= uhi)) hi = -1.0f;
           } else {
@@ -687,7 +689,10 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             }
           }
         };
-        for (int i = 0; i < nsamp; ++i) {
+        // one sample at pos (the interior form when fast_c is true): shade,
+        // UnderCompositing (Compositing.glsl:33-38), advance; false once the
+        // segment ends ('a >= 0.99' or early ray termination)
+        auto step = [&](auto fast_c) -> bool {
           ++ns;
           float s0, s1, s2, s3;
           if constexpr (MODE == 0 && !LIT) {
@@ -699,25 +704,9 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             s0 = tv.x; s1 = tv.y; s2 = tv.z;
             s3 = 1.0f - opacity_pow(1.0f - tv.w, ssc);
           } else {
-            // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
-            bool inner = true;
-            if (LEARN == 0) {
-              inner = i >= in_lo && i <= in_hi;   // the segment's interior range
-            } else {
-              const float pc[3] = {pos.x, pos.y, pos.z};
-              const int nn3[3] = {nx, ny, nz};
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
-                const int ia = (int)floorf(u);
-                inner = inner && ia >= 1 && ia + 2 < nn3[a];
-              }
-            }
             pos_s = pos;
-            if (__ballot(!inner) == 0) shade(std::true_type{}, s0, s1, s2, s3);
-            else shade(std::false_type{}, s0, s1, s2, s3);
+            shade(fast_c, s0, s1, s2, s3);
           }
-          // UnderCompositing (Compositing.glsl:33-38)
           const float wgt = (1.0f - c3) * s3;
           c0 += s0 * wgt;
           c1 += s1 * wgt;
@@ -729,6 +718,43 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           if (c3 >= 0.99f) return false;
           if (eps > 0.0f && tray * (1.0f - c3) < eps) return false;
           pos = mk(fmaf(incTex.x, mlt, pos.x), fmaf(incTex.y, mlt, pos.y), fmaf(incTex.z, mlt, pos.z));
+          return true;
+        };
+        if constexpr (LEARN == 0 && !(MODE == 0 && !LIT)) {
+          // Fixed step: each lane marches its samples before the segment's
+          // interior range in the general form, the interior range in the
+          // interior form, and the rest in the general form again, so that
+          // only the few samples near a brick face pay for clamped taps
+          // whatever step the other lanes of the wave are at.  Every lane
+          // executing the interior loop is inside its range.
+          const int h_end = in_hi >= in_lo ? in_lo : nsamp;
+          int i = 0;
+          for (; i < h_end; ++i)
+            if (!step(std::false_type{})) return false;
+          for (; i <= in_hi; ++i)
+            if (!step(std::true_type{})) return false;
+          for (; i < nsamp; ++i)
+            if (!step(std::false_type{})) return false;
+        } else {
+          for (int i = 0; i < nsamp; ++i) {
+            bool ok;
+            if constexpr (MODE == 0 && !LIT) {
+              ok = step(std::false_type{});
+            } else {
+              // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
+              bool inner = true;
+              const float pc[3] = {pos.x, pos.y, pos.z};
+              const int nn3[3] = {nx, ny, nz};
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
+                const int ia = (int)floorf(u);
+                inner = inner && ia >= 1 && ia + 2 < nn3[a];
+              }
+              ok = __ballot(!inner) == 0 ? step(std::true_type{}) : step(std::false_type{});
+            }
+            if (!ok) return false;
+          }
         }
         return true;
       };
