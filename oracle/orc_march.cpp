@@ -80,11 +80,53 @@ struct BrickTex {
     float c0 = lerp(c00, c10, f[1]), c1 = lerp(c01, c11, f[1]);
     return lerp(c0, c1, f[2]) * inv_max;
   }
-  // Volume3D.glsl:43-53
+  // trilinear filter of the footprint with lower corner i0 and weights f
+  // (sample()'s lerps and scale)
+  float filter(const int i0[3], const float f[3]) const {
+    float c000 = texel(i0[0], i0[1], i0[2]), c100 = texel(i0[0] + 1, i0[1], i0[2]);
+    float c010 = texel(i0[0], i0[1] + 1, i0[2]), c110 = texel(i0[0] + 1, i0[1] + 1, i0[2]);
+    float c001 = texel(i0[0], i0[1], i0[2] + 1), c101 = texel(i0[0] + 1, i0[1], i0[2] + 1);
+    float c011 = texel(i0[0], i0[1] + 1, i0[2] + 1), c111 = texel(i0[0] + 1, i0[1] + 1, i0[2] + 1);
+    auto lerp = [](float a, float b, float t) { return std::fma(t, b - a, a); };
+    float c00 = lerp(c000, c100, f[0]), c10 = lerp(c010, c110, f[0]);
+    float c01 = lerp(c001, c101, f[0]), c11 = lerp(c011, c111, f[0]);
+    float c0 = lerp(c00, c10, f[1]), c1 = lerp(c01, c11, f[1]);
+    return lerp(c0, c1, f[2]) * inv_max;
+  }
+  // Volume3D.glsl:43-53: six taps at c +- h, h = vVoxelStepsize = one texel
+  // (1 / texture size, GLRaycaster.cpp:242-251).  GL_LINEAR at s + 1/n is the
+  // footprint of s moved by one texel with the same weights ((s + 1/n) n - 1/2
+  // = u + 1), so on an axis where both taps stay inside [0, 1] (no coordinate
+  // clamp) the taps are taken in that form: the centre's lower corner +-1 on
+  // that axis and the centre's weights.  On an axis where a tap leaves [0, 1]
+  // the clamped coordinate moves the footprint differently, and both of that
+  // axis's taps are sample(c +- h) as the shader states them.
   V3 gradient(const V3& c, const V3& h) const {
-    float xp = sample(c + V3(+h.x, 0, 0)), xm = sample(c + V3(-h.x, 0, 0));
-    float yp = sample(c + V3(0, -h.y, 0)), ym = sample(c + V3(0, +h.y, 0));
-    float zp = sample(c + V3(0, 0, +h.z)), zm = sample(c + V3(0, 0, -h.z));
+    int i0[3];
+    float f[3];
+    for (int a = 0; a < 3; ++a) {
+      float s = std::min(std::max(c[a], 0.0f), 1.0f);
+      float u = s * (float)n[a] - 0.5f, fl = std::floor(u);
+      i0[a] = (int)fl;
+      f[a] = u - fl;
+    }
+    float plus[3], minus[3];   // taps at c + h e_a and c - h e_a
+    for (int a = 0; a < 3; ++a) {
+      if (c[a] - h[a] >= 0.0f && c[a] + h[a] <= 1.0f) {
+        int ip[3] = {i0[0], i0[1], i0[2]}, im[3] = {i0[0], i0[1], i0[2]};
+        ip[a] += 1;
+        im[a] -= 1;
+        plus[a] = filter(ip, f);
+        minus[a] = filter(im, f);
+      } else {
+        V3 e(0, 0, 0);
+        e[a] = h[a];
+        plus[a] = sample(c + e);
+        minus[a] = sample(c - e);
+      }
+    }
+    // the shader's names: xp = +x, xm = -x, yp = -y, ym = +y, zp = +z, zm = -z
+    const float xp = plus[0], xm = minus[0], yp = minus[1], ym = plus[1], zp = plus[2], zm = minus[2];
     return V3(xm - xp, yp - ym, zm - zp) / 2.0f;
   }
 };

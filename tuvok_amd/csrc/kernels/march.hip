@@ -42,25 +42,16 @@ __device__ __forceinline__ float tri_lerp(float c000, float c100, float c010, fl
   return lerp_fma(c0, c1, f2);
 }
 
-// texture3D with GL_LINEAR and GL_CLAMP (border colour 0) or GL_CLAMP_TO_EDGE
+// GL_LINEAR filter of the footprint with lower corner i0 and weights f, with
+// GL_CLAMP (border colour 0) or GL_CLAMP_TO_EDGE
 template <typename T>
-__device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p) {
+__device__ __forceinline__ float filter_brick(const T* __restrict__ d, int nx, int ny, int nz, bool edge,
+                                              const int i0[3], const float f[3]) {
   const float inv = Unorm<T>::inv();
-  int i0[3];
-  float f[3];
-  int n[3] = {nx, ny, nz};
-  float pc[3] = {p.x, p.y, p.z};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float s = fminf(fmaxf(pc[a], 0.0f), 1.0f);
-    float u = s * (float)n[a] - 0.5f;
-    float fl = floorf(u);
-    i0[a] = (int)fl;
-    f[a] = u - fl;
-  }
   // per-axis tap indices (clamped for the address) and validity (border)
   int xa = i0[0], xb = i0[0] + 1, ya = i0[1], yb = i0[1] + 1, za = i0[2], zb = i0[2] + 1;
-  bool vxa = xa >= 0, vxb = xb < nx, vya = ya >= 0, vyb = yb < ny, vza = za >= 0, vzb = zb < nz;
+  bool vxa = xa >= 0 && xa < nx, vxb = xb >= 0 && xb < nx, vya = ya >= 0 && ya < ny;
+  bool vyb = yb >= 0 && yb < ny, vza = za >= 0 && za < nz, vzb = zb >= 0 && zb < nz;
   if (edge) { vxa = vxb = vya = vyb = vza = vzb = true; }
   xa = min(max(xa, 0), nx - 1); xb = min(max(xb, 0), nx - 1); ya = min(max(ya, 0), ny - 1);
   yb = min(max(yb, 0), ny - 1); za = min(max(za, 0), nz - 1); zb = min(max(zb, 0), nz - 1);
@@ -82,25 +73,68 @@ __device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, i
   return tri_lerp(c000, c100, c010, c110, c001, c101, c011, c111, f[0], f[1], f[2]) * inv;
 }
 
-// Volume3D.glsl:43-53 (6-tap central difference at +-1 brick voxel, halved)
-template <typename T>
-__device__ __forceinline__ V3 gradient(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 c, V3 h) {
-  float xp = sample_brick<T>(d, nx, ny, nz, edge, c + mk(+h.x, 0, 0));
-  float xm = sample_brick<T>(d, nx, ny, nz, edge, c + mk(-h.x, 0, 0));
-  float yp = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, -h.y, 0));
-  float ym = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, +h.y, 0));
-  float zp = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, 0, +h.z));
-  float zm = sample_brick<T>(d, nx, ny, nz, edge, c + mk(0, 0, -h.z));
-  return mk(xm - xp, yp - ym, zm - zp) / 2.0f;
+// lower corner and weights of the GL_LINEAR footprint at texture coordinate p
+__device__ __forceinline__ void footprint(V3 p, int nx, int ny, int nz, int i0[3], float f[3]) {
+  const int n[3] = {nx, ny, nz};
+  const float pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float s = fminf(fmaxf(pc[a], 0.0f), 1.0f);
+    const float u = s * (float)n[a] - 0.5f;
+    const float fl = floorf(u);
+    i0[a] = (int)fl;
+    f[a] = u - fl;
+  }
 }
 
-// Centre sample plus Volume3D.glsl gradient from one 32-voxel cross.
-// Every tap keeps its own clamped coordinate, index and weight exactly as
-// sample_brick computes them; the voxels are shared when each tap's index is
-// the centre's +-1 (always, except where a coordinate clamps at the texture
-// edge), otherwise the generic six-sample gradient runs.  value() first loads
-// the centre cube only, so samples whose TF entry is fully transparent never
-// pay for the gradient.
+// texture3D with GL_LINEAR and GL_CLAMP (border colour 0) or GL_CLAMP_TO_EDGE
+template <typename T>
+__device__ __forceinline__ float sample_brick(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p) {
+  int i0[3];
+  float f[3];
+  footprint(p, nx, ny, nz, i0, f);
+  return filter_brick<T>(d, nx, ny, nz, edge, i0, f);
+}
+
+// Volume3D.glsl:43-53: 6-tap central difference at c +- h, h = one texel
+// (1 / texture size, GLRaycaster.cpp:242-251), halved.  GL_LINEAR at s + 1/n is
+// the footprint of s moved by one texel with the same weights, so on an axis
+// where both taps stay inside [0, 1] the taps are the centre's footprint +-1
+// texel on that axis with the centre's weights; on an axis where a tap leaves
+// [0, 1] (its coordinate clamps) both taps are sampled at c +- h as stated.
+__device__ __forceinline__ bool taps_unclamped(float c, float h) { return c - h >= 0.0f && c + h <= 1.0f; }
+
+template <typename T>
+__device__ __forceinline__ V3 gradient(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 c, V3 h) {
+  int i0[3];
+  float f[3];
+  footprint(c, nx, ny, nz, i0, f);
+  const float cc[3] = {c.x, c.y, c.z}, hh[3] = {h.x, h.y, h.z};
+  float plus[3], minus[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (taps_unclamped(cc[a], hh[a])) {
+      int ip[3] = {i0[0], i0[1], i0[2]}, im[3] = {i0[0], i0[1], i0[2]};
+      ip[a] += 1;
+      im[a] -= 1;
+      plus[a] = filter_brick<T>(d, nx, ny, nz, edge, ip, f);
+      minus[a] = filter_brick<T>(d, nx, ny, nz, edge, im, f);
+    } else {
+      const V3 e = a == 0 ? mk(hh[0], 0, 0) : (a == 1 ? mk(0, hh[1], 0) : mk(0, 0, hh[2]));
+      plus[a] = sample_brick<T>(d, nx, ny, nz, edge, c + e);
+      minus[a] = sample_brick<T>(d, nx, ny, nz, edge, c - e);
+    }
+  }
+  // the shader's taps: xp = +x, xm = -x, yp = -y, ym = +y, zp = +z, zm = -z
+  return mk(minus[0] - plus[0], minus[1] - plus[1], minus[2] - plus[2]) / 2.0f;
+}
+
+// Centre sample plus gradient()'s six taps from one 32-voxel cross: the
+// centre footprint and, per axis, the footprints one texel below and above it
+// with the centre's weights (the taps of an axis whose coordinates clamp at the
+// texture edge are resampled at p +- h, as gradient() does).  value() first
+// loads the centre cube only, so samples whose TF entry is fully transparent
+// never pay for the gradient.
 // sample_brick's interpolation of stored integers, scaled to unorm once
 template <typename T>
 __device__ __forceinline__ float tri(float c000, float c100, float c010, float c110, float c001, float c101, float c011,
@@ -171,34 +205,10 @@ struct CrossSampler {
   }
 
   __device__ __forceinline__ V3 gradient(V3 h) const {
-    // per axis: minus tap (p - h) and plus tap (p + h); Volume3D.glsl's y "p" tap is at -h
-    const float pm[3] = {p.x + -h.x, p.y + -h.y, p.z + -h.z};
-    const float pp[3] = {p.x + h.x, p.y + h.y, p.z + h.z};
-    int im[3], iq[3];
-    float fm[3], fq[3];
-    bool shared = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      float s = fminf(fmaxf(pm[a], 0.0f), 1.0f), u = s * (float)n[a] - 0.5f, fl = floorf(u);
-      im[a] = (int)fl; fm[a] = u - fl;
-      s = fminf(fmaxf(pp[a], 0.0f), 1.0f); u = s * (float)n[a] - 0.5f; fl = floorf(u);
-      iq[a] = (int)fl; fq[a] = u - fl;
-      shared = shared && im[a] == ic[a] - 1 && iq[a] == ic[a] + 1;
-    }
-    if (!shared) {
-      // rare (texture-edge clamping): the six taps one at a time, as Volume3D.glsl
-      float t[6];
-#pragma unroll 1
-      for (int k = 0; k < 6; ++k) {
-        const int a = k >> 1;
-        const float off = (k & 1) ? (a == 1 ? h.y : (a == 0 ? -h.x : -h.z)) : (a == 1 ? -h.y : (a == 0 ? h.x : h.z));
-        V3 q = p;
-        if (a == 0) q = p + mk(off, 0, 0); else if (a == 1) q = p + mk(0, off, 0); else q = p + mk(0, 0, off);
-        t[k] = sample_brick<T>(d, n[0], n[1], n[2], edge, q);
-      }
-      // t = {xp, xm, yp, ym, zp, zm} in Volume3D.glsl's naming
-      return mk(t[1] - t[0], t[2] - t[3], t[5] - t[4]) / 2.0f;
-    }
+    // the index form of all six taps from the cross (gradient()'s rule); an
+    // axis whose taps clamp (rare: a sample next to the texture edge) is
+    // resampled at p +- h below
+    const float pc[3] = {p.x, p.y, p.z}, hc[3] = {h.x, h.y, h.z};
     float xl[2][2], xr[2][2], yl[2][2], yr[2][2], zl[2][2], zr[2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j)
@@ -208,18 +218,28 @@ struct CrossSampler {
         yl[j][k] = V(1 + j, 0, 1 + k); yr[j][k] = V(1 + j, 3, 1 + k);
         zl[j][k] = V(1 + j, 1 + k, 0); zr[j][k] = V(1 + j, 1 + k, 3);
       }
-    const float fxm = tri<T>(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
-                          fm[0], fc[1], fc[2]);
-    const float fxp = tri<T>(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
-                          fq[0], fc[1], fc[2]);
-    const float fym = tri<T>(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
-                          fc[0], fm[1], fc[2]);
-    const float fyp = tri<T>(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
-                          fc[0], fq[1], fc[2]);
-    const float fzm = tri<T>(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
-                          fc[0], fc[1], fm[2]);
-    const float fzp = tri<T>(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
-                          fc[0], fc[1], fq[2]);
+    float fxm = tri<T>(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
+                          fc[0], fc[1], fc[2]);
+    float fxp = tri<T>(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
+                          fc[0], fc[1], fc[2]);
+    float fym = tri<T>(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
+                          fc[0], fc[1], fc[2]);
+    float fyp = tri<T>(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
+                          fc[0], fc[1], fc[2]);
+    float fzm = tri<T>(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
+                          fc[0], fc[1], fc[2]);
+    float fzp = tri<T>(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
+                          fc[0], fc[1], fc[2]);
+    if (!(taps_unclamped(pc[0], hc[0]) && taps_unclamped(pc[1], hc[1]) && taps_unclamped(pc[2], hc[2]))) {
+#pragma unroll 1
+      for (int a = 0; a < 3; ++a) {
+        if (taps_unclamped(pc[a], hc[a])) continue;
+        const V3 e = a == 0 ? mk(hc[0], 0, 0) : (a == 1 ? mk(0, hc[1], 0) : mk(0, 0, hc[2]));
+        const float tp = sample_brick<T>(d, n[0], n[1], n[2], edge, p + e);
+        const float tm = sample_brick<T>(d, n[0], n[1], n[2], edge, p - e);
+        if (a == 0) { fxp = tp; fxm = tm; } else if (a == 1) { fyp = tp; fym = tm; } else { fzp = tp; fzm = tm; }
+      }
+    }
     return mk(fxm - fxp, fym - fyp, fzm - fzp) / 2.0f;
   }
 };
@@ -314,33 +334,9 @@ struct CrossSampler<T, true> {
                   fc[0], fc[1], fc[2]);
   }
 
-  __device__ __forceinline__ V3 gradient(V3 h) const {
-    const float pm[3] = {p.x + -h.x, p.y + -h.y, p.z + -h.z};
-    const float pp[3] = {p.x + h.x, p.y + h.y, p.z + h.z};
-    int im[3], iq[3];
-    float fm[3], fq[3];
-    bool shared = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      float s = fminf(fmaxf(pm[a], 0.0f), 1.0f), u = s * (float)n[a] - 0.5f, fl = floorf(u);
-      im[a] = (int)fl; fm[a] = u - fl;
-      s = fminf(fmaxf(pp[a], 0.0f), 1.0f); u = s * (float)n[a] - 0.5f; fl = floorf(u);
-      iq[a] = (int)fl; fq[a] = u - fl;
-      shared = shared && im[a] == ic[a] - 1 && iq[a] == ic[a] + 1;
-    }
-    if (!shared) {
-      // rare (a coordinate rounds across a voxel boundary): the six taps one at a time
-      float t[6];
-#pragma unroll 1
-      for (int k = 0; k < 6; ++k) {
-        const int a = k >> 1;
-        const float off = (k & 1) ? (a == 1 ? h.y : (a == 0 ? -h.x : -h.z)) : (a == 1 ? -h.y : (a == 0 ? h.x : h.z));
-        V3 q = p;
-        if (a == 0) q = p + mk(off, 0, 0); else if (a == 1) q = p + mk(0, off, 0); else q = p + mk(0, 0, off);
-        t[k] = sample_brick<T>(d, n[0], n[1], n[2], edge, q);
-      }
-      return mk(t[1] - t[0], t[2] - t[3], t[5] - t[4]) / 2.0f;
-    }
+  // every tap of an interior sample is unclamped (1 <= floor(u) and
+  // floor(u) + 2 < n on every axis put p +- 1/n inside [0, 1]): the index form
+  __device__ __forceinline__ V3 gradient(V3) const {
     float yl[2][2], yr[2][2], zl[2][2], zr[2][2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -359,17 +355,17 @@ struct CrossSampler<T, true> {
       zr[0][j] = (float)b[0]; zr[1][j] = (float)b[1];
     }
     const float fxm = tri<T>(xl[0][0], c[0][0][0], xl[1][0], c[0][1][0], xl[0][1], c[0][0][1], xl[1][1], c[0][1][1],
-                             fm[0], fc[1], fc[2]);
+                             fc[0], fc[1], fc[2]);
     const float fxp = tri<T>(c[1][0][0], xr[0][0], c[1][1][0], xr[1][0], c[1][0][1], xr[0][1], c[1][1][1], xr[1][1],
-                             fq[0], fc[1], fc[2]);
+                             fc[0], fc[1], fc[2]);
     const float fym = tri<T>(yl[0][0], yl[1][0], c[0][0][0], c[1][0][0], yl[0][1], yl[1][1], c[0][0][1], c[1][0][1],
-                             fc[0], fm[1], fc[2]);
+                             fc[0], fc[1], fc[2]);
     const float fyp = tri<T>(c[0][1][0], c[1][1][0], yr[0][0], yr[1][0], c[0][1][1], c[1][1][1], yr[0][1], yr[1][1],
-                             fc[0], fq[1], fc[2]);
+                             fc[0], fc[1], fc[2]);
     const float fzm = tri<T>(zl[0][0], zl[1][0], zl[0][1], zl[1][1], c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0],
-                             fc[0], fc[1], fm[2]);
+                             fc[0], fc[1], fc[2]);
     const float fzp = tri<T>(c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1], zr[0][0], zr[1][0], zr[0][1], zr[1][1],
-                             fc[0], fc[1], fq[2]);
+                             fc[0], fc[1], fc[2]);
     return mk(fxm - fxp, fym - fyp, fzm - fzp) / 2.0f;
   }
 };
