@@ -153,6 +153,24 @@ int tvk_download_volume(tvk_ctx* ctx, void* host_out, size_t bytes);
  * file is written (keep_old has no effect).  plane_used (may be NULL) receives the
  * normalised-coordinate plane that was applied. */
 int tvk_crop_dataset(tvk_ctx* ctx, int keep_old, float plane_used[4]);
+/* The renderer's compatibility flags, AbstrRenderer's constructor arguments
+ * bUseOnlyPowerOfTwo and bDownSampleTo8Bits (AbstrRenderer.h:160-174;
+ * bDisableBorder is the clamp_to_edge parameter).  Rendering then reads
+ * textures made from the pool's bricks as GLVolumeListElem::CreateTexture makes
+ * them (GPUMemManDataStructs.cpp:292-360, :396-425): 16-bit bricks quantized
+ * to (unsigned char)((255 v - fMin) / (fMax - fMin)) over the dataset range
+ * (x86's conversion of out-of-range values: truncation to 32 bits, low byte),
+ * and/or bricks padded to the next power of two per axis (one repeated element,
+ * row and slice past the data unless the border is disabled, zeros beyond),
+ * with the power-of-two texture coordinates, voxel step and ray step
+ * (UVFDataset::GetTextCoords, uvfDataset.cpp:1747-1774; GLRaycaster.cpp:242-256),
+ * a TF scale of 1 and the 8-bit isovalue normalisation (GLRenderer.cpp:1813-1819,
+ * AbstrRenderer.cpp:412-438).  The textures are made (in HBM, beside the
+ * pool) on the next render after the dataset or the flags change.  GridLeaper,
+ * a pool budget and the step proposal read the pool's own bricks and return
+ * TVK_ERR_UNSUPPORTED while a flag is set; the allocation-free render returns
+ * TVK_ERR_STATE until a render has made the textures. */
+int tvk_set_compat(tvk_ctx* ctx, int use_only_power_of_two, int downsample_to_8bits);
 int tvk_dataset_info(tvk_ctx* ctx, tvk_dataset_desc* out);
 int tvk_lod_info(tvk_ctx* ctx, int lod, uint64_t size[3], uint64_t bricks[3], uint64_t* offset,
                  double aspect[3]);

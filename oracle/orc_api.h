@@ -90,6 +90,8 @@ typedef struct {
                            total transmittance drops below ert_eps           */
   int   clip_on;        /* m_bClipPlaneOn                                      */
   float clip[4];        /* m_ClipPlane.Plane() (world space: x, y, z, w)      */
+  int   pow2;           /* bUseOnlyPowerOfTwo: textures padded to powers of two */
+  int   down8;          /* bDownSampleTo8Bits: 16-bit bricks quantized to 8 bits */
 } orc_params;
 
 typedef struct {

@@ -34,6 +34,12 @@ struct V3 {
   float minVal() const { return std::min(x, std::min(y, z)); }
 };
 inline float dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// MathTools::NextPow2 (MathTools.h:71; returns n itself when it is a power of two)
+inline uint32_t next_pow2(uint32_t n) {
+  uint32_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
 
 // GLSL pow(y, s) of the opacity correction 1 - (1 - a)^s (VRender1D.glsl:47-57,
 // OpacityCorrectColor of GLGridLeaper-blend.glsl; the built-in's precision is

@@ -24,6 +24,8 @@
 //     compose (GLRaycaster-ISO-FS.glsl:56-105, Compose-FS.glsl).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <limits>
 #include <vector>
 #include "orc_internal.hpp"
 #ifdef _OPENMP
@@ -181,6 +183,58 @@ V3 iso_lighting(const V3& pos, V3 n, const V3& amb, const V3& diff, const V3& sp
   return V3(clamp01(c.x), clamp01(c.y), clamp01(c.z));
 }
 
+// (unsigned char)((255.0 * v - fMin) / (fMax - fMin)), GLVolumeListElem::CreateTexture
+// (GPUMemManDataStructs.cpp:410-425).  C++ leaves the conversion of an
+// out-of-range double undefined; x86 compilers truncate to a 32-bit integer
+// (0x80000000 when it does not fit or is NaN) and keep the low byte.
+inline uint8_t quantize8(unsigned v, double fmin, double fmax) {
+  const double q = (255.0 * (double)v - fmin) / (fmax - fmin);
+  if (!(q > -2147483649.0 && q < 2147483648.0)) return 0;
+  return (uint8_t)(uint32_t)(int32_t)q;
+}
+
+// The texture GLVolumeListElem::CreateTexture makes of one brick (reference
+// layout, x fastest): 16-bit values quantized to 8 bits (down8), then padded to
+// the next power of two per axis (pow2) by PadData (GPUMemManDataStructs.cpp:
+// 306-360): rows copied, the element after each row, the row after each slice
+// and the slice after the last repeated unless the border is disabled, the rest
+// zero.  n_out: the texture size.
+std::vector<uint8_t> compat_texture(const uint8_t* src, const uint32_t size[3], int bpv, bool down8, bool pow2,
+                                    bool disable_border, double fmin, double fmax, int n_out[3]) {
+  const size_t nv = (size_t)size[0] * size[1] * size[2];
+  std::vector<uint8_t> data;
+  int ob = bpv;
+  if (down8) {
+    data.resize(nv);
+    const uint16_t* s16 = reinterpret_cast<const uint16_t*>(src);
+    for (size_t i = 0; i < nv; ++i) data[i] = quantize8(s16[i], fmin, fmax);
+    ob = 1;
+  } else {
+    data.assign(src, src + nv * (size_t)bpv);
+  }
+  for (int k = 0; k < 3; ++k) n_out[k] = (int)size[k];
+  if (!pow2) return data;
+  const size_t P[3] = {next_pow2(size[0]), next_pow2(size[1]), next_pow2(size[2])};
+  const size_t el = (size_t)ob, row_src = size[0] * el, row_dst = P[0] * el;
+  std::vector<uint8_t> out(row_dst * P[1] * P[2], 0);
+  size_t t = 0, s = 0;
+  for (size_t z = 0; z < size[2]; ++z) {
+    for (size_t y = 0; y < size[1]; ++y) {
+      std::memcpy(&out[t], &data[s], row_src);
+      if (!disable_border && row_dst > row_src) std::memcpy(&out[t + row_src], &out[t + row_src - el], el);
+      t += row_dst;
+      s += row_src;
+    }
+    if (P[1] > size[1]) {
+      if (!disable_border) std::memcpy(&out[t], &out[t - row_dst], row_dst);
+      t += (P[1] - size[1]) * row_dst;
+    }
+  }
+  if (!disable_border && P[2] > size[2]) std::memcpy(&out[t], &out[t - P[1] * row_dst], P[1] * row_dst);
+  for (int k = 0; k < 3; ++k) n_out[k] = (int)P[k];
+  return out;
+}
+
 struct Entry {
   BrickTex tex;
   V3 bmin, bmax;      // model-space box of the brick
@@ -202,6 +256,8 @@ struct Frame {
   V3 amb, dif, spc, ldir, dscale, iso_col;
   bool clip_on = false;
   float clip[4] = {0, 0, 0, 0};   // model space, GLRaycaster::RenderBox
+  float iso_n = 0.0f;             // AbstrRenderer::GetNormalizedIsovalue
+  std::vector<std::vector<uint8_t>> compat_tex;   // the compatibility textures in use
 
   bool setup(const orc_ds* d, const orc_view* v, const orc_params* prm) {
     ds = d; p = prm;
@@ -239,6 +295,20 @@ struct Frame {
     if (n < 0) { list.resize((size_t)(-n)); n = orc_plan(d, v, prm, list.data(), (int)list.size()); }
     list.resize((size_t)n);
     const Octree& t = d->tree;
+    // dataset range (UVFDataset::GetRange over the LOD-0 bricks)
+    double fmin = std::numeric_limits<double>::max(), fmax = -std::numeric_limits<double>::max();
+    for (const auto& te : t.toc)
+      if (te.coords[3] == 0) { fmin = std::min(fmin, te.minmax[0]); fmax = std::max(fmax, te.minmax[1]); }
+    const bool down8 = prm->down8 != 0 && t.bpv == 2;
+    if (down8) {
+      // MathTools::lerp(m_fIsovalue, 0.f, float(mx), 0.f, 1.f), AbstrRenderer.cpp:412-424
+      const double mx = fmin > fmax ? (double)prm->tf1d_n : fmax;
+      const float fmx = (float)mx;
+      iso_n = (float)(0.0f + (double)(prm->isovalue - 0.0f) * (1.0 / (double)(fmx - 0.0f)));
+    } else {
+      iso_n = (float)((double)prm->isovalue / (double)(1 << prm->bit_width));
+    }
+    compat_tex.reserve(list.size());
     for (const auto& b : list) {
       Entry e;
       e.empty = b.is_empty != 0;
@@ -246,7 +316,14 @@ struct Frame {
       e.tex.data = t.data.data() + te.offset;
       e.tex.n[0] = (int)te.size[0]; e.tex.n[1] = (int)te.size[1]; e.tex.n[2] = (int)te.size[2];
       e.tex.bpv = (int)t.bpv;
-      e.tex.inv_max = t.bpv == 1 ? 1.0f / 255.0f : 1.0f / 65535.0f;
+      if ((prm->pow2 || down8) && !e.empty) {
+        const uint32_t sz[3] = {(uint32_t)te.size[0], (uint32_t)te.size[1], (uint32_t)te.size[2]};
+        compat_tex.push_back(compat_texture(e.tex.data, sz, (int)t.bpv, down8, prm->pow2 != 0,
+                                            prm->clamp_to_edge != 0, fmin, fmax, e.tex.n));
+        e.tex.data = compat_tex.back().data();
+        if (down8) e.tex.bpv = 1;
+      }
+      e.tex.inv_max = e.tex.bpv == 1 ? 1.0f / 255.0f : 1.0f / 65535.0f;
       e.tex.clamp_edge = prm->clamp_to_edge != 0;
       V3 c(b.center[0], b.center[1], b.center[2]), x(b.extension[0], b.extension[1], b.extension[2]);
       // GLRaycaster::RenderBox (GLRaycaster.cpp:302-345)
@@ -338,7 +415,7 @@ struct Frame {
       V3 pos = entryTex;
       if (p->mode == 2) {
         // GLRaycaster-ISO-FS.glsl:56-105
-        float iso = (float)((double)p->isovalue / (double)(1 << p->bit_width));
+        const float iso = iso_n;
         bool found = false;
         for (int i = 0; i < steps; ++i) {
           ++ns;

@@ -608,15 +608,6 @@ uint64_t orc_ds_table_order(const orc_ds* ds, uint64_t* out, uint64_t cap) {
 
 }  // extern "C"
 
-namespace orc {
-static uint32_t next_pow2(uint32_t n) {
-  // MathTools::NextPow2 (returns n itself when it is a power of two)
-  uint32_t p = 1;
-  while (p < n) p <<= 1;
-  return p;
-}
-}  // namespace orc
-
 void orc_ds::build_table() {
   // uvfDataset.cpp:242-288.  Insertion order (x outer, y, z inner) is kept so
   // that an unordered_map with the same hash sees the same history.

@@ -180,7 +180,7 @@ extern "C" int orc_plan(const orc_ds* ds, const orc_view* view, const orc_params
     b.is_empty = contains ? 0 : 1;
     if (!b.is_empty) {
       V3 a, z;
-      ds->texcoords(prm->lod, b.key_index, false, a, z);
+      ds->texcoords(prm->lod, b.key_index, prm->pow2 != 0, a, z);
       for (int i = 0; i < 3; ++i) { b.tex_min[i] = a[i]; b.tex_max[i] = z[i]; }
       b.distance = brick_distance(cen, ext, mv);
     }
@@ -235,8 +235,12 @@ extern "C" int orc_min_lod(const orc_ds* ds, const orc_view* view, float sse) {
 
 extern "C" void orc_brick_shader_vars(const orc_ds* ds, const orc_params* prm, const orc_brick* b,
                                       float* ray_step, float* step_scale, float vstep[3]) {
-  // GLRaycaster.cpp:239-257 (no power-of-two padding, no sample-rate drop)
-  V3 vox = V3(1, 1, 1) / V3((float)b->voxel_count[0], (float)b->voxel_count[1], (float)b->voxel_count[2]);
+  // GLRaycaster.cpp:239-257 (no sample-rate drop): one texel of the brick's
+  // texture, padded to powers of two with bUseOnlyPowerOfTwo
+  uint32_t tn[3] = {b->voxel_count[0], b->voxel_count[1], b->voxel_count[2]};
+  if (prm->pow2)
+    for (int i = 0; i < 3; ++i) tn[i] = next_pow2(tn[i]);
+  V3 vox = V3(1, 1, 1) / V3((float)tn[0], (float)tn[1], (float)tn[2]);
   float srm = prm->srm;
   V3 ext(b->extension[0], b->extension[1], b->extension[2]);
   *ray_step = (ext * vox * 0.5f * (1.0f / srm)).minVal();

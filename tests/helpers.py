@@ -29,8 +29,11 @@ def oracle_camera_from(ren, rot=None):
 
 
 def product_tf_scale(ren):
-    """fTransScale as GLRenderer::CalculateScaling computes it: (2^bits-1)/range max."""
+    """fTransScale as GLRenderer::CalculateScaling computes it: (2^bits-1)/range max,
+    1 for 16-bit data downsampled to 8-bit textures."""
     d = ren.GetDatasetDesc()
+    if getattr(ren, "_down8", False) and d.bit_width != 8:
+        return 1.0
     return float(np.float32(((1 << d.bit_width) - 1) / d.range_max))
 
 
@@ -46,7 +49,8 @@ def oracle_params_from(ren, tf1d=None, tf2d=None, bit_width=None, tf_scale=None)
                 light_diffuse=tuple(p.light_diffuse), light_specular=tuple(p.light_specular),
                 light_dir=tuple(p.light_dir), iso_color=tuple(p.iso_color), refine_steps=p.refine_steps,
                 clamp_to_edge=p.clamp_to_edge, ert_eps=p.ert_eps,
-                clip=getattr(ren, "_clip", None) if getattr(ren, "_clip_on", False) else None)
+                clip=getattr(ren, "_clip", None) if getattr(ren, "_clip_on", False) else None,
+                pow2=getattr(ren, "_pow2", False), down8=getattr(ren, "_down8", False))
 
 
 def err_stats(a, b):

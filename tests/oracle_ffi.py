@@ -32,7 +32,7 @@ class OrcParams(C.Structure):
                 ("light_specular", C.c_float * 3), ("light_dir", C.c_float * 3),
                 ("iso_color", C.c_float * 3), ("refine_steps", C.c_int),
                 ("clamp_to_edge", C.c_int), ("ert_eps", C.c_float), ("clip_on", C.c_int),
-                ("clip", C.c_float * 4)]
+                ("clip", C.c_float * 4), ("pow2", C.c_int), ("down8", C.c_int)]
 
 
 class OrcBrick(C.Structure):
@@ -299,6 +299,8 @@ def make_params(p, keep):
     if p.get("clip") is not None:
         o.clip_on = 1
         o.clip[:] = [float(v) for v in p["clip"]]
+    o.pow2 = int(bool(p.get("pow2", False)))
+    o.down8 = int(bool(p.get("down8", False)))
     return o
 
 

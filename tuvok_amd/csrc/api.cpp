@@ -24,6 +24,7 @@ void free_dataset(tvk_ctx* c) {
   c->pool_bytes = c->lin_bytes = 0;
   c->has_ds = false;
   c->minmax.clear();
+  c->cp = tvk_ctx::CompatPool{};
 }
 
 
@@ -35,6 +36,7 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
   const uint32_t core[3] = {L.brick[0] - 2 * L.overlap, L.brick[1] - 2 * L.overlap, L.brick[2] - 2 * L.overlap};
   if (keep > 1 && (core[0] % 2 || core[1] % 2 || core[2] % 2))
     return fail(c, TVK_ERR_UNSUPPORTED, "coarser LODs need an even brick core (brick size - 2*overlap)");
+  c->cp = tvk_ctx::CompatPool{};
   c->pool_bytes = (size_t)L.pool_elems * bpv;
   HIPCHK(c, hipMalloc(&c->d_pool, c->pool_bytes));
   c->pool_hold = std::shared_ptr<void>(c->d_pool, [](void* p) { (void)hipFree(p); });
@@ -214,6 +216,7 @@ void restore_dataset(tvk_ctx* c, DatasetState&& s) {
   c->build_ms = s.build_ms;
   c->has_ds = s.has_ds;
   c->leap_resident = std::move(s.leap_resident);
+  c->cp = tvk_ctx::CompatPool{};
 }
 
 }  // namespace
@@ -516,6 +519,13 @@ int tvk_crop_dataset(tvk_ctx* c, int keep_old, float plane_used[4]) {
   rc = build_pool(c, d_lin);
   if (rc) free_dataset(c);
   return rc;
+}
+
+int tvk_set_compat(tvk_ctx* c, int use_only_power_of_two, int downsample_to_8bits) {
+  if (!c) return TVK_ERR_ARG;
+  c->compat_pow2 = use_only_power_of_two ? 1 : 0;
+  c->compat_8bit = downsample_to_8bits ? 1 : 0;
+  return TVK_OK;
 }
 
 int tvk_share_dataset(tvk_ctx* dst, const tvk_ctx* src) {
@@ -881,6 +891,7 @@ int frame_lod(tvk_ctx* c, const ViewState& V, int* lod) {
 }
 
 int plan_params(tvk_ctx* c, int lod, PlanParams& pp) {
+  pp.pow2 = c->compat_pow2 != 0;
   pp.mode = c->P.mode;
   pp.lod = lod;
   pp.srm = c->P.srm;
@@ -899,9 +910,70 @@ int plan_params(tvk_ctx* c, int lod, PlanParams& pp) {
   return TVK_OK;
 }
 
+float normalized_isovalue(const tvk_ctx* c) {
+  if (downsampled(c)) {
+    // MathTools::lerp(iso, 0.f, float(mx), 0.f, 1.f) with mx the range maximum
+    // (the TF size for an empty range): float operands, a double quotient
+    const double mx = c->range_min > c->range_max ? (double)c->tf1d_n : c->range_max;
+    const float fmx = (float)mx;
+    return (float)(0.0f + (double)(c->P.isovalue - 0.0f) * (1.0 / (double)(fmx - 0.0f)));
+  }
+  return (float)((double)c->P.isovalue / (double)(1 << bit_width(c)));
+}
+
+int ensure_compat(tvk_ctx* c) {
+  if (!compat_active(c) || compat_ready(c)) return TVK_OK;
+  const Layout& L = c->L;
+  tvk_ctx::CompatPool cp;
+  cp.pow2 = c->compat_pow2;
+  cp.down8 = downsampled(c) ? 1 : 0;
+  cp.dup = c->P.clamp_to_edge ? 0 : 1;
+  cp.bpv = render_bpv(c);
+  const size_t nb = L.toc.size();
+  cp.offset.resize(nb);
+  cp.dims.resize(3 * nb);
+  std::vector<GatherBrick> src(nb), dst(nb);
+  uint64_t elems = 0, max_elems = 0;
+  const int lx = block_lx(cp.bpv);
+  for (size_t i = 0; i < nb; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t n = L.toc[i].size[k];
+      src[i].c[k] = dst[i].c[k] = (uint32_t)L.toc[i].coords[k];
+      src[i].n[k] = n;
+      dst[i].n[k] = cp.pow2 ? next_pow2(n) : n;
+      cp.dims[3 * i + k] = dst[i].n[k];
+    }
+    src[i].offset = L.toc[i].offset;
+    dst[i].offset = elems;
+    cp.offset[i] = elems;
+    const uint64_t e = padded_count(dst[i].n, lx);
+    elems += e;
+    max_elems = std::max(max_elems, e);
+  }
+  const size_t bytes = (size_t)elems * (size_t)cp.bpv;
+  c->cp = tvk_ctx::CompatPool{};   // the old textures go before the new ones are made
+  HIPCHK(c, hipMalloc(&cp.d, std::max<size_t>(bytes, 1)));
+  cp.hold = std::shared_ptr<void>(cp.d, [](void* p) { (void)hipFree(p); });
+  GatherBrick* d_gb = nullptr;
+  HIPCHK(c, hipMalloc(&d_gb, 2 * nb * sizeof(GatherBrick)));
+  hipError_t e = hipMemcpyAsync(d_gb, src.data(), nb * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_gb + nb, dst.data(), nb * sizeof(GatherBrick), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = launch_compat(c->d_pool, L.bpv, d_gb, cp.d, cp.bpv, d_gb + nb, (uint32_t)nb, max_elems, c->range_min,
+                      c->range_max, cp.dup, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_gb);
+  if (e != hipSuccess) return fail(c, TVK_ERR_HIP, std::string("compatibility textures: ") + hipGetErrorString(e));
+  cp.valid = true;
+  c->cp = std::move(cp);
+  return TVK_OK;
+}
+
 // GLRenderer::CalculateScaling (GLRenderer.cpp:1813-1819)
 float tf_scale_of(const tvk_ctx* c) {
   if (c->P.tf_scale > 0.0f) return c->P.tf_scale;
+  if (downsampled(c)) return 1.0f;   // the 8-bit textures span the TF already
   const double fMaxValue = c->range_max > 0 ? c->range_max : (double)std::max(c->tf1d_n, 1);
   const uint32_t iMaxValue = (uint32_t)(1u << bit_width(c)) - 1u;
   return float(iMaxValue / fMaxValue);
@@ -1068,7 +1140,7 @@ void brick_tf_stats(tvk_ctx* c, FrameBrick& fb) {
 // Shading constants of the context's dataset, TF and parameters
 void fill_shade(const tvk_ctx* c, ShadeConst& S) {
   S = ShadeConst{};
-  S.pool = (const uint8_t*)c->d_pool;
+  S.pool = (const uint8_t*)(compat_active(c) ? c->cp.d : c->d_pool);
   S.tf1 = c->d_tf1f;
   S.tf2f = c->d_tf2f;
   S.tf2_cols = c->d_tf2cols;
@@ -1077,7 +1149,7 @@ void fill_shade(const tvk_ctx* c, ShadeConst& S) {
   S.tf_h = c->tf2d_h;
   S.tf_scale = tf_scale_of(c);
   S.grad_scale = c->P.grad_scale;
-  S.iso_n = (float)((double)c->P.isovalue / (double)(1 << bit_width(c)));  // GetNormalizedIsovalue
+  S.iso_n = normalized_isovalue(c);
   S.ert_eps = c->P.ert_eps;
   for (int i = 0; i < 3; ++i) {
     S.dscale[i] = c->P.domain_scale[i];
@@ -1129,7 +1201,7 @@ static int launch_fixed(tvk_ctx* c, int nviews, uint64_t pixels, uint32_t groups
   // a kernel, not hipMemsetAsync, clears the counter (capturable into a graph)
   if (c->count) HIPCHK(c, launch_fill_u32((unsigned int*)fx.counter, 2u, 0u, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
-  if (a.groups_total > 0) HIPCHK(c, launch_march(a, c->L.bpv, c->stream, 0));
+  if (a.groups_total > 0) HIPCHK(c, launch_march(a, render_bpv(c), c->stream, 0));
   HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
   if (st) {
     std::memset(st, 0, sizeof(*st));
@@ -1168,6 +1240,14 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
     if (learn > 0) return fail(c, TVK_ERR_UNSUPPORTED, "the learned step needs every brick resident (pool budget 0)");
     return render_budget_impl(c, nviews, rots, host_out, st);
   }
+  if (compat_active(c)) {
+    if (learn > 0)
+      return fail(c, TVK_ERR_UNSUPPORTED, "the learned step reads the dataset's own bricks (compatibility textures set)");
+    if (fixed && !compat_ready(c))
+      return fail(c, TVK_ERR_STATE, "the allocation-free render needs the compatibility textures: render once first");
+    int rc = ensure_compat(c);
+    if (rc) return rc;
+  }
   // the host vectors are the upload source of an allocation-free render: the
   // previous call's copies must have consumed them
   if (fixed) HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1194,7 +1274,7 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
     ViewConst& vc = c->h_views[(size_t)v];
     make_view_const(V, vc);
     vc.list_offset = (uint32_t)c->h_list.size();
-    make_frame_bricks(c->L, V, pp, planned, c->h_list);
+    make_frame_bricks(c->L, V, pp, planned, c->h_list, compat_dims(c), compat_offsets(c));
     vc.nlist = (uint32_t)c->h_list.size() - vc.list_offset;
     if (learn > 0)
       for (size_t q = vc.list_offset; q < c->h_list.size(); ++q) brick_tf_stats(c, c->h_list[q]);
@@ -1258,7 +1338,7 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   a.groups_padded = (groups + 7u) & ~7u;
   if (c->count) HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
-  if (a.groups_total > 0) HIPCHK(c, launch_march(a, c->L.bpv, c->stream, learn));
+  if (a.groups_total > 0) HIPCHK(c, launch_march(a, render_bpv(c), c->stream, learn));
   HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
   c->last_views = nviews;
   c->last_w = c->V.W;

@@ -153,14 +153,19 @@ int plan_batch(tvk_batch* b, uint32_t* bricks_total) {
       V.H = r.H;
       V.proj = perspective(V.fov, (float)r.W / (float)r.H, V.znear, V.zfar);   // GLRenderer.cpp:889-920
     }
+    if (compat_active(c) && b->net && c->P.mode != TVK_RM_ISOSURFACE)
+      return bfail(b, TVK_ERR_UNSUPPORTED, "request " + std::to_string(i) +
+                                               ": the learned step reads the dataset's own bricks (compatibility "
+                                               "textures set)");
     int lod = 0;
     int rc = frame_lod(c, V, &lod);
+    if (!rc) rc = ensure_compat(c);
     if (!rc) {
       PlanParams pp;
       rc = plan_params(c, lod, pp);
       if (!rc) {
         plan_frame(c->L, c->minmax, c->range_max, V, pp, planned);
-        make_frame_bricks(c->L, V, pp, planned, P[i].list);
+        make_frame_bricks(c->L, V, pp, planned, P[i].list, compat_dims(c), compat_offsets(c));
         if (b->net && c->P.mode != TVK_RM_ISOSURFACE) {
           rc = learn_prepare(c);
           if (!rc)
@@ -179,7 +184,7 @@ int plan_batch(tvk_batch* b, uint32_t* bricks_total) {
     br.ntiles = br.tiles_x * ((r.H + th - 1) / th);
     P[i].mode = c->P.mode;
     P[i].lit = c->P.lighting != 0 && c->P.mode != TVK_RM_ISOSURFACE;
-    P[i].bpv = c->L.bpv;
+    P[i].bpv = render_bpv(c);
     P[i].ntoc = (uint32_t)c->L.toc.size();
     if (P[i].mode == TVK_RM_1DTRANS && !c->d_tf1f)
       return bfail(b, TVK_ERR_STATE, "request " + std::to_string(i) + ": no 1D transfer function");

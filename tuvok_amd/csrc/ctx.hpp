@@ -47,6 +47,9 @@ hipError_t launch_brick_max_gradient(const void* pool, int bpv, const GatherBric
 hipError_t launch_flatten(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t nb[3],
                           const uint32_t core[3], uint32_t ov, const uint64_t dims[3], void* out, hipStream_t st);
 hipError_t launch_crop(void* vol, int bpv, const uint64_t dims[3], const float pl[4], hipStream_t st);
+hipError_t launch_compat(const void* src, int src_bpv, const GatherBrick* d_src, void* dst, int dst_bpv,
+                         const GatherBrick* d_dst, uint32_t nb, uint64_t max_elems, double fmin, double fmax,
+                         int dup, hipStream_t st);
 hipError_t launch_swizzle(const void* src, int bpv, const GatherBrick* d_bricks, const uint64_t* d_src_off, uint32_t nb,
                           uint64_t max_elems, void* pool, hipStream_t st);
 
@@ -190,6 +193,20 @@ struct tvk_ctx {
   int last_views = 0;
   uint32_t last_w = 0, last_h = 0;
   int last_lod = 0;
+  // the renderer's compatibility textures (AbstrRenderer's bUseOnlyPowerOfTwo
+  // and bDownSampleTo8Bits; bDisableBorder is the clamp_to_edge parameter): a
+  // second pool of padded and/or quantized bricks that rendering reads, made
+  // on first use after the dataset or the flags change
+  int compat_pow2 = 0, compat_8bit = 0;
+  struct CompatPool {
+    void* d = nullptr;
+    std::shared_ptr<void> hold;
+    std::vector<uint64_t> offset;   // per TOC brick: element offset
+    std::vector<uint32_t> dims;     // per TOC brick: texture size (3 each)
+    int bpv = 0;
+    int pow2 = 0, down8 = 0, dup = 0;
+    bool valid = false;
+  } cp;
   // device address of this GPU's copy of the allocation-free render's scratch
   // array (looked up once per device at tvk_create); null if the lookup failed
   void* scratch = nullptr;
@@ -234,6 +251,24 @@ int ensure_pinned(tvk_ctx* c, T** p, size_t* cap, size_t n) {
 }
 
 inline int bit_width(const tvk_ctx* c) { return c->L.bpv == 1 ? 8 : 16; }
+
+// The compatibility textures (AbstrRenderer.h:170-174): bDownSampleTo8Bits
+// applies to data that is not 8 bits already; bUseOnlyPowerOfTwo to all.
+inline bool downsampled(const tvk_ctx* c) { return c->compat_8bit != 0 && c->L.bpv == 2; }
+inline bool compat_active(const tvk_ctx* c) { return c->has_ds && (c->compat_pow2 != 0 || downsampled(c)); }
+// bytes per voxel of the textures rendering reads
+inline int render_bpv(const tvk_ctx* c) { return downsampled(c) ? 1 : c->L.bpv; }
+// the compatibility textures exist for the current dataset, flags and border
+inline bool compat_ready(const tvk_ctx* c) {
+  return c->cp.valid && c->cp.pow2 == c->compat_pow2 && c->cp.down8 == (downsampled(c) ? 1 : 0) &&
+         c->cp.dup == (c->P.clamp_to_edge ? 0 : 1);
+}
+inline const uint32_t* compat_dims(const tvk_ctx* c) { return compat_active(c) ? c->cp.dims.data() : nullptr; }
+inline const uint64_t* compat_offsets(const tvk_ctx* c) { return compat_active(c) ? c->cp.offset.data() : nullptr; }
+// (re)makes the compatibility textures when they are out of date
+int ensure_compat(tvk_ctx* c);
+// AbstrRenderer::GetNormalizedIsovalue (AbstrRenderer.cpp:412-424)
+float normalized_isovalue(const tvk_ctx* c);
 
 // LOD of a frame (fixed, or the screen-space-error minimum of the view)
 int frame_lod(tvk_ctx* c, const ViewState& V, int* lod);

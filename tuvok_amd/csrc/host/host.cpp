@@ -129,13 +129,21 @@ uint64_t Layout::toc_index(uint64_t x, uint64_t y, uint64_t z, uint64_t lod) con
   return l.offset + x + y * l.bricks[0] + z * l.bricks[0] * l.bricks[1];
 }
 
-void Layout::texcoords(int lod, uint64_t lin, F3& tmin, F3& tmax) const {
+void Layout::texcoords(int lod, uint64_t lin, F3& tmin, F3& tmax, bool pow2) const {
   const BrickMeta& m = table.at(BrickKey(0, (size_t)lod, (size_t)lin));
   const LodInfo& l = lods[lod];
   float ov = (float)overlap;
   F3 nv((float)m.n_voxels[0], (float)m.n_voxels[1], (float)m.n_voxels[2]);
-  tmin = F3(ov, ov, ov) / nv;
-  tmax = F3(1, 1, 1) - tmin;
+  if (pow2) {
+    // the brick's voxels at the start of a texture of the next powers of two
+    F3 real((float)next_pow2(m.n_voxels[0]), (float)next_pow2(m.n_voxels[1]), (float)next_pow2(m.n_voxels[2]));
+    tmin = F3(ov, ov, ov) / real;
+    tmax = F3(1, 1, 1) - tmin;
+    tmax = tmax - (real - nv) / real;
+  } else {
+    tmin = F3(ov, ov, ov) / nv;
+    tmax = F3(1, 1, 1) - tmin;
+  }
   tmax = tmax * F3((float)l.aspect[0], (float)l.aspect[1], (float)l.aspect[2]);
 }
 
@@ -260,7 +268,7 @@ void plan_frame(const Layout& L, const std::vector<double>& minmax, double range
     }
     b.is_empty = contains ? 0 : 1;
     if (!b.is_empty) {
-      L.texcoords(P.lod, b.key_index, b.tex_min, b.tex_max);
+      L.texcoords(P.lod, b.key_index, b.tex_min, b.tex_max, P.pow2);
       b.distance = brick_distance(b.center, b.extension, mv);
     }
     out.push_back(b);
@@ -307,7 +315,8 @@ void make_view_const(const ViewState& V, ViewConst& vc) {
 }
 
 void make_frame_bricks(const Layout& L, const ViewState& V, const PlanParams& P,
-                       const std::vector<PlannedBrick>& list, std::vector<FrameBrick>& out) {
+                       const std::vector<PlannedBrick>& list, std::vector<FrameBrick>& out,
+                       const uint32_t* tex_dims, const uint64_t* tex_offset) {
   Mat4 mv = V.modelview();
   Mat4 imv = inverse(mv);
   Mat4 mvp = mv * V.proj;
@@ -324,14 +333,18 @@ void make_frame_bricks(const Layout& L, const ViewState& V, const PlanParams& P,
     for (int r = 0; r < 4; ++r)
       for (int c = 0; c < 3; ++c) f.e2t[r * 3 + c] = e2t.at(r, c);
     for (int i = 0; i < 3; ++i) { f.bmin[i] = bmin[i]; f.bmax[i] = bmax[i]; }
-    // GLRaycaster::SetBrickDepShaderVars (GLRaycaster.cpp:239-257)
-    F3 vox = F3(1, 1, 1) / F3((float)b.voxel_count[0], (float)b.voxel_count[1], (float)b.voxel_count[2]);
+    // GLRaycaster::SetBrickDepShaderVars (GLRaycaster.cpp:239-257): one texel of
+    // the brick's texture (padded to powers of two with bUseOnlyPowerOfTwo)
+    uint32_t tn[3] = {b.voxel_count[0], b.voxel_count[1], b.voxel_count[2]};
+    if (tex_dims)
+      for (int i = 0; i < 3; ++i) tn[i] = tex_dims[3 * b.toc_index + i];
+    F3 vox = F3(1, 1, 1) / F3((float)tn[0], (float)tn[1], (float)tn[2]);
     f.ray_step = min3(b.extension * vox * 0.5f * (1.0f / P.srm));
     F3 q = F3((float)d0[0], (float)d0[1], (float)d0[2]) / F3((float)dl[0], (float)dl[1], (float)dl[2]);
     f.step_scale = 1.0f / P.srm * max3(q);
-    for (int i = 0; i < 3; ++i) { f.vstep[i] = vox[i]; f.n[i] = b.voxel_count[i]; }
+    for (int i = 0; i < 3; ++i) { f.vstep[i] = vox[i]; f.n[i] = tn[i]; }
     const TocBrick& t = L.toc[b.toc_index];
-    f.offset = t.offset;
+    f.offset = tex_offset ? tex_offset[b.toc_index] : t.offset;
     f.toc = (uint32_t)b.toc_index;
     // conservative pixel rectangle of the projected box
     double xmin = 1e30, xmax = -1e30, ymin = 1e30, ymax = -1e30;

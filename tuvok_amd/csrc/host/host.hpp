@@ -68,9 +68,16 @@ struct Layout {
              std::string* err);
   void brick_size(const uint64_t c[4], uint32_t out[3]) const;
   uint64_t toc_index(uint64_t x, uint64_t y, uint64_t z, uint64_t lod) const;
-  // UVFDataset::GetTextCoords (uvfDataset.cpp:1747-1774), no pow2 padding
-  void texcoords(int lod, uint64_t lin, F3& tmin, F3& tmax) const;
+  // UVFDataset::GetTextCoords (uvfDataset.cpp:1747-1774)
+  void texcoords(int lod, uint64_t lin, F3& tmin, F3& tmax, bool pow2 = false) const;
 };
+
+// MathTools::NextPow2 (MathTools.h:71): the smallest power of two >= n
+inline uint32_t next_pow2(uint32_t n) {
+  uint32_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
 
 struct ViewState {
   Mat4 view, rot, trans, proj;
@@ -88,6 +95,7 @@ struct PlanParams {
   float isovalue = 0.f;
   uint64_t lim[4] = {0, 0, 0, 0};  // TF non-zero limits (1D: [0..1], 2D: [0..3])
   int tf_n = 256;                  // 1D TF size or 2D TF width
+  bool pow2 = false;               // power-of-two textures (texture coordinates of the padded brick)
 };
 
 struct PlannedBrick {
@@ -105,8 +113,11 @@ void plan_frame(const Layout& L, const std::vector<double>& minmax, double range
 // ComputeMinLODForCurrentView (AbstrRenderer.cpp:789-803)
 int min_lod_for_view(const Layout& L, const ViewState& V, float sse);
 // Per-brick uniforms + eye->texture matrix + screen rect (GLRaycaster.cpp:239-345, :589-612)
+// tex_dims/tex_offset (per TOC brick, may be null): the textures rendering
+// reads when they are not the pool's bricks (power-of-two padded sizes)
 void make_frame_bricks(const Layout& L, const ViewState& V, const PlanParams& P,
-                       const std::vector<PlannedBrick>& list, std::vector<FrameBrick>& out);
+                       const std::vector<PlannedBrick>& list, std::vector<FrameBrick>& out,
+                       const uint32_t* tex_dims = nullptr, const uint64_t* tex_offset = nullptr);
 void make_view_const(const ViewState& V, ViewConst& vc);
 
 // TransferFunction1D::Set(rgba8) + ComputeNonZeroLimits; TransferFunction2D::ComputeNonZeroLimits

@@ -582,6 +582,69 @@ hipError_t launch_swizzle(const void* src, int bpv, const GatherBrick* d_bricks,
   return hipGetLastError();
 }
 
+// (unsigned char) of (255 v - fMin) / (fMax - fMin), GLVolumeListElem::CreateTexture
+// (GPUMemManDataStructs.cpp:410-425).  A conversion of an out-of-range double to
+// unsigned char is left undefined by C++; x86 compilers truncate to a 32-bit
+// integer (0x80000000 when it does not fit or is NaN) and keep the low byte,
+// which is what this returns.
+__device__ __forceinline__ uint8_t quantize8(uint32_t v, double fmin, double fmax) {
+  const double q = (255.0 * (double)v - fmin) / (fmax - fmin);
+  if (!(q > -2147483649.0 && q < 2147483648.0)) return 0;
+  return (uint8_t)(uint32_t)(int32_t)q;
+}
+
+// The renderer's compatibility textures of one brick set: brick k of dst is
+// brick k of src, quantized to 8 bits (quant) and/or padded to dst's size
+// (GLVolumeListElem::PadData, GPUMemManDataStructs.cpp:306-360): voxel (x,y,z)
+// of the padded texture is the source voxel while every coordinate is inside
+// the source brick; with dup (the border in use) one more element, row and
+// slice repeat the last ones; everything else is 0.
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void compat_bricks(const S* __restrict__ src, const GatherBrick* __restrict__ sb,
+                                                     D* __restrict__ dst, const GatherBrick* __restrict__ db,
+                                                     double fmin, double fmax, int quant, int dup) {
+  const GatherBrick s = sb[blockIdx.y], d = db[blockIdx.y];
+  const int ls = lx_of<S>(), ld = lx_of<D>();
+  const uint32_t snbx = nblocks(s.n[0], ls), snby = nblocks(s.n[1], 2);
+  const uint32_t dnbx = nblocks(d.n[0], ld), dnby = nblocks(d.n[1], 2);
+  const uint64_t n = padded_count(d.n, ld);
+  const uint32_t lim0 = s.n[0] - (dup ? 0u : 1u), lim1 = s.n[1] - (dup ? 0u : 1u), lim2 = s.n[2] - (dup ? 0u : 1u);
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    uint32_t x, y, z;
+    unswz(i, dnbx, dnby, ld, x, y, z);
+    x = min(x, d.n[0] - 1); y = min(y, d.n[1] - 1); z = min(z, d.n[2] - 1);
+    D v = 0;
+    if (x <= lim0 && y <= lim1 && z <= lim2) {
+      const uint32_t u = (uint32_t)src[s.offset + swz(min(x, s.n[0] - 1), min(y, s.n[1] - 1), min(z, s.n[2] - 1),
+                                                      snbx, snby, ls)];
+      v = quant ? (D)quantize8(u, fmin, fmax) : (D)u;
+    }
+    dst[d.offset + i] = v;
+  }
+}
+
+hipError_t launch_compat(const void* src, int src_bpv, const GatherBrick* d_src, void* dst, int dst_bpv,
+                         const GatherBrick* d_dst, uint32_t nb, uint64_t max_elems, double fmin, double fmax,
+                         int dup, hipStream_t st) {
+  uint64_t gx = (max_elems + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  if (gx == 0) gx = 1;
+  for (uint32_t first = 0; first < nb; first += 65535) {
+    const uint32_t cnt = nb - first < 65535 ? nb - first : 65535;
+    const dim3 g((uint32_t)gx, cnt);
+    if (src_bpv == 2 && dst_bpv == 1)
+      hipLaunchKernelGGL((compat_bricks<uint16_t, uint8_t>), g, dim3(256), 0, st, (const uint16_t*)src, d_src + first,
+                         (uint8_t*)dst, d_dst + first, fmin, fmax, 1, dup);
+    else if (src_bpv == 2)
+      hipLaunchKernelGGL((compat_bricks<uint16_t, uint16_t>), g, dim3(256), 0, st, (const uint16_t*)src,
+                         d_src + first, (uint16_t*)dst, d_dst + first, fmin, fmax, 0, dup);
+    else
+      hipLaunchKernelGGL((compat_bricks<uint8_t, uint8_t>), g, dim3(256), 0, st, (const uint8_t*)src, d_src + first,
+                         (uint8_t*)dst, d_dst + first, fmin, fmax, 0, dup);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_flatten(const void* pool, int bpv, const GatherBrick* d_bricks, const uint32_t nb[3],
                           const uint32_t core[3], uint32_t ov, const uint64_t dims[3], void* out, hipStream_t st) {
   const uint64_t n = dims[0] * dims[1] * dims[2];

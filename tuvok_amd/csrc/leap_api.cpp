@@ -256,6 +256,8 @@ int tvk_leap_set_resident(tvk_ctx* c, const uint8_t* flags, size_t n) {
 int tvk_leap_render(tvk_ctx* c, int first, float* host_out, tvk_leap_stats* st) {
   if (!c) return TVK_ERR_ARG;
   if (!c->has_ds || !c->has_view) return fail(c, TVK_ERR_STATE, "no dataset or view");
+  if (compat_active(c))
+    return fail(c, TVK_ERR_UNSUPPORTED, "GridLeaper reads the volume pool's bricks (compatibility textures set)");
   HIPCHK(c, hipSetDevice(c->device));
   Grid G;
   int rc = make_grid(c, G);

@@ -477,6 +477,8 @@ int render_budgeted(tvk_ctx* c, int nviews, const float* rots, float* host_out, 
 
 namespace tvk {
 int render_budget_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_stats* st) {
+  if (compat_active(c))
+    return fail(c, TVK_ERR_UNSUPPORTED, "compatibility textures need every brick resident (pool budget 0)");
   return render_budgeted(c, nviews, rots, host_out, st);
 }
 }  // namespace tvk
@@ -603,6 +605,7 @@ int tvk_upload_brick(tvk_ctx* c, uint32_t lod, uint64_t index, const void* data,
     c->ingest_missing[t] = 0;
     --c->ingest_pending;
   }
+  c->cp = tvk_ctx::CompatPool{};   // made again from the new contents on the next render
   if (c->cells && c->cells->d) c->cells = std::make_shared<CellGrid>();   // rebuilt on next use
   return TVK_OK;
 }

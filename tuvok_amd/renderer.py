@@ -34,16 +34,24 @@ def rotation_y(angle):
 
 
 class HIPRaycaster:
-    def __init__(self, device=0):
+    def __init__(self, device=0, use_only_power_of_two=False, downsample_to_8bits=False, disable_border=None):
+        """AbstrRenderer(pMasterController, bUseOnlyPowerOfTwo, bDownSampleTo8Bits,
+        bDisableBorder) (AbstrRenderer.h:160-174).  disable_border (None: the
+        clamp_to_edge parameter's default) selects GL_CLAMP_TO_EDGE textures
+        without the repeated padding element (GPUMemManDataStructs.cpp:340-360)."""
         self._L = _abi.load()
         h = C.c_void_p()
         rc = self._L.tvk_create(C.byref(h), device)
         if rc != 0:
             raise TvkError(rc, "tvk_create failed (no GPU?)")
         self._h = h
+        self._pow2, self._down8 = bool(use_only_power_of_two), bool(downsample_to_8bits)
+        self._chk(self._L.tvk_set_compat(self._h, int(self._pow2), int(self._down8)))
         self._p = _abi.Params()
         self._L.tvk_default_params(C.byref(self._p))
         self._p.lod = 0      # RT_CAPTURE renders LOD 0 (AbstrRenderer.cpp:1188-1190)
+        if disable_border is not None:
+            self._p.clamp_to_edge = int(bool(disable_border))
         self._eye = np.array([0, 0, 1.6], np.float32)   # AbstrRenderer.cpp:63-69
         self._at = np.array([0, 0, 0], np.float32)
         self._up = np.array([0, 1, 0], np.float32)
