@@ -133,6 +133,14 @@ int tvk_load_volume(tvk_ctx* ctx, const void* host_voxels, int dtype, const uint
 int tvk_generate_volume(tvk_ctx* ctx, int kind, uint64_t seed, int dtype, const uint64_t dims[3],
                         const uint32_t brick[3], uint32_t overlap, int clamp_to_edge,
                         int max_lods, int keep_linear);
+/* The filter of the coarser levels for the next bricking (load, generate,
+ * rebrick, crop): ExtendedOctreeConverter::Convert's bComputeMedian
+ * (ExtendedOctreeConverter.h:97-113; VolumeTools::Filter, VolumeTools.h:150-258):
+ * 0 = the mean of each 2x2x2 group (default), 1 = the reference's median.  Odd
+ * brick cores (brick size - 2*overlap) filter brick to brick as the reference
+ * does (ExtendedOctreeConverter.inc:250-356) and need an overlap of at least 1
+ * and bClampToEdge off when coarser levels are kept (TVK_ERR_UNSUPPORTED). */
+int tvk_set_lod_filter(tvk_ctx* ctx, int median);
 /* Copy the retained linear LOD0 volume back to the host. */
 /* Makes dst render src's dataset: the brick pool (and kept linear volume) in HBM
  * is shared, not copied, and stays allocated while any context holds it; dst

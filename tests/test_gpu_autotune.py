@@ -54,11 +54,11 @@ def test_oversized_bricks_are_rejected():
     r.Resize(40, 32)
     before = r.Paint()
     pool = r.GetPool()
-    # a rejected size, and an odd core with a pyramid (unsupported after the
-    # layout was accepted), both leave the dataset in place
-    for brick, lods in (((4097, 8, 8), 0), ((2048, 2048, 2048), 0), ((11, 11, 11), 0)):
+    # a rejected size, and an odd core without overlap under a pyramid
+    # (unsupported after the layout was accepted), both leave the dataset in place
+    for brick, ov, lods in (((4097, 8, 8), 1, 0), ((2048, 2048, 2048), 1, 0), ((11, 11, 11), 0, 0)):
         with pytest.raises(Exception, match="brick"):
-            r.Rebrick(brick, 1, max_lods=lods)
+            r.Rebrick(brick, ov, max_lods=lods)
         assert np.array_equal(r.Paint(), before)
         assert np.array_equal(r.DownloadVolume(), vol)
         assert np.array_equal(r.GetPool(), pool)

@@ -572,3 +572,37 @@ def test_crop_dataset_matches_closed_form(ren):
             assert np.array_equal(ren.DownloadVolume(), want)
     ren.SetRotation(np.eye(4, dtype=np.float32))
     ren.DisableClipPlane()
+
+
+@pytest.mark.parametrize("dims,brick,ov,dt,median", [
+    ((44, 36, 40), (10, 10, 10), 1, np.uint8, True),        # even cores, median filter
+    ((37, 21, 50), (12, 10, 16), 1, np.uint16, True),
+    ((29, 16, 9), (17, 17, 17), 1, np.uint8, False),        # odd cores: brick-to-brick filtering
+    ((45, 23, 30), (19, 19, 19), 2, np.uint16, False),
+    ((33, 40, 22), (13, 15, 13), 1, np.uint8, True),
+    ((61, 7, 33), (11, 9, 13), 1, np.uint16, True),
+])
+def test_lod_filter_and_odd_cores_bit_exact(ren, dims, brick, ov, dt, median):
+    """ExtendedOctreeConverter's bComputeMedian and odd brick cores: every level,
+    overlaps included, and every brick's min/max equal the oracle's."""
+    print(f"lod filter dims={dims} brick={brick} ov={ov} median={median}", flush=True)
+    full = 255 if dt == np.uint8 else 65535
+    vol = np.random.default_rng(9).integers(0, full + 1, size=dims[::-1]).astype(dt)
+    ds = O.Dataset(vol, brick, ov, median=median)
+    ren.SetLODFilter(median)
+    try:
+        ren.LoadVolume(vol, brick, ov, max_lods=0)
+    finally:
+        ren.SetLODFilter(False)
+    pool = ren.GetPool()
+    ref = ds.all_data()
+    assert ds.lod_count > 1 and pool.nbytes == ref.nbytes
+    assert np.array_equal(pool, ref)
+    for i in range(ds.brick_count):
+        assert ren.MaxMinForKey(i) == ds.brick_info(i)["minmax"]
+
+
+def test_odd_cores_with_clamp_are_refused(ren):
+    vol = np.zeros((20, 20, 20), np.uint8)
+    with pytest.raises(Exception):
+        ren.LoadVolume(vol, (11, 11, 11), 1, clamp_to_edge=True, max_lods=0)

@@ -208,3 +208,61 @@ def test_fp16_rounding_matches_ieee():
     for x in xs:
         want = np.float32(np.float16(x))
         assert O.round_fp16(float(x)) == want or (np.isinf(want) and np.isinf(O.round_fp16(float(x))))
+
+
+def odd_core_level1_closed_form(vol, core, ov, split, lod1_bricks, lod1_size, median=False):
+    """Level-1 core voxels of the reference's brick-to-brick filter for odd cores,
+    in global indices: coarse brick B's core voxel j (per axis) is the filter of
+    the finer level's voxels child*core + 2*l + {0, 1} of child 2B + (j >= split),
+    l = j - split * (j >= split), the second tap only while it is in the child's
+    core (the odd plane filters one)."""
+    nz, ny, nx = vol.shape
+    dims = (nx, ny, nz)
+    out = {}
+    for bz in range(lod1_bricks[2]):
+        for by in range(lod1_bricks[1]):
+            for bx in range(lod1_bricks[0]):
+                B = (bx, by, bz)
+                cc = [min(core, lod1_size[a] - B[a] * core) for a in range(3)]
+                blk = np.zeros(cc[::-1], np.int64)
+                for k in range(cc[2]):
+                    for jj in range(cc[1]):
+                        for i in range(cc[0]):
+                            taps_axes = []
+                            for a, j in zip(range(3), (i, jj, k)):
+                                d = 1 if j >= split else 0
+                                child = 2 * B[a] + d
+                                l = j - split * d
+                                cf = min(core, dims[a] - child * core)     # the child's core size
+                                g = child * core + 2 * l
+                                taps_axes.append([g] + ([g + 1] if 2 * l + 1 < cf else []))
+                            taps = [int(vol[z, y, x]) for x in taps_axes[0] for y in taps_axes[1]
+                                    for z in taps_axes[2]]
+                            blk[k, jj, i] = sum(taps) // len(taps)
+                out[B] = blk
+    return out
+
+
+@pytest.mark.parametrize("dims,brick,ov", [((29, 16, 9), (17, 17, 17), 1), ((33, 40, 22), (13, 13, 13), 1),
+                                           ((45, 23, 30), (19, 19, 19), 2)])
+def test_odd_core_coarse_level_closed_form(dims, brick, ov):
+    vol = np.random.default_rng(11).integers(0, 256, size=dims[::-1]).astype(np.uint8)
+    ds = O.Dataset(vol, brick, ov)
+    core = brick[0] - 2 * ov
+    assert all(b - 2 * ov == core for b in brick[:2])
+    if brick[2] != brick[0]:
+        pytest.skip("closed form written for cubic cores")
+    split = (core + 1) // 2
+    want = odd_core_level1_closed_form(vol, core, ov, split, ds.lod_bricks(1), ds.lod_size(1))
+    checked = 0
+    for i in range(ds.brick_count):
+        info = ds.brick_info(i)
+        c = info["coords"]
+        if c[3] != 1:
+            continue
+        b = ds.brick_data(i)
+        w = want[(c[0], c[1], c[2])]
+        got = b[ov:ov + w.shape[0], ov:ov + w.shape[1], ov:ov + w.shape[2]]
+        assert np.array_equal(got, w), c
+        checked += 1
+    assert checked == int(np.prod(ds.lod_bricks(1)))

@@ -29,7 +29,7 @@ EXPORTS = [
     "tvk_leap_page_in", "tvk_leap_resume_state",
     "tvk_set_bricks", "tvk_upload_brick", "tvk_set_pool_budget", "tvk_get_pool_stats",
     "tvk_leap_set_request_table", "tvk_leap_requests", "tvk_pool_resident",
-    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat",
+    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat", "tvk_set_lod_filter",
 ]
 
 
@@ -132,6 +132,7 @@ def load(path=LIB_PATH):
         "tvk_set_rotation": (C.c_int, [vp, fp]),
         "tvk_set_clip_plane": (C.c_int, [vp, fp, C.c_int]),
         "tvk_set_compat": (C.c_int, [vp, C.c_int, C.c_int]),
+        "tvk_set_lod_filter": (C.c_int, [vp, C.c_int]),
         "tvk_crop_dataset": (C.c_int, [vp, C.c_int, fp]),
         "tvk_pick": (C.c_int, [vp, C.c_uint32, C.c_uint32, fp]),
         "tvk_get_view_matrices": (C.c_int, [vp, fp, fp]),

@@ -34,8 +34,13 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
   const int bpv = L.bpv;
   const size_t keep = L.toc.empty() ? 0 : (size_t)(L.toc.back().coords[3] + 1);
   const uint32_t core[3] = {L.brick[0] - 2 * L.overlap, L.brick[1] - 2 * L.overlap, L.brick[2] - 2 * L.overlap};
-  if (keep > 1 && (core[0] % 2 || core[1] % 2 || core[2] % 2))
-    return fail(c, TVK_ERR_UNSUPPORTED, "coarser LODs need an even brick core (brick size - 2*overlap)");
+  // odd cores: the coarser levels are filtered brick to brick (the 2x2x2 groups
+  // stop at each brick's core), which differs from filtering the level volume
+  const bool odd = keep > 1 && (core[0] % 2 || core[1] % 2 || core[2] % 2);
+  if (odd && c->clamp)
+    return fail(c, TVK_ERR_UNSUPPORTED, "odd brick cores with coarser LODs need bClampToEdge off");
+  if (odd && L.overlap == 0)
+    return fail(c, TVK_ERR_UNSUPPORTED, "odd brick cores with coarser LODs need an overlap of at least 1");
   c->cp = tvk_ctx::CompatPool{};
   c->pool_bytes = (size_t)L.pool_elems * bpv;
   HIPCHK(c, hipMalloc(&c->d_pool, c->pool_bytes));
@@ -64,30 +69,56 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
     if (e != hipSuccess) rc = hipfail(c, "hipMalloc(minmax)", e);
   }
   bool refilled = false;
+  void* d_snap = nullptr;   // a level's bricks before FillOverlap (odd cores)
   for (size_t lod = 0; lod < keep && rc == TVK_OK; ++lod) {
     const LodInfo& li = L.lods[lod];
-    if (lod > 0) {
-      const uint64_t* sd = L.lods[lod - 1].size;
-      void*& dst = tmp[(lod - 1) % 2];
-      if (!dst) {
-        size_t bytes = (size_t)(li.size[0] * li.size[1] * li.size[2]) * bpv;
-        hipError_t e = hipMalloc(&dst, bytes);
-        if (e != hipSuccess) { rc = hipfail(c, "hipMalloc(lod temp)", e); break; }
-      }
-      hipError_t e = launch_downsample(src, sd, dst, li.size, bpv, c->stream);
-      if (e != hipSuccess) { rc = hipfail(c, "downsample", e); break; }
-      src = dst;
-    }
     const uint32_t nb = (uint32_t)(li.bricks[0] * li.bricks[1] * li.bricks[2]);
     uint32_t maxv = 0;
     for (uint32_t b = 0; b < nb; ++b) {
       const TocBrick& t = L.toc[li.offset + b];
       maxv = std::max<uint32_t>(maxv, (uint32_t)t.count);
     }
-    hipError_t e = launch_gather(src, bpv, li.size, d_gb + li.offset, nb, (uint32_t)li.bricks[0], maxv, c->d_pool,
-                                 core, L.overlap, c->clamp, d_mm + 2 * li.offset, c->stream);
-    if (e != hipSuccess) { rc = hipfail(c, "gather", e); break; }
+    if (lod > 0 && odd) {
+      // ExtendedOctreeConverter::DownsampleBrick from the finer level's bricks
+      const LodInfo& pl = L.lods[lod - 1];
+      const uint32_t fb[3] = {(uint32_t)pl.bricks[0], (uint32_t)pl.bricks[1], (uint32_t)pl.bricks[2]};
+      const uint32_t split[3] = {(core[0] + 1) / 2, (core[1] + 1) / 2, (core[2] + 1) / 2};
+      hipError_t e = launch_downsample_bricks(c->d_pool, bpv, d_gb + li.offset, nb, d_gb + pl.offset, fb, L.overlap,
+                                              split, maxv, c->lod_median, c->stream);
+      if (e != hipSuccess) { rc = hipfail(c, "downsample_bricks", e); break; }
+      refilled = true;   // min/max and repeated rows of the coarse levels are made after the fill
+    } else {
+      if (lod > 0) {
+        const uint64_t* sd = L.lods[lod - 1].size;
+        void*& dst = tmp[(lod - 1) % 2];
+        if (!dst) {
+          size_t bytes = (size_t)(li.size[0] * li.size[1] * li.size[2]) * bpv;
+          hipError_t e = hipMalloc(&dst, bytes);
+          if (e != hipSuccess) { rc = hipfail(c, "hipMalloc(lod temp)", e); break; }
+        }
+        hipError_t e = launch_downsample(src, sd, dst, li.size, bpv, c->stream, c->lod_median);
+        if (e != hipSuccess) { rc = hipfail(c, "downsample", e); break; }
+        src = dst;
+      }
+      hipError_t e = launch_gather(src, bpv, li.size, d_gb + li.offset, nb, (uint32_t)li.bricks[0], maxv, c->d_pool,
+                                   core, L.overlap, c->clamp, d_mm + 2 * li.offset, c->stream);
+      if (e != hipSuccess) { rc = hipfail(c, "gather", e); break; }
+    }
     if (lod > 0 && !c->clamp && nb > 1) {
+      hipError_t e = hipSuccess;
+      uint64_t snap_base = 0;
+      if (odd) {
+        // the pre-fill state that later neighbours are read in
+        const TocBrick& last = L.toc[li.offset + nb - 1];
+        snap_base = L.toc[li.offset].offset;
+        const size_t bytes = (size_t)(last.offset + last.count - snap_base) * bpv;
+        if (d_snap) { (void)hipFree(d_snap); d_snap = nullptr; }
+        e = hipMalloc(&d_snap, bytes);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(d_snap, (uint8_t*)c->d_pool + snap_base * bpv, bytes, hipMemcpyDeviceToDevice,
+                             c->stream);
+        if (e != hipSuccess) { rc = hipfail(c, "fill snapshot", e); break; }
+      }
       refilled = true;
       // reproduce FillOverlap's copy order on the coarse level (see bricking.hip)
       const uint32_t bxyz[3] = {(uint32_t)li.bricks[0], (uint32_t)li.bricks[1], (uint32_t)li.bricks[2]};
@@ -103,7 +134,7 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
       }
       start.push_back((uint32_t)order.size());
       uint32_t* d_order = nullptr;
-      e = hipMalloc(&d_order, order.size() * sizeof(uint32_t));
+      if (e == hipSuccess) e = hipMalloc(&d_order, order.size() * sizeof(uint32_t));
       if (e == hipSuccess)
         e = hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
       uint32_t shell = 0;
@@ -115,7 +146,7 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
       for (size_t w = 0; w + 1 < start.size() && e == hipSuccess; ++w)
         if (start[w + 1] > start[w])
           e = launch_fill_overlap(c->d_pool, bpv, d_gb + li.offset, bxyz, L.overlap, d_order + start[w],
-                                  start[w + 1] - start[w], shell, c->stream);
+                                  start[w + 1] - start[w], shell, c->stream, d_snap, snap_base);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (d_order) (void)hipFree(d_order);
       if (e != hipSuccess) { rc = hipfail(c, "fill_overlap", e); break; }
@@ -146,6 +177,7 @@ int build_pool(tvk_ctx* c, const void* d_lin) {
   for (hipEvent_t e : ev)
     if (e) (void)hipEventDestroy(e);
   if (d_mm) (void)hipFree(d_mm);
+  if (d_snap) (void)hipFree(d_snap);
   (void)hipFree(d_gb);
   for (void* t : tmp) if (t) (void)hipFree(t);
   if (rc != TVK_OK) return rc;
@@ -519,6 +551,12 @@ int tvk_crop_dataset(tvk_ctx* c, int keep_old, float plane_used[4]) {
   rc = build_pool(c, d_lin);
   if (rc) free_dataset(c);
   return rc;
+}
+
+int tvk_set_lod_filter(tvk_ctx* c, int median) {
+  if (!c) return TVK_ERR_ARG;
+  c->lod_median = median ? 1 : 0;
+  return TVK_OK;
 }
 
 int tvk_set_compat(tvk_ctx* c, int use_only_power_of_two, int downsample_to_8bits) {

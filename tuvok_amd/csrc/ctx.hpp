@@ -22,7 +22,10 @@ hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const
                          uint32_t row_len, uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp,
                          uint32_t* mm, hipStream_t st);
 hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
-                             hipStream_t st);
+                             hipStream_t st, int median);
+hipError_t launch_downsample_bricks(void* pool, int bpv, const GatherBrick* d_coarse, uint32_t nb,
+                                    const GatherBrick* d_fine, const uint32_t fine_bricks[3], uint32_t ov,
+                                    const uint32_t split[3], uint64_t max_elems, int median, hipStream_t st);
 hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,
                          uint32_t* out, hipStream_t st);
 hipError_t launch_unswizzle(const void* pool, int bpv, const GatherBrick* d_bricks, const uint64_t* d_ref_off,
@@ -31,7 +34,7 @@ hipError_t launch_max_gradient(const void* lin, int bpv, const uint64_t d[3], in
                                hipStream_t st);
 hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bricks, const uint32_t bricks[3],
                                uint32_t ov, const uint32_t* d_wave, uint32_t wave_n, uint32_t max_shell,
-                               hipStream_t st);
+                               hipStream_t st, const void* snap = nullptr, uint64_t snap_base = 0);
 }  // namespace tvk
 
 
@@ -198,6 +201,7 @@ struct tvk_ctx {
   // second pool of padded and/or quantized bricks that rendering reads, made
   // on first use after the dataset or the flags change
   int compat_pow2 = 0, compat_8bit = 0;
+  int lod_median = 0;   // ExtendedOctreeConverter's bComputeMedian for the next bricking
   struct CompatPool {
     void* d = nullptr;
     std::shared_ptr<void> hold;

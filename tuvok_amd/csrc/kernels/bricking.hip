@@ -229,33 +229,134 @@ __global__ __launch_bounds__(256) void crop_plane(T* __restrict__ vol, uint64_t 
   }
 }
 
-// 2x box filter with the reference's mean (double accumulation, truncation);
-// an axis of size 1 is not halved (ExtendedOctree.cpp:201-213).
+// VolumeTools::Filter (VolumeTools.h:150-258) over the taps of one 2x2x2 group
+// that exist, in the reference's order (x-major: p0 = (0,0,0), p1 = +z, p2 = +y,
+// p3 = +y+z, p4 = +x, ...; the odd planes and lines of ExtendedOctreeConverter.inc:
+// 60-248 keep the same relative order): the mean in double precision truncated
+// to T, or the reference's median -- of 8: a sorting network over 4, two
+// insertions and a clamp of the 7th tap; of 4: the median of the first three;
+// of 2 and 1: the first tap.
+template <typename T> __device__ __forceinline__ void order2(T& a, T& b) {
+  if (a > b) { const T t = a; a = b; b = t; }
+}
+template <typename T> __device__ __forceinline__ void insert4(T& a, T& b, T& c, T& d, T p) {
+  if (p > c) {
+    order2(d, p);
+  } else if (p < b) {
+    d = c; c = b; b = p;
+    order2(a, b);
+  } else {
+    d = c; c = p;
+  }
+}
+template <typename T> __device__ __forceinline__ T filter_taps(const T* t, int n, bool median) {
+  if (!median) {
+    uint32_t sum = 0;
+    for (int i = 0; i < n; ++i) sum += t[i];
+    return (T)(sum / (uint32_t)n);   // double(sum) / n truncated: n is a power of two
+  }
+  if (n == 8) {
+    T a = t[0], b = t[1], c = t[2], d = t[3];
+    order2(a, b); order2(c, d); order2(a, c); order2(b, d); order2(b, c);
+    insert4(a, b, c, d, t[4]);
+    insert4(a, b, c, d, t[5]);
+    return max(min(d, t[6]), c);
+  }
+  if (n == 4) {
+    T a = t[0], b = t[1], c = t[2];
+    order2(a, b); order2(b, c);
+    return max(a, b);
+  }
+  return t[0];
+}
+
+// 2x filter of a level volume (mean or median); an axis of size 1 is not halved
+// (ExtendedOctree.cpp:201-213).  With even brick cores the reference's
+// brick-to-brick filtering (ExtendedOctreeConverter.inc) equals this
+// whole-level form: the 2x2x2 groups never straddle a brick.
 template <typename T>
 __global__ __launch_bounds__(256) void downsample(const T* __restrict__ src, uint64_t SX, uint64_t SY, uint64_t SZ,
-                                                  T* __restrict__ dst, uint64_t DX, uint64_t DY, uint64_t DZ) {
+                                                  T* __restrict__ dst, uint64_t DX, uint64_t DY, uint64_t DZ,
+                                                  int median) {
   const uint64_t n = DX * DY * DZ;
+  const uint64_t fx = SX > 1 ? 2 : 1, fy = SY > 1 ? 2 : 1, fz = SZ > 1 ? 2 : 1;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
     const uint64_t x = i % DX, r = i / DX, y = r % DY, z = r / DY;
-    const uint64_t fx = SX > 1 ? 2 : 1, fy = SY > 1 ? 2 : 1, fz = SZ > 1 ? 2 : 1;
-    uint32_t sum = 0, cnt = 0;
-    for (uint64_t dz = 0; dz < fz; ++dz) {
-      const uint64_t zz = z * fz + dz;
-      if (zz >= SZ) continue;
+    T taps[8];
+    int cnt = 0;
+    for (uint64_t dx = 0; dx < fx; ++dx) {
+      const uint64_t xx = x * fx + dx;
+      if (xx >= SX) continue;
       for (uint64_t dy = 0; dy < fy; ++dy) {
         const uint64_t yy = y * fy + dy;
         if (yy >= SY) continue;
-        const T* row = src + SX * (yy + SY * zz);
-        for (uint64_t dx = 0; dx < fx; ++dx) {
-          const uint64_t xx = x * fx + dx;
-          if (xx >= SX) continue;
-          sum += row[xx];
-          ++cnt;
+        for (uint64_t dz = 0; dz < fz; ++dz) {
+          const uint64_t zz = z * fz + dz;
+          if (zz >= SZ) continue;
+          taps[cnt++] = src[xx + SX * (yy + SY * zz)];
         }
       }
     }
-    // double(sum)/cnt is exact (cnt is a power of two) and T() truncates
-    dst[i] = (T)(sum / cnt);
+    dst[i] = filter_taps(taps, cnt, median != 0);
+  }
+}
+
+// The reference's brick-to-brick filtering for odd brick cores
+// (ExtendedOctreeConverter::DownsampleBrick, ExtendedOctreeConverter.inc:250-356):
+// coarse brick B is cleared, then child 2B + d (d in {0,1}^3, if it exists)
+// filters its core into the coarse core at offset d * ceil(core / 2): child
+// voxel groups 2j, 2j+1 (only 2j in the last, odd plane) give coarse voxel
+// j + offset, so the second child's last voxel lands in the first overlap
+// voxel.  Positions no child writes stay 0.  The overlaps are FillOverlap's
+// (fill_overlap_wave with the pre-fill snapshot) afterwards.
+template <typename T>
+__global__ __launch_bounds__(256) void downsample_bricks(T* __restrict__ pool, const GatherBrick* __restrict__ coarse,
+                                                         const GatherBrick* __restrict__ fine, uint32_t fbx,
+                                                         uint32_t fby, uint32_t fbz, uint32_t ov, uint32_t sx,
+                                                         uint32_t sy, uint32_t sz, int median) {
+  const GatherBrick B = coarse[blockIdx.y];
+  const int lx = lx_of<T>();
+  const uint32_t nbx = nblocks(B.n[0], lx), nby = nblocks(B.n[1], 2);
+  const uint64_t n = padded_count(B.n, lx);
+  const uint32_t split[3] = {sx, sy, sz}, fcount[3] = {fbx, fby, fbz};
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    uint32_t p[3];
+    unswz(i, nbx, nby, lx, p[0], p[1], p[2]);
+    p[0] = min(p[0], B.n[0] - 1); p[1] = min(p[1], B.n[1] - 1); p[2] = min(p[2], B.n[2] - 1);
+    bool written = true;
+    uint32_t child[3], lj[3], ev[3];
+    for (int a = 0; a < 3 && written; ++a) {
+      if (p[a] < ov) { written = false; break; }
+      const uint32_t j = p[a] - ov;
+      uint32_t d = 0, l = j;
+      if (j >= split[a]) { d = 1; l = j - split[a]; }
+      child[a] = 2 * B.c[a] + d;
+      if (child[a] >= fcount[a]) { written = false; break; }
+      lj[a] = l;
+    }
+    T v = 0;
+    if (written) {
+      const GatherBrick F = fine[(child[2] * fby + child[1]) * fbx + child[0]];
+      uint32_t ntap[3];
+      for (int a = 0; a < 3 && written; ++a) {
+        ev[a] = (F.n[a] - 2 * ov) / 2;                     // evenSize of the child's core
+        const uint32_t contrib = ev[a] + (F.n[a] & 1u);    // groups, the odd plane included
+        if (lj[a] >= contrib) { written = false; break; }
+        ntap[a] = lj[a] < ev[a] ? 2u : 1u;
+      }
+      if (written) {
+        const uint32_t fnbx = nblocks(F.n[0], lx), fnby = nblocks(F.n[1], 2);
+        T taps[8];
+        int cnt = 0;
+        for (uint32_t dx = 0; dx < ntap[0]; ++dx)
+          for (uint32_t dy = 0; dy < ntap[1]; ++dy)
+            for (uint32_t dz = 0; dz < ntap[2]; ++dz)
+              taps[cnt++] = pool[F.offset + swz(2 * lj[0] + dx + ov, 2 * lj[1] + dy + ov, 2 * lj[2] + dz + ov, fnbx,
+                                                fnby, lx)];
+        v = filter_taps(taps, cnt, median != 0);
+      }
+    }
+    pool[B.offset + i] = v;
   }
 }
 
@@ -409,10 +510,15 @@ __global__ __launch_bounds__(256) void brick_max_gradient(const T* __restrict__ 
 // pre-fill state of the others, so the bricks with equal x+y+z can run at once:
 // one launch per such wave reproduces the sequential result exactly.  The pool
 // holds the closed-form bricks on entry; the cores never change.
+// snap (may be null): the level's bricks before the fill, at the pool's offsets
+// minus snap_base; the later neighbours are read there (any voxel), and a voxel
+// no copy covers keeps its value.  Without it the pre-fill overlaps are taken as
+// 0 (the even-core case: the closed-form bricks hold neighbour data there).
 template <typename T>
 __global__ __launch_bounds__(256) void fill_overlap_wave(T* __restrict__ pool, const GatherBrick* __restrict__ lod_bricks,
                                                          uint32_t bx, uint32_t by, uint32_t bz, uint32_t ov,
-                                                         const uint32_t* __restrict__ wave) {
+                                                         const uint32_t* __restrict__ wave,
+                                                         const T* __restrict__ snap, uint64_t snap_base) {
   const uint32_t me = wave[blockIdx.y];
   const GatherBrick B = lod_bricks[me];
   const uint32_t x = B.c[0], y = B.c[1], z = B.c[2];
@@ -456,6 +562,7 @@ __global__ __launch_bounds__(256) void fill_overlap_wave(T* __restrict__ pool, c
     if (hasR && hasK && xh && zh) sel = 7;
     if (hasB && hasK && yh && zh) sel = 8;
     if (hasR && hasB && hasK && xh && yh && zh) sel = 9;
+    if (sel < 0 && snap) continue;   // no copy: the pre-fill value stays
     T val = 0;
     if (sel >= 0) {
       const int dxs[10] = {1, 0, 0, -1, 0, 0, 1, 1, 0, 1};
@@ -470,8 +577,10 @@ __global__ __launch_bounds__(256) void fill_overlap_wave(T* __restrict__ pool, c
       const int pz = dz > 0 ? o + vz - (tz - o) : (dz < 0 ? nz - 2 * o + vz : vz);
       const bool earlier = dx < 0 || dy < 0 || dz < 0;   // already filled: final state
       const bool core = px >= o && px < nx - o && py >= o && py < ny - o && pz >= o && pz < nz - o;
-      if (earlier || core)
-        val = pool[N.offset + swz((uint32_t)px, (uint32_t)py, (uint32_t)pz, nblocks(N.n[0], lx), nblocks(N.n[1], 2), lx)];
+      const uint64_t at = N.offset + swz((uint32_t)px, (uint32_t)py, (uint32_t)pz, nblocks(N.n[0], lx), nblocks(N.n[1], 2), lx);
+      if (earlier) val = pool[at];
+      else if (snap) val = snap[at - snap_base];
+      else if (core) val = pool[at];
     }
     const uint64_t dst = B.offset + swz((uint32_t)vx, (uint32_t)vy, (uint32_t)vz, tbx, tby, lx);
     pool[dst] = val;
@@ -490,17 +599,38 @@ __global__ __launch_bounds__(256) void fill_overlap_wave(T* __restrict__ pool, c
 
 hipError_t launch_fill_overlap(void* pool, int bpv, const GatherBrick* d_lod_bricks, const uint32_t bricks[3],
                                uint32_t ov, const uint32_t* d_wave, uint32_t wave_n, uint32_t max_shell,
-                               hipStream_t st) {
+                               hipStream_t st, const void* snap, uint64_t snap_base) {
   uint32_t gx = (max_shell + 255) / 256;
   if (gx > 1024) gx = 1024;
   for (uint32_t first = 0; first < wave_n; first += 65535) {
     const uint32_t cnt = wave_n - first < 65535 ? wave_n - first : 65535;
     if (bpv == 1)
       hipLaunchKernelGGL(fill_overlap_wave<uint8_t>, dim3(gx, cnt), dim3(256), 0, st, (uint8_t*)pool, d_lod_bricks,
-                         bricks[0], bricks[1], bricks[2], ov, d_wave + first);
+                         bricks[0], bricks[1], bricks[2], ov, d_wave + first, (const uint8_t*)snap, snap_base);
     else
       hipLaunchKernelGGL(fill_overlap_wave<uint16_t>, dim3(gx, cnt), dim3(256), 0, st, (uint16_t*)pool,
-                         d_lod_bricks, bricks[0], bricks[1], bricks[2], ov, d_wave + first);
+                         d_lod_bricks, bricks[0], bricks[1], bricks[2], ov, d_wave + first, (const uint16_t*)snap,
+                         snap_base);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_downsample_bricks(void* pool, int bpv, const GatherBrick* d_coarse, uint32_t nb,
+                                    const GatherBrick* d_fine, const uint32_t fine_bricks[3], uint32_t ov,
+                                    const uint32_t split[3], uint64_t max_elems, int median, hipStream_t st) {
+  uint64_t gx = (max_elems + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  if (gx == 0) gx = 1;
+  for (uint32_t first = 0; first < nb; first += 65535) {
+    const uint32_t cnt = nb - first < 65535 ? nb - first : 65535;
+    if (bpv == 1)
+      hipLaunchKernelGGL(downsample_bricks<uint8_t>, dim3((uint32_t)gx, cnt), dim3(256), 0, st, (uint8_t*)pool,
+                         d_coarse + first, d_fine, fine_bricks[0], fine_bricks[1], fine_bricks[2], ov, split[0],
+                         split[1], split[2], median);
+    else
+      hipLaunchKernelGGL(downsample_bricks<uint16_t>, dim3((uint32_t)gx, cnt), dim3(256), 0, st, (uint16_t*)pool,
+                         d_coarse + first, d_fine, fine_bricks[0], fine_bricks[1], fine_bricks[2], ov, split[0],
+                         split[1], split[2], median);
   }
   return hipGetLastError();
 }
@@ -536,16 +666,16 @@ hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const
 }
 
 hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
-                             hipStream_t st) {
+                             hipStream_t st, int median) {
   const uint64_t n = dd[0] * dd[1] * dd[2];
   uint64_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (bpv == 1)
     hipLaunchKernelGGL(downsample<uint8_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint8_t*)src, sd[0], sd[1],
-                       sd[2], (uint8_t*)dst, dd[0], dd[1], dd[2]);
+                       sd[2], (uint8_t*)dst, dd[0], dd[1], dd[2], median);
   else
     hipLaunchKernelGGL(downsample<uint16_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint16_t*)src, sd[0],
-                       sd[1], sd[2], (uint16_t*)dst, dd[0], dd[1], dd[2]);
+                       sd[1], sd[2], (uint16_t*)dst, dd[0], dd[1], dd[2], median);
   return hipGetLastError();
 }
 

@@ -84,6 +84,10 @@ class HIPRaycaster:
         self.close()
 
     # -- dataset (RAWConverter / UVFDataset / BrickedDataset) ---------------
+    def SetLODFilter(self, median):
+        """bComputeMedian of ExtendedOctreeConverter::Convert for the next bricking."""
+        self._chk(self._L.tvk_set_lod_filter(self._h, int(bool(median))))
+
     def LoadVolume(self, vol, brick, overlap, clamp_to_edge=False, max_lods=0, keep_linear=False):
         vol = np.ascontiguousarray(vol)
         dt = {np.dtype(np.uint8): _abi.TVK_U8, np.dtype(np.uint16): _abi.TVK_U16}[vol.dtype]
