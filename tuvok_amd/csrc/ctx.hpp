@@ -70,8 +70,11 @@ struct PoolState {
   std::vector<uint64_t> slot_time;     // per slot: last use (raycaster) / creation time (traversal)
   std::vector<int32_t> brick_slot;     // per TOC brick: slot or -1
   uint64_t clock = 2;                  // 0: never used, 1: flagged empty (GLVolumePool)
-  // host store (pinned): bricks in TOC order, reference layout
+  // host store: bricks in TOC order, reference layout, in chunks of pinned
+  // memory while the host grants it, pageable memory beyond
   std::vector<void*> chunks;
+  std::vector<uint8_t> chunk_pinned;   // per chunk
+  bool pinned_refused = false;
   std::vector<uint8_t*> brick_host;    // per TOC brick
   std::vector<uint8_t> present;        // per TOC brick: payload handed over
   // device staging for the uploads

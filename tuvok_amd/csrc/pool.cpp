@@ -14,6 +14,7 @@
 //   the traversal's paging (GLVolumePool::UploadBricks, GLVolumePool.cpp:673-717,
 //       :955-959, :1720-1789) uses the same slots with its own policy (leap_api.cpp)
 #include <chrono>
+#include <cstdlib>
 #include "ctx.hpp"
 
 using namespace tvk;
@@ -35,7 +36,9 @@ GatherBrick gather_of(const Layout& L, size_t t, uint64_t offset) {
   return g;
 }
 
-// Pinned host memory for every brick of L in the reference layout.
+// Host memory for every brick of L in the reference layout: pinned chunks
+// (asynchronous uploads) until the host refuses more page-locked memory, then
+// pageable chunks (the runtime stages those uploads itself).
 int alloc_store(tvk_ctx* c, PoolState& P) {
   const Layout& L = c->L;
   P.brick_host.assign(L.toc.size(), nullptr);
@@ -50,8 +53,22 @@ int alloc_store(tvk_ctx* c, PoolState& P) {
       ++j;
     }
     void* p = nullptr;
-    HIPCHK(c, hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+    bool pinned = false;
+    if (!P.pinned_refused) {
+      if (hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault) == hipSuccess) {
+        pinned = true;
+      } else {
+        (void)hipGetLastError();   // clear the refusal
+        P.pinned_refused = true;
+        p = nullptr;
+      }
+    }
+    if (!pinned) {
+      p = std::malloc(std::max<size_t>(bytes, 1));
+      if (!p) return fail(c, TVK_ERR_NOMEM, "host memory for the brick store");
+    }
     P.chunks.push_back(p);
+    P.chunk_pinned.push_back(pinned ? 1 : 0);
     size_t off = 0;
     for (size_t k = i; k < j; ++k) {
       P.brick_host[k] = (uint8_t*)p + off;
@@ -137,7 +154,10 @@ int upload_to(tvk_ctx* c, PoolState* P, const std::vector<const uint8_t*>& src, 
 }  // namespace
 
 PoolState::~PoolState() {
-  for (void* p : chunks) (void)hipHostFree(p);
+  for (size_t i = 0; i < chunks.size(); ++i) {
+    if (i < chunk_pinned.size() && chunk_pinned[i]) (void)hipHostFree(chunks[i]);
+    else std::free(chunks[i]);
+  }
   if (d_stage) (void)hipFree(d_stage);
   if (d_desc) (void)hipFree(d_desc);
   if (d_srcoff) (void)hipFree(d_srcoff);
@@ -265,6 +285,8 @@ int pool_set_budget(tvk_ctx* c, uint64_t budget) {
     // keep the host store, re-make the slots
     PoolState& O = *c->pst;
     P->chunks.swap(O.chunks);
+    P->chunk_pinned.swap(O.chunk_pinned);
+    P->pinned_refused = O.pinned_refused;
     P->brick_host.swap(O.brick_host);
     P->present.swap(O.present);
   } else {
