@@ -27,6 +27,12 @@ class Raycaster {
     tvk_default_params(&p_);
     p_.lod = 0;
   }
+  // AbstrRenderer(mc, bUseOnlyPowerOfTwo, bDownSampleTo8Bits, bDisableBorder)
+  Raycaster(int device, bool pow2, bool to8, bool disable_border) : Raycaster(device) {
+    chk(tvk_set_compat(ctx_, pow2 ? 1 : 0, to8 ? 1 : 0));
+    p_.clamp_to_edge = disable_border ? 1 : 0;
+    push();
+  }
   ~Raycaster() { tvk_destroy(ctx_); }
   Raycaster(const Raycaster&) = delete;
   Raycaster& operator=(const Raycaster&) = delete;
@@ -48,6 +54,24 @@ class Raycaster {
   }
   // GPUMemMan's memory budget for this dataset's bricks (0: all resident)
   void SetPoolBudget(uint64_t bytes) { chk(tvk_set_pool_budget(ctx_, bytes)); }
+  // ExtendedOctreeConverter's bComputeMedian for the next bricking
+  void SetLODFilter(bool median) { chk(tvk_set_lod_filter(ctx_, median ? 1 : 0)); }
+  // SetClipPlane + EnableClipPlane / DisableClipPlane (AbstrRenderer.h:215-219)
+  void SetClipPlane(const std::array<float, 4>& plane, bool enabled = true) {
+    chk(tvk_set_clip_plane(ctx_, plane.data(), enabled ? 1 : 0));
+  }
+  // AbstrRenderer::CropDataset (AbstrRenderer.h:480)
+  std::array<float, 4> CropDataset(bool keep_old = false) {
+    std::array<float, 4> used{};
+    chk(tvk_crop_dataset(ctx_, keep_old ? 1 : 0, used.data()));
+    return used;
+  }
+  // AbstrRenderer::Pick (AbstrRenderer.h:245): the isosurface hit under a window position
+  std::array<float, 3> Pick(uint32_t x, uint32_t y) {
+    std::array<float, 3> pos{};
+    chk(tvk_pick(ctx_, x, y, pos.data()));
+    return pos;
+  }
   tvk_pool_stats PoolStats() {
     tvk_pool_stats s{};
     chk(tvk_get_pool_stats(ctx_, &s));
