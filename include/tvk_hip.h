@@ -215,6 +215,29 @@ typedef struct {
 } tvk_brick_key;
 int tvk_set_bricks(tvk_ctx* ctx, int dtype, const uint64_t dims[3], const uint32_t brick[3], uint32_t overlap,
                    int lod_count, const tvk_brick_key* keys, size_t n, uint64_t budget_bytes);
+/* A UVF file's ExtendedOctree dataset into the pool: UVFDataset's open of a
+ * version-5 file (IO/uvfDataset.cpp; UVF::Open, IO/UVF/UVF.cpp:74-305) with the
+ * octree's header, TOC and bricks as ExtendedOctree::Open / GetBrickData read
+ * them (IO/UVF/ExtendedOctree/ExtendedOctree.cpp:87-165, :313-360): scalar
+ * 8- or 16-bit voxels, bricks uncompressed or zlib-compressed, not stored as
+ * 2D atlases (TVK_ERR_UNSUPPORTED otherwise).  The bricks go through
+ * tvk_set_bricks / tvk_upload_brick with budget_bytes (0: every brick in HBM),
+ * with each brick's min/max taken from its stored voxels; the dataset's
+ * voxel aspect becomes its scale.  info (may be NULL) describes the file. */
+typedef struct {
+  uint64_t file_version;   /* GlobalHeader ulFileVersion                         */
+  uint32_t octree_version; /* ExtendedOctree m_iVersion                          */
+  uint64_t octree_offset;  /* byte offset of the octree header in the file      */
+  uint64_t dims[3];        /* m_vVolumeSize                                      */
+  uint32_t brick_size[3];  /* m_iBrickSize (stored, incl. overlap)               */
+  double aspect[3];        /* m_vVolumeAspect                                    */
+  uint32_t overlap;
+  int bit_width;
+  int lod_count;
+  uint64_t brick_count;
+  uint64_t zlib_bricks;    /* bricks stored zlib-compressed                     */
+} tvk_uvf_info;
+int tvk_open_uvf(tvk_ctx* ctx, const char* path, uint64_t budget_bytes, tvk_uvf_info* info);
 /* One brick's payload, Dataset::GetBrick (IO/Dataset.h:93-100): the reference
  * layout, x fastest, stored size incl. overlap (bytes must match). */
 int tvk_upload_brick(tvk_ctx* ctx, uint32_t lod, uint64_t index, const void* data, size_t bytes);

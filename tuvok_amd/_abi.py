@@ -29,7 +29,7 @@ EXPORTS = [
     "tvk_leap_page_in", "tvk_leap_resume_state",
     "tvk_set_bricks", "tvk_upload_brick", "tvk_set_pool_budget", "tvk_get_pool_stats",
     "tvk_leap_set_request_table", "tvk_leap_requests", "tvk_pool_resident",
-    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat", "tvk_set_lod_filter",
+    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat", "tvk_set_lod_filter", "tvk_open_uvf",
 ]
 
 
@@ -77,6 +77,13 @@ class PoolStats(C.Structure):
     _fields_ = [("pool_bytes", C.c_uint64), ("slots", C.c_uint64), ("slot_bytes", C.c_uint64),
                 ("resident", C.c_uint64), ("uploads", C.c_uint64), ("evictions", C.c_uint64),
                 ("upload_bytes", C.c_uint64), ("upload_ms", C.c_float), ("parts", C.c_uint32)]
+
+
+class UvfInfo(C.Structure):
+    _fields_ = [("file_version", C.c_uint64), ("octree_version", C.c_uint32), ("octree_offset", C.c_uint64),
+                ("dims", C.c_uint64 * 3), ("brick_size", C.c_uint32 * 3), ("aspect", C.c_double * 3),
+                ("overlap", C.c_uint32), ("bit_width", C.c_int), ("lod_count", C.c_int),
+                ("brick_count", C.c_uint64), ("zlib_bricks", C.c_uint64)]
 
 
 class GradientStop(C.Structure):
@@ -133,6 +140,7 @@ def load(path=LIB_PATH):
         "tvk_set_clip_plane": (C.c_int, [vp, fp, C.c_int]),
         "tvk_set_compat": (C.c_int, [vp, C.c_int, C.c_int]),
         "tvk_set_lod_filter": (C.c_int, [vp, C.c_int]),
+        "tvk_open_uvf": (C.c_int, [vp, C.c_char_p, C.c_uint64, C.c_void_p]),
         "tvk_crop_dataset": (C.c_int, [vp, C.c_int, fp]),
         "tvk_pick": (C.c_int, [vp, C.c_uint32, C.c_uint32, fp]),
         "tvk_get_view_matrices": (C.c_int, [vp, fp, fp]),

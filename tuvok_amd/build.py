@@ -33,6 +33,7 @@ SOURCES = [
     ("noalloc.hip", ["-ffp-contract=off"]),
     ("pool.cpp", ["-ffp-contract=off"]),
     ("host/tf2d.cpp", ["-ffp-contract=off"]),
+    ("host/uvf.cpp", []),
 ]
 
 COMMON = [
@@ -62,7 +63,7 @@ def build(verbose=True):
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-fPIC"]
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-fPIC", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

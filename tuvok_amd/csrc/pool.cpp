@@ -545,6 +545,7 @@ int tvk_set_bricks(tvk_ctx* c, int dtype, const uint64_t dims[3], const uint32_t
   c->pool_bytes = c->lin_bytes = 0;
   c->has_ds = false;
   c->leap_resident.clear();
+  c->cp = tvk_ctx::CompatPool{};
   c->L = std::move(fresh);
   c->clamp = 0;
   c->minmax = std::move(mm);

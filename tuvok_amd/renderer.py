@@ -84,6 +84,17 @@ class HIPRaycaster:
         self.close()
 
     # -- dataset (RAWConverter / UVFDataset / BrickedDataset) ---------------
+    def OpenUVF(self, path, budget_bytes=0):
+        """A UVF file's ExtendedOctree dataset into the pool (0: every brick in HBM);
+        returns the file's description.  The dataset's aspect becomes the
+        domain scale (vDomainScale = 1 / aspect)."""
+        info = _abi.UvfInfo()
+        self._chk(self._L.tvk_open_uvf(self._h, str(path).encode(), int(budget_bytes), C.byref(info)))
+        for k in range(3):
+            self._p.domain_scale[k] = 1.0 / info.aspect[k]
+        self._chk(self._L.tvk_set_params(self._h, C.byref(self._p)))
+        return info
+
     def SetLODFilter(self, median):
         """bComputeMedian of ExtendedOctreeConverter::Convert for the next bricking."""
         self._chk(self._L.tvk_set_lod_filter(self._h, int(bool(median))))
