@@ -82,6 +82,44 @@ def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
                 view0_parity=parity)
 
 
+def hbm_roofline(alg_bytes, ms):
+    """Algorithmic bytes of one frame (or batch) over its kernel time against
+    MI355X's HBM peak."""
+    gbs = alg_bytes / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "algorithmic_bytes": int(alg_bytes), "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)}
+
+
+def batch_bytes(requests):
+    """V_eff of a batch, deduplicated per dataset: the union of the bricks each
+    request's counted render samples (stored bytes), plus 16 B of output and
+    4 B of descriptors per pixel.  requests: (renderer, dataset owner key,
+    pixels); every renderer has just run a counted render."""
+    union, sizes, px = {}, {}, 0
+    for r, key, pixels in requests:
+        m = r.TouchedMask().astype(bool)
+        union[key] = union.get(key, np.zeros_like(m)) | m
+        if key not in sizes:
+            sizes[key] = r.BrickStoredBytes()
+        px += pixels
+    return int(sum(int(sizes[k][union[k]].sum()) for k in union)) + 20 * px
+
+
+def counted_batch_bytes(rens, reqs):
+    """batch_bytes of a batch's requests, each counted by one untimed render of
+    its renderer (which holds the request's camera and size).  reqs: (dataset
+    key, rotation, width, height) per renderer."""
+    items = []
+    for r, (key, _rot, w, h) in zip(rens, reqs):
+        r.SetCountSamples(True)
+        r.Paint(download=False)
+        items.append((r, key, w * h))
+    b = batch_bytes(items)
+    for r in rens:
+        r.SetCountSamples(False)
+    return b
+
+
 def other_workloads(learned=True):
     """Frame times of the other full-size workloads (reported beside the main
     line, not part of its value): device bricking of the 1024^3 uint16 volume
@@ -137,14 +175,18 @@ def other_workloads(learned=True):
     try:
         c = workloads.SDF_ISO_2048
         r = workloads.setup(c)
+        r.SetCountSamples(True)
         r.Paint(download=False)
+        alg = batch_bytes([(r, 0, c["width"] * c["height"])])
+        r.SetCountSamples(False)
         ms = []
         for _ in range(5):
             r.Paint(download=False)
             ms.append(r.last_stats.ms)
         r.close()
         m = float(np.median(ms))
-        res["iso_sdf2048"] = dict(workload=c["name"], ms_per_frame=round(m, 3), frames_per_s=round(1e3 / m, 1))
+        res["iso_sdf2048"] = dict(workload=c["name"], ms_per_frame=round(m, 3), frames_per_s=round(1e3 / m, 1),
+                                  roofline=hbm_roofline(alg, m))
     except Exception as e:
         res["iso_sdf2048"] = dict(error=f"{type(e).__name__}: {e}")
     print(f"[bench] iso: {res['iso_sdf2048']}", file=sys.stderr, flush=True)
@@ -156,8 +198,9 @@ def other_workloads(learned=True):
         ms = [batch.Render().ms for _ in range(5)]
         px = sum(w * h for (_, _, _, w, h) in reqs)
         m = float(np.median(ms))
+        alg = counted_batch_bytes(rens, [(q[0], q[2], q[3], q[4]) for q in reqs])
         res["batch48"] = dict(requests=len(reqs), mpixels=round(px / 1e6, 2), ms_per_frame=round(m, 3),
-                              mpixels_per_s=round(px / m / 1e3, 1), tuning=best)
+                              mpixels_per_s=round(px / m / 1e3, 1), tuning=best, roofline=hbm_roofline(alg, m))
         batch.close()
         for x in rens + list(owners.values()):
             x.close()
@@ -174,9 +217,10 @@ def other_workloads(learned=True):
         ms = [batch.Render().ms for _ in range(3)]
         px = sum(w * h for (_, _, w, h) in reqs)
         m = float(np.median(ms))
+        alg = counted_batch_bytes(rens, [(0, rot, w, h) for (_, rot, w, h) in reqs])
         res["lod64g"] = dict(requests=len(reqs), mpixels=round(px / 1e6, 2), ms_per_frame=round(m, 3),
                              mpixels_per_s=round(px / m / 1e3, 1), pool_gib=round(d.pool_bytes / 2 ** 30, 1),
-                             levels=sorted(set(int(x.CurrentLOD()) for x in rens)))
+                             levels=sorted(set(int(x.CurrentLOD()) for x in rens)), roofline=hbm_roofline(alg, m))
         batch.close()
         for x in rens:
             x.close()

@@ -337,6 +337,9 @@ int tvk_set_count_samples(tvk_ctx* ctx, int on);
  * brick once). */
 int tvk_touched_bytes(tvk_ctx* ctx, uint64_t* bytes, uint64_t* bricks, uint64_t* unique_bytes,
                       uint64_t* unique_bricks);
+/* Per TOC brick: 1 where the last counted render (any of its views) took a
+ * sample in the brick (V_eff, the bricks a batch's requests deduplicate). */
+int tvk_touched_mask(tvk_ctx* ctx, uint8_t* mask, size_t n);
 /* Allocation-free variant of tvk_render_views.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory while they
  * render (HIP graph capture, tight serving loops, harnesses that forbid

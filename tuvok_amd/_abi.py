@@ -29,7 +29,7 @@ EXPORTS = [
     "tvk_leap_page_in", "tvk_leap_resume_state",
     "tvk_set_bricks", "tvk_upload_brick", "tvk_set_pool_budget", "tvk_get_pool_stats",
     "tvk_leap_set_request_table", "tvk_leap_requests", "tvk_pool_resident",
-    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat", "tvk_set_lod_filter", "tvk_open_uvf",
+    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat", "tvk_set_lod_filter", "tvk_open_uvf", "tvk_touched_mask",
 ]
 
 
@@ -141,6 +141,7 @@ def load(path=LIB_PATH):
         "tvk_set_compat": (C.c_int, [vp, C.c_int, C.c_int]),
         "tvk_set_lod_filter": (C.c_int, [vp, C.c_int]),
         "tvk_open_uvf": (C.c_int, [vp, C.c_char_p, C.c_uint64, C.c_void_p]),
+        "tvk_touched_mask": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_size_t]),
         "tvk_crop_dataset": (C.c_int, [vp, C.c_int, fp]),
         "tvk_pick": (C.c_int, [vp, C.c_uint32, C.c_uint32, fp]),
         "tvk_get_view_matrices": (C.c_int, [vp, fp, fp]),

@@ -1559,6 +1559,20 @@ int tvk_touched_bytes(tvk_ctx* c, uint64_t* bytes, uint64_t* bricks, uint64_t* u
   return TVK_OK;
 }
 
+int tvk_touched_mask(tvk_ctx* c, uint8_t* mask, size_t n) {
+  if (!c || !mask) return TVK_ERR_ARG;
+  if (!c->touched_n) return fail(c, TVK_ERR_STATE, "no counted render yet");
+  const size_t nt = c->L.toc.size();
+  if (n != nt) return fail(c, TVK_ERR_ARG, "mask size must equal the brick count");
+  std::vector<uint32_t> t(c->touched_n);
+  HIPCHK(c, hipMemcpyAsync(t.data(), c->d_touched, t.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::memset(mask, 0, n);
+  for (size_t i = 0; i < t.size(); ++i)
+    if (t[i]) mask[i % nt] = 1;
+  return TVK_OK;
+}
+
 int tvk_display(tvk_ctx* c, const float bg_top[3], const float bg_bottom[3], uint8_t* out, size_t bytes) {
   if (!c || !bg_top || !bg_bottom || !out) return TVK_ERR_ARG;
   if (c->last_views < 1 || !c->d_out) return fail(c, TVK_ERR_STATE, "no rendered frame");

@@ -546,6 +546,22 @@ class HIPRaycaster:
         self._chk(self._L.tvk_touched_bytes(self._h, C.byref(b), C.byref(n), C.byref(ub), C.byref(un)))
         return (ub.value, un.value) if unique else (b.value, n.value)
 
+    def TouchedMask(self):
+        """Per TOC brick: 1 where the last counted render took a sample."""
+        n = self.GetDatasetDesc().brick_count
+        m = np.zeros(n, np.uint8)
+        self._chk(self._L.tvk_touched_mask(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8)), n))
+        return m
+
+    def BrickStoredBytes(self):
+        """Per TOC brick: stored voxels (incl. overlap) x bytes per voxel."""
+        d = self.GetDatasetDesc()
+        out = np.zeros(d.brick_count, np.uint64)
+        bpv = d.bit_width // 8
+        for i in range(d.brick_count):
+            out[i] = int(np.prod(self.GetBrickInfo(i)["size"])) * bpv
+        return out
+
     # -- timing ---------------------------------------------------------------------
     def EventRecord(self, slot):
         self._chk(self._L.tvk_event_record(self._h, slot))
