@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ERT_EPS = 2.0 ** -11
+DISPATCH_NAMES = {0: "raster", 1: "groups", 2: "rows"}
 
 
 def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
@@ -173,8 +174,10 @@ def other_workloads(learned=True):
     print(f"[bench] bricking: {res.get('bricking_1024')} leaper: {res.get('leaper_1024')}", file=sys.stderr,
           flush=True)
     try:
+        from tuvok_amd.autotune import tune_dispatch_order
         c = workloads.SDF_ISO_2048
         r = workloads.setup(c)
+        iso_order, _ = tune_dispatch_order(r, [np.eye(4, dtype=np.float32)])
         r.SetCountSamples(True)
         r.Paint(download=False)
         alg = batch_bytes([(r, 0, c["width"] * c["height"])])
@@ -186,7 +189,7 @@ def other_workloads(learned=True):
         r.close()
         m = float(np.median(ms))
         res["iso_sdf2048"] = dict(workload=c["name"], ms_per_frame=round(m, 3), frames_per_s=round(1e3 / m, 1),
-                                  roofline=hbm_roofline(alg, m))
+                                  dispatch_order=DISPATCH_NAMES[iso_order], roofline=hbm_roofline(alg, m))
     except Exception as e:
         res["iso_sdf2048"] = dict(error=f"{type(e).__name__}: {e}")
     print(f"[bench] iso: {res['iso_sdf2048']}", file=sys.stderr, flush=True)
@@ -250,6 +253,7 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     views at 480x270, training on the device, calibration on 10 other views
     (bound 0.75/255, the fixed march a candidate), then `views` timed with and
     without the proposal and compared pixel by pixel."""
+    from tuvok_amd.autotune import tune_dispatch_order
     from tuvok_amd.stepnet import calibrate, train_proposal
     from tuvok_amd.renderer import rotation_y
     rng = np.random.default_rng(seed)
@@ -266,13 +270,15 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     for (p, e, ms) in sorted(tried, key=lambda t: t[2])[:6]:
         print(f"[bench] calibration: {p} max err {e * 255:.3f}/255 {ms:.2f} ms", file=sys.stderr, flush=True)
     ren.SetStepProposal(None)
+    orders = {"fixed": tune_dispatch_order(ren, views)[0]}
     fixed = ren.PaintViews(views)
     t_fixed = []
     for _ in range(3):
         ren.PaintViews(views, download=False)
         t_fixed.append(ren.last_stats.ms)
-    def timed(prop, use_net=True):
+    def timed(prop, use_net=True, label=""):
         ren.SetStepProposal(net if (prop is not None and use_net) else None, **(prop or {}))
+        orders[label] = tune_dispatch_order(ren, views)[0]
         img = ren.PaintViews(views)
         ts = []
         for _ in range(3):
@@ -282,10 +288,11 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
 
     # the chunked march with the exact skip only (multiplier 1, no network
     # evaluated): what the method gives without the network
-    exact, ems = timed(dict(fixed_log2=0, skip_bound=0.0), use_net=False)
-    learned, lms = timed(prop)   # prop None: the calibration kept the fixed march
+    exact, ems = timed(dict(fixed_log2=0, skip_bound=0.0), use_net=False, label="exact_skip")
+    learned, lms = timed(prop, label="learned")   # prop None: the calibration kept the fixed march
     ren.SetStepProposal(None)
     ren.SetSampleRateModifier(1.0)
+    ren.SetDispatchOrder(0)
     d = np.abs(learned - fixed)
     fms = float(np.median(t_fixed))
     return dict(reference_step_voxels=0.25, views=len(views), fixed_fps=round(len(views) / (fms / 1e3), 2),
@@ -294,7 +301,8 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
                 exact_skip_max_err=float(np.abs(exact - fixed).max()),
                 max_err=float(d.max()), mean_err=float(d.mean()), max_err_x255=round(float(d.max()) * 255, 3),
                 proposal=prop if prop is not None else "fixed march (no proposal setting faster within the bound)",
-                train_s=round(train_s, 1), train=info)
+                train_s=round(train_s, 1), train=info,
+                dispatch_order={k: DISPATCH_NAMES[v] for k, v in orders.items()})
 
 
 PROFILE = os.path.join(ROOT, "profiles", "march_fbm1024_lit2dtf_1080p_x16.md")
@@ -381,6 +389,11 @@ def main():
     print(f"[bench] N_ref={n_ref} executed={n_exec} touched bricks={n_touched} ({b_bricks / 1e9:.2f} GB over "
           f"views, {u_bricks} bricks / {u_bytes / 1e9:.2f} GB distinct)", file=sys.stderr, flush=True)
 
+    # the group order of the launch (frames are the same in every order)
+    from tuvok_amd.autotune import tune_dispatch_order
+    order, order_ms = tune_dispatch_order(ren, views)
+    print(f"[bench] dispatch order: {DISPATCH_NAMES[order]} "
+          f"({', '.join(f'{DISPATCH_NAMES[k]} {v:.2f} ms' for k, v in order_ms.items())})", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ren.PaintViews(views, download=False)
     ren.Synchronize()
@@ -451,7 +464,7 @@ def main():
         "config": {"workload": f"fbm_shells{n}_u16_2dtf_lit_1920x1080_x{nviews}views",
                    "volume": list(dims), "brick_stored": list(cfg["brick"]), "overlap": cfg["overlap"],
                    "mode": "2D TF + lighting", "step_voxels": "0.5 (fRayStepsize)", "views": nviews,
-                   "ert_eps": ERT_EPS},
+                   "ert_eps": ERT_EPS, "dispatch_order": DISPATCH_NAMES[order]},
         "frames_per_s": round(nviews / (ms_step / 1e3), 2),
         "samples_ref": n_ref,
         "samples_executed": n_exec,

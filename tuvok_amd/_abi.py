@@ -9,6 +9,7 @@ TVK_OK = 0
 TVK_U8, TVK_U16 = 1, 2
 RM_1DTRANS, RM_2DTRANS, RM_ISOSURFACE = 0, 1, 2
 GEN_BLOBS, GEN_FBM_SHELLS, GEN_ELLIPSOID_SDF, GEN_PHANTOM = 1, 2, 3, 4
+DISPATCH_RASTER, DISPATCH_GROUPS, DISPATCH_ROWS = 0, 1, 2
 
 # every symbol include/tvk_hip.h declares
 EXPORTS = [
@@ -29,7 +30,7 @@ EXPORTS = [
     "tvk_leap_page_in", "tvk_leap_resume_state",
     "tvk_set_bricks", "tvk_upload_brick", "tvk_set_pool_budget", "tvk_get_pool_stats",
     "tvk_leap_set_request_table", "tvk_leap_requests", "tvk_pool_resident",
-    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat", "tvk_set_lod_filter", "tvk_open_uvf", "tvk_touched_mask",
+    "tvk_rasterize_tf2d", "tvk_set_tf2d_swatches", "tvk_set_clip_plane", "tvk_crop_dataset", "tvk_pick", "tvk_set_compat", "tvk_set_lod_filter", "tvk_set_dispatch_order", "tvk_open_uvf", "tvk_touched_mask",
 ]
 
 
@@ -140,6 +141,7 @@ def load(path=LIB_PATH):
         "tvk_set_clip_plane": (C.c_int, [vp, fp, C.c_int]),
         "tvk_set_compat": (C.c_int, [vp, C.c_int, C.c_int]),
         "tvk_set_lod_filter": (C.c_int, [vp, C.c_int]),
+        "tvk_set_dispatch_order": (C.c_int, [vp, C.c_int]),
         "tvk_open_uvf": (C.c_int, [vp, C.c_char_p, C.c_uint64, C.c_void_p]),
         "tvk_touched_mask": (C.c_int, [vp, C.POINTER(C.c_uint8), C.c_size_t]),
         "tvk_crop_dataset": (C.c_int, [vp, C.c_int, fp]),

@@ -324,6 +324,7 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_tf2cols) (void)hipFree(c->d_tf2cols);
   if (c->d_list) (void)hipFree(c->d_list);
   if (c->d_views) (void)hipFree(c->d_views);
+  if (c->d_order) (void)hipFree(c->d_order);
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->d_counter) (void)hipFree(c->d_counter);
   if (c->d_touched) (void)hipFree(c->d_touched);
@@ -331,6 +332,7 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_hitpos) (void)hipFree(c->d_hitpos);
   if (c->h_list_pinned) (void)hipHostFree(c->h_list_pinned);
   if (c->h_views_pinned) (void)hipHostFree(c->h_views_pinned);
+  if (c->h_order_pinned) (void)hipHostFree(c->h_order_pinned);
   for (auto& ev : c->ev) if (ev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -556,6 +558,14 @@ int tvk_crop_dataset(tvk_ctx* c, int keep_old, float plane_used[4]) {
 int tvk_set_lod_filter(tvk_ctx* c, int median) {
   if (!c) return TVK_ERR_ARG;
   c->lod_median = median ? 1 : 0;
+  return TVK_OK;
+}
+
+int tvk_set_dispatch_order(tvk_ctx* c, int order) {
+  if (!c) return TVK_ERR_ARG;
+  if (order < TVK_DISPATCH_RASTER || order > TVK_DISPATCH_ROWS)
+    return fail(c, TVK_ERR_ARG, "dispatch order must be TVK_DISPATCH_RASTER, _GROUPS or _ROWS");
+  c->dispatch = order;
   return TVK_OK;
 }
 
@@ -1324,22 +1334,39 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   }
   if (pixels >= (1ull << 32)) return fail(c, TVK_ERR_ARG, "too many pixels in one call");
   if (fixed) return launch_fixed(c, nviews, pixels, groups, in_list, *fixed, st);
+  // the dispatch order (raster: none)
+  const bool ordered = c->dispatch != TVK_DISPATCH_RASTER && groups > 0;
+  uint32_t xcd_start[8] = {}, xcd_count[8] = {};
+  if (ordered) {
+    c->h_cost.assign((size_t)groups, 0.0f);
+    for (int v = 0; v < nviews; ++v) {
+      const ViewConst& vc = c->h_views[(size_t)v];
+      group_costs(vc, c->h_list.data() + vc.list_offset, c->h_cost.data() + vc.group_offset);
+    }
+    dispatch_order(c->h_views.data(), nviews, c->h_cost, c->dispatch, c->h_order, xcd_start, xcd_count);
+  }
   int rc = ensure(c, &c->d_list, &c->list_cap, std::max<size_t>(c->h_list.size(), 1));
   if (!rc) rc = ensure(c, &c->d_views, &c->views_cap, (size_t)nviews);
+  if (!rc && ordered) rc = ensure(c, &c->d_order, &c->order_cap, (size_t)groups);
   if (!rc) rc = ensure(c, &c->d_out, &c->out_cap, (size_t)pixels * 4);
   if (!rc) rc = ensure_pinned(c, &c->h_list_pinned, &c->h_list_pinned_cap, std::max<size_t>(c->h_list.size(), 1));
   if (!rc) rc = ensure_pinned(c, &c->h_views_pinned, &c->h_views_pinned_cap, (size_t)nviews);
+  if (!rc && ordered) rc = ensure_pinned(c, &c->h_order_pinned, &c->h_order_pinned_cap, (size_t)groups);
   if (rc) return rc;
   // staging through pinned memory keeps the uploads asynchronous; wait for the
   // previous frame's uploads before overwriting the staging buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
   std::copy(c->h_list.begin(), c->h_list.end(), c->h_list_pinned);
   std::copy(c->h_views.begin(), c->h_views.end(), c->h_views_pinned);
+  if (ordered) std::copy(c->h_order.begin(), c->h_order.end(), c->h_order_pinned);
   if (!c->h_list.empty())
     HIPCHK(c, hipMemcpyAsync(c->d_list, c->h_list_pinned, c->h_list.size() * sizeof(FrameBrick), hipMemcpyHostToDevice,
                              c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_views, c->h_views_pinned, (size_t)nviews * sizeof(ViewConst), hipMemcpyHostToDevice,
                            c->stream));
+  if (ordered)
+    HIPCHK(c, hipMemcpyAsync(c->d_order, c->h_order_pinned, (size_t)groups * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             c->stream));
   MarchArgs a{};
   fill_shade(c, a.S);
   a.list = c->d_list;
@@ -1374,6 +1401,16 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   a.lighting = c->P.lighting;
   a.groups_total = groups;
   a.groups_padded = (groups + 7u) & ~7u;
+  if (ordered) {
+    a.order = c->d_order;
+    uint32_t most = 0;
+    for (int x = 0; x < 8; ++x) {
+      a.xcd_start[x] = xcd_start[x];
+      a.xcd_count[x] = xcd_count[x];
+      most = std::max(most, xcd_count[x]);
+    }
+    a.groups_padded = 8u * most;
+  }
   if (c->count) HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
   if (a.groups_total > 0) HIPCHK(c, launch_march(a, render_bpv(c), c->stream, learn));

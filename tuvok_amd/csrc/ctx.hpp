@@ -148,6 +148,8 @@ struct tvk_ctx {
   size_t list_cap = 0;
   ViewConst* d_views = nullptr;
   size_t views_cap = 0;
+  uint32_t* d_order = nullptr;   // dispatch order of the 16x16 groups
+  size_t order_cap = 0;
   float* d_out = nullptr;
   size_t out_cap = 0;
   unsigned long long* d_counter = nullptr;
@@ -196,6 +198,10 @@ struct tvk_ctx {
   size_t h_list_pinned_cap = 0;
   ViewConst* h_views_pinned = nullptr;
   size_t h_views_pinned_cap = 0;
+  std::vector<float> h_cost;
+  std::vector<uint32_t> h_order;
+  uint32_t* h_order_pinned = nullptr;
+  size_t h_order_pinned_cap = 0;
   int last_views = 0;
   uint32_t last_w = 0, last_h = 0;
   int last_lod = 0;
@@ -204,6 +210,7 @@ struct tvk_ctx {
   // second pool of padded and/or quantized bricks that rendering reads, made
   // on first use after the dataset or the flags change
   int compat_pow2 = 0, compat_8bit = 0;
+  int dispatch = 0;     // TVK_DISPATCH_* of multi-view renders
   int lod_median = 0;   // ExtendedOctreeConverter's bComputeMedian for the next bricking
   struct CompatPool {
     void* d = nullptr;

@@ -314,6 +314,93 @@ void make_view_const(const ViewState& V, ViewConst& vc) {
   vc.groups = vc.groups_x * ((V.H + 15) / 16);
 }
 
+void group_costs(const ViewConst& vc, const FrameBrick* list, float* cost) {
+  const int gx = (int)vc.groups_x, gy = gx > 0 ? (int)(vc.groups / vc.groups_x) : 0;
+  if (gx <= 0 || gy <= 0) return;
+  // 2D difference array over the groups, then prefix sums
+  std::vector<double> d((size_t)(gx + 1) * (size_t)(gy + 1), 0.0);
+  auto at = [&](int x, int y) -> double& { return d[(size_t)y * (size_t)(gx + 1) + (size_t)x]; };
+  double unit = 0.0;   // the largest brick weight
+  for (uint32_t i = 0; i < vc.nlist; ++i) {
+    const FrameBrick& f = list[i];
+    if (f.rect[2] < f.rect[0] || f.rect[3] < f.rect[1] || !(f.ray_step > 0.0f)) continue;
+    const int x0 = std::min(std::max((int)f.rect[0], 0) / 16, gx - 1), x1 = std::min(std::max((int)f.rect[2], 0) / 16, gx - 1);
+    const int y0 = std::min(std::max((int)f.rect[1], 0) / 16, gy - 1), y1 = std::min(std::max((int)f.rect[3], 0) / 16, gy - 1);
+    const float ext = std::max(f.bmax[0] - f.bmin[0], std::max(f.bmax[1] - f.bmin[1], f.bmax[2] - f.bmin[2]));
+    const double w = (double)(ext / f.ray_step);
+    if (!(w > 0.0) || !std::isfinite(w)) continue;
+    unit = std::max(unit, w);
+    at(x0, y0) += w;
+    at(x1 + 1, y0) -= w;
+    at(x0, y1 + 1) -= w;
+    at(x1 + 1, y1 + 1) += w;
+  }
+  const double base = 0.05 * unit;
+  // in-place 2D prefix sums: the summed weight of the rectangles covering each group
+  for (int y = 0; y < gy; ++y)
+    for (int x = 0; x < gx; ++x) {
+      double v = at(x, y);
+      if (x > 0) v += at(x - 1, y);
+      if (y > 0) v += at(x, y - 1);
+      if (x > 0 && y > 0) v -= at(x - 1, y - 1);
+      at(x, y) = v;
+      cost[(size_t)y * (size_t)gx + (size_t)x] = (float)(std::max(v, 0.0) + base);
+    }
+}
+
+void dispatch_order(const ViewConst* views, int nviews, const std::vector<float>& cost, int mode,
+                    std::vector<uint32_t>& order, uint32_t start[8], uint32_t count[8]) {
+  // units: one group (mode 1) or one row of a view's groups (mode 2), in
+  // launch order; a unit covers groups first .. first + n - 1
+  struct Unit { float cost; uint32_t first, n; };
+  std::vector<Unit> units;
+  size_t ngroups = 0;
+  for (int v = 0; v < nviews; ++v) {
+    const ViewConst& vc = views[v];
+    const uint32_t gx = vc.groups_x, gy = gx > 0 ? vc.groups / gx : 0;
+    for (uint32_t y = 0; y < gy; ++y) {
+      const uint32_t row = vc.group_offset + y * gx;
+      if (mode == 2) {
+        double sum = 0.0;
+        for (uint32_t x = 0; x < gx; ++x) sum += cost[row + x];
+        units.push_back(Unit{(float)sum, row, gx});
+      } else {
+        for (uint32_t x = 0; x < gx; ++x) units.push_back(Unit{cost[row + x], row + x, 1u});
+      }
+    }
+    ngroups += (size_t)gx * gy;
+  }
+  order.clear();
+  order.reserve(ngroups);
+  // 8 contiguous ranges of about ngroups / 8 groups; each range's units by
+  // decreasing cost, bucketed into 1024 levels of its largest cost (ties and
+  // equal buckets in launch order)
+  constexpr int NB = 1024;
+  std::vector<uint32_t> bucket_n(NB + 1);
+  std::vector<uint32_t> idx;
+  size_t b = 0, acc = 0;
+  for (int x = 0; x < 8; ++x) {
+    start[x] = (uint32_t)order.size();
+    const size_t b0 = b;
+    const double target = (double)ngroups * (double)(x + 1) / 8.0;
+    while (b < units.size() && (x == 7 || (double)acc + 0.5 * (double)units[b].n < target)) acc += units[b++].n;
+    float top = 0.0f;
+    for (size_t i = b0; i < b; ++i) top = std::max(top, units[i].cost);
+    auto level = [&](float c) {
+      const int l = top > 0.0f ? (int)((double)c / (double)top * (NB - 1) + 0.5) : 0;
+      return NB - 1 - std::min(std::max(l, 0), NB - 1);   // 0: the most costly
+    };
+    std::fill(bucket_n.begin(), bucket_n.end(), 0u);
+    for (size_t i = b0; i < b; ++i) ++bucket_n[(size_t)level(units[i].cost) + 1];
+    for (int l = 0; l < NB; ++l) bucket_n[(size_t)l + 1] += bucket_n[(size_t)l];
+    idx.assign(b - b0, 0u);
+    for (size_t i = b0; i < b; ++i) idx[bucket_n[(size_t)level(units[i].cost)]++] = (uint32_t)i;
+    for (uint32_t i : idx)
+      for (uint32_t q = 0; q < units[i].n; ++q) order.push_back(units[i].first + q);
+    count[x] = (uint32_t)(order.size() - start[x]);
+  }
+}
+
 void make_frame_bricks(const Layout& L, const ViewState& V, const PlanParams& P,
                        const std::vector<PlannedBrick>& list, std::vector<FrameBrick>& out,
                        const uint32_t* tex_dims, const uint64_t* tex_offset) {

@@ -119,6 +119,19 @@ void make_frame_bricks(const Layout& L, const ViewState& V, const PlanParams& P,
                        const std::vector<PlannedBrick>& list, std::vector<FrameBrick>& out,
                        const uint32_t* tex_dims = nullptr, const uint64_t* tex_offset = nullptr);
 void make_view_const(const ViewState& V, ViewConst& vc);
+// Work estimate of each 16x16 group of a view (vc.groups entries from cost):
+// the samples a ray takes across every brick whose pixel rectangle covers the
+// group (the box's longest edge over its ray step), summed, plus a twentieth of
+// the largest brick's weight for the fixed cost (ray set-up, brick tests, the
+// pixel write).
+void group_costs(const ViewConst& vc, const FrameBrick* list, float* cost);
+// The dispatch order of a launch's groups (cost: group_costs of every view at
+// its group_offset): 8 contiguous ranges of about equal group count, one per
+// XCD, each range's units -- single groups (mode 1) or rows of a view's groups
+// (mode 2) -- in decreasing cost, groups of a row in order: order[start[x] + j]
+// is XCD x's j-th group, j < count[x].
+void dispatch_order(const ViewConst* views, int nviews, const std::vector<float>& cost, int mode,
+                    std::vector<uint32_t>& order, uint32_t start[8], uint32_t count[8]);
 
 // TransferFunction1D::Set(rgba8) + ComputeNonZeroLimits; TransferFunction2D::ComputeNonZeroLimits
 void tf1d_limits(const uint8_t* rgba8, int n, uint64_t lim[2]);

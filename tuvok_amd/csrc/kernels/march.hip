@@ -563,17 +563,19 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
       const int nx = (int)fb->n[0], ny = (int)fb->n[1], nz = (int)fb->n[2];
       const V3 vstep = mk(fb->vstep[0], fb->vstep[1], fb->vstep[2]);
       const float sscale = fb->step_scale;
-      // Fixed step (LEARN 0): the samples whose 4x4x4 neighbourhood lies inside the
-      // brick, decided once per segment instead of per sample.  Sample i sits at
+      // The fixed-step indices whose 4x4x4 neighbourhood lies inside the brick,
+      // decided once per segment instead of per sample.  Sample i sits at
       // u_a = (entryTex_a + i * incTex_a) * n_a - 0.5 up to the drift of the
       // accumulated position: each addition of a coordinate below 1 rounds by at
       // most 2^-24, so after i < steps additions the drift is below steps * 2^-24
       // in texture space, steps * n_a * 2^-24 voxels; the margin is twice that
       // plus 1/16 voxel.  The interior form needs 1 <= floor(u_a) and
       // floor(u_a) + 2 < n_a on every axis.  The range only selects between two
-      // forms that give the same values, so a conservative one is enough.
+      // forms that give the same values, so a conservative one is enough.  A
+      // learned march samples a subset of these indices, after no more than
+      // steps additions either.
       int in_lo = 0, in_hi = -1;
-      if (LEARN == 0 && MODE != 2) {
+      if (MODE != 2) {
         float lo = 0.0f, hi = (float)(steps - 1);
         const float ec[3] = {entryTex.x, entryTex.y, entryTex.z}, ic3[3] = {incTex.x, incTex.y, incTex.z};
         const int nn3[3] = {nx, ny, nz};
@@ -631,13 +633,13 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         continue;
       }
 
-      // March nsamp samples from (pos, cur) with the step multiplied by 2^k
-      // (k = 0: GLRaycaster's step; the opacity-correction exponent scales with
-      // it), under-compositing into (c0..c3).  Returns false once the segment
-      // ends: 'a >= 0.99', or early termination of the ray whose transmittance
-      // before the brick is tray.
+      // March nsamp samples from (pos, cur), the fixed-step index i0 of the
+      // segment, with the step multiplied by 2^k (k = 0: GLRaycaster's step;
+      // the opacity-correction exponent scales with it), under-compositing into
+      // (c0..c3).  Returns false once the segment ends: 'a >= 0.99', or early
+      // termination of the ray whose transmittance before the brick is tray.
       const V3 rayInc = dir / L * rstep;
-      auto run = [&](int k, int nsamp, V3& pos, V3& cur, float& c0, float& c1, float& c2, float& c3,
+      auto run = [&](int k, int nsamp, int i0, V3& pos, V3& cur, float& c0, float& c1, float& c2, float& c3,
                      float tray) -> bool {
         const float mlt = (float)(1 << k);
         const float ssc = k == 0 ? sscale : sscale * mlt;
@@ -716,41 +718,28 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           pos = mk(fmaf(incTex.x, mlt, pos.x), fmaf(incTex.y, mlt, pos.y), fmaf(incTex.z, mlt, pos.z));
           return true;
         };
-        if constexpr (LEARN == 0 && !(MODE == 0 && !LIT)) {
-          // Fixed step: each lane marches its samples before the segment's
-          // interior range in the general form, the interior range in the
-          // interior form, and the rest in the general form again, so that
-          // only the few samples near a brick face pay for clamped taps
-          // whatever step the other lanes of the wave are at.  Every lane
-          // executing the interior loop is inside its range.
-          const int h_end = in_hi >= in_lo ? in_lo : nsamp;
+        if constexpr (!(MODE == 0 && !LIT)) {
+          // Each lane marches its samples before the segment's interior range
+          // in the general form, the interior range in the interior form, and
+          // the rest in the general form again, so that only the few samples
+          // near a brick face pay for clamped taps whatever step the other
+          // lanes of the wave are at.  Sample j of this run has the fixed-step
+          // index i0 + j 2^k: the run's interior samples are those j with
+          // in_lo <= i0 + j 2^k <= in_hi, one range as the interior indices are
+          // one.  Every lane executing the interior loop is inside its range.
+          const int j_lo = max(in_lo - i0 > 0 ? (in_lo - i0 + (1 << k) - 1) >> k : 0, 0);
+          const int j_hi = min(in_hi - i0 >= 0 ? (in_hi - i0) >> k : -1, nsamp - 1);
+          const int h_end = j_hi >= j_lo ? j_lo : nsamp;
           int i = 0;
           for (; i < h_end; ++i)
             if (!step(std::false_type{})) return false;
-          for (; i <= in_hi; ++i)
+          for (; i <= j_hi; ++i)
             if (!step(std::true_type{})) return false;
           for (; i < nsamp; ++i)
             if (!step(std::false_type{})) return false;
         } else {
-          for (int i = 0; i < nsamp; ++i) {
-            bool ok;
-            if constexpr (MODE == 0 && !LIT) {
-              ok = step(std::false_type{});
-            } else {
-              // does the 4x4x4 neighbourhood of every active lane lie inside the brick?
-              bool inner = true;
-              const float pc[3] = {pos.x, pos.y, pos.z};
-              const int nn3[3] = {nx, ny, nz};
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                const float sa = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = sa * (float)nn3[a] - 0.5f;
-                const int ia = (int)floorf(u);
-                inner = inner && ia >= 1 && ia + 2 < nn3[a];
-              }
-              ok = __ballot(!inner) == 0 ? step(std::true_type{}) : step(std::false_type{});
-            }
-            if (!ok) return false;
-          }
+          for (int i = 0; i < nsamp; ++i)
+            if (!step(std::false_type{})) return false;
         }
         return true;
       };
@@ -758,7 +747,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
       float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
       if constexpr (LEARN == 0) {
         V3 pos = entryTex, cur = entry;
-        run(0, steps, pos, cur, c0, c1, c2, c3, 1.0f - dst3);
+        run(0, steps, 0, pos, cur, c0, c1, c2, c3, 1.0f - dst3);
       } else {
         // Learned step: the segment is marched in chunks of SN_CHUNK fixed
         // steps.  Per chunk every lane bounds its values with the brick's cell
@@ -810,12 +799,14 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         const int tn = S.tfst_n;
         const float* __restrict__ ts = S.tfst;
         const float vmaxs = (float)Unorm<T>::max_value();
-        for (int i0 = 0;; i0 += SN_CHUNK) {
-          const bool has = live && i0 < steps;
-          if (__ballot(has) == 0) break;
-          const int nf = has ? min(SN_CHUNK, steps - i0) : 1;
-          const float T0 = tray * (1.0f - c3);
-          // value range of the chunk (stored values) from the cell grid
+        // the lane's chunk at i0: its steps, the transmittance before it, its
+        // value range from the cell grid, the TF statistics over that range
+        // and the contribution bound T0 (1 - (1 - amax)^(s nf))
+        int nf = 1, lo = 0, hi = 0;
+        float T0 = 0.0f, vl = 0.0f, vh = 0.0f, cnt = 1.0f, amax = 0.0f, smax = 0.0f, obound = 0.0f;
+        auto bound_chunk = [&](int i0) {
+          nf = min(SN_CHUNK, steps - i0);
+          T0 = tray * (1.0f - c3);
           uint32_t vlo = 0xffffu, vhi = 0u;
 #pragma unroll
           for (int q = 0; q <= SN_CHUNK; q += SN_PROBE) {
@@ -838,44 +829,71 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             const float cc = fminf(fmaxf(v * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
             return min((int)floorf(cc * (float)tn), tn - 1);
           };
-          const float vl = fmaxf((float)vlo - 1.0f, 0.0f), vh = fminf((float)vhi + 1.0f, vmaxs);
-          const int lo = entry_of(vl), hi = max(entry_of(vh), lo);
-          const float cnt = (float)(hi - lo + 1);
+          vl = fmaxf((float)vlo - 1.0f, 0.0f);
+          vh = fminf((float)vhi + 1.0f, vmaxs);
+          lo = entry_of(vl);
+          hi = max(entry_of(vh), lo);
+          cnt = (float)(hi - lo + 1);
           const int kk = 31 - __clz(hi - lo + 1);
-          const float amax = fmaxf(ts[tfst_amax(tn, kk) + lo], ts[tfst_amax(tn, kk) + hi - (1 << kk) + 1]);
-          float smax = 0.0f;
+          amax = fmaxf(ts[tfst_amax(tn, kk) + lo], ts[tfst_amax(tn, kk) + hi - (1 << kk) + 1]);
+          smax = 0.0f;
           if (hi > lo) {
             const int k2 = 31 - __clz(hi - lo);
             const int r = tfst_smax(tn, S.tfst_levels, k2);
             smax = fmaxf(ts[r + lo], ts[r + hi - (1 << k2)]);
           }
-          const float obound = 1.0f - powf(1.0f - fminf(amax, 1.0f), sscale * (float)nf);
-          // the network inputs that change per chunk, put(index, value)
-          auto chunk_features = [&](auto&& put) {
-            const float x0 = fminf(fmaxf(vl * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
-            const float x1 = fminf(fmaxf(vh * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
-            put(0, x0);
-            put(1, x1);
-            put(2, x1 - x0);
-            put(3, (ts[tfst_pa(tn) + hi + 1] - ts[tfst_pa(tn) + lo]) / cnt);
-            put(4, amax);
-            put(5, ts[tfst_ptv(tn) + hi] - ts[tfst_ptv(tn) + lo]);
-            put(6, ts[tfst_prgb(tn) + hi] - ts[tfst_prgb(tn) + lo]);
-            put(7, smax);
-            put(8, (ts[tfst_pnz(tn) + hi + 1] - ts[tfst_pnz(tn) + lo]) / cnt);
-            put(14, (float)nf * (1.0f / (float)SN_CHUNK));
-            put(15, T0);
-            put(16, -log2f(fmaxf(T0, 1e-6f)) * (1.0f / 20.0f));
-            put(18, obound);
-            put(19, T0 * obound);
-          };
-          bool skip = has && strict && T0 * obound <= LC->skip_bound;
-          int k = 0;
-          if constexpr (LEARN == 1) {
+          obound = 1.0f - powf(1.0f - fminf(amax, 1.0f), sscale * (float)nf);
+        };
+        // the network inputs that change per chunk, put(index, value)
+        auto chunk_features = [&](auto&& put) {
+          const float x0 = fminf(fmaxf(vl * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
+          const float x1 = fminf(fmaxf(vh * Unorm<T>::inv() * S.tf_scale, 0.0f), 1.0f);
+          put(0, x0);
+          put(1, x1);
+          put(2, x1 - x0);
+          put(3, (ts[tfst_pa(tn) + hi + 1] - ts[tfst_pa(tn) + lo]) / cnt);
+          put(4, amax);
+          put(5, ts[tfst_ptv(tn) + hi] - ts[tfst_ptv(tn) + lo]);
+          put(6, ts[tfst_prgb(tn) + hi] - ts[tfst_prgb(tn) + lo]);
+          put(7, smax);
+          put(8, (ts[tfst_pnz(tn) + hi + 1] - ts[tfst_pnz(tn) + lo]) / cnt);
+          put(14, (float)nf * (1.0f / (float)SN_CHUNK));
+          put(15, T0);
+          put(16, -log2f(fmaxf(T0, 1e-6f)) * (1.0f / 20.0f));
+          put(18, obound);
+          put(19, T0 * obound);
+        };
+        // a skipped chunk: the fixed march's positions, bit for bit
+        auto skip_steps = [&]() {
+          for (int i = 0; i < nf; ++i) {
+            if (LIT) cur = cur + rayInc;
+            pos = pos + incTex;
+          }
+        };
+        if constexpr (LEARN == 1) {
+          // Each lane first goes by the chunks it skips exactly (bound at most
+          // skip_bound) on its own, so a wave marches together only chunks
+          // that some of its lanes must march; then the network (or the
+          // fallback, or fixed_k) decides for the lanes that stopped at such a
+          // chunk, the wave takes the smallest multiplier of its marching
+          // lanes, and each lane marches or skips its own chunk.
+          int i0 = 0;
+          for (;;) {
+            bool has = live && i0 < steps;
+            while (has) {
+              bound_chunk(i0);
+              if (!(strict && T0 * obound <= LC->skip_bound)) break;
+              skip_steps();
+              i0 += SN_CHUNK;
+              has = i0 < steps;
+            }
+            if (__ballot(has) == 0) break;
+            bool skip = false;
+            int k = 0;
             // a lane whose bound exceeds the fallback marches the fixed step and
             // so sets the wave's multiplier to 1: then the network is not asked
-            const bool forced = has && !skip && T0 * obound > LC->fallback;
-            const bool ask = has && !skip && !forced;
+            const bool forced = has && T0 * obound > LC->fallback;
+            const bool ask = has && !forced;
             if (LC->fixed_k >= 0) {
               k = LC->fixed_k;
             } else if (__ballot(forced) == 0 && __ballot(ask) != 0) {
@@ -892,38 +910,45 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             int kw = LC->max_log2;
             for (int q = LC->max_log2 - 1; q >= 0; --q)
               if (__ballot(marching && k <= q) != 0) kw = q;
-            k = kw;
-          } else if (has && !skip) {
-            // record this chunk (hash-selected) with its RGBA at multipliers 1, 2, 4, 8
-            uint32_t h = (uint32_t)(px * 73856093) ^ (uint32_t)(py * 19349663) ^ (fb->toc * 83492791u) ^
-                         (view_id * 2654435761u) ^ ((uint32_t)i0 * 2246822519u) ^ SO->seed;
-            h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
-            if (h < SO->keep) {
-              const unsigned int slot = atomicAdd(SO->count, 1u);
-              if (slot < SO->cap) {
-                SegSample* o = SO->out + slot;
-                seg_features([&](int q, float v) { o->f[q] = v; });
-                chunk_features([&](int q, float v) { o->f[q] = v; });
-                o->t_before = T0;
-                o->steps = (float)nf;
+            if (has) {
+              if (skip) skip_steps();
+              else live = run(kw, (nf + (1 << kw) - 1) >> kw, i0, pos, cur, c0, c1, c2, c3, tray);
+              i0 += SN_CHUNK;
+            }
+          }
+        } else {
+          // LEARN 2: the fixed march, chunk by chunk across the wave, recording
+          // hash-selected chunks with their RGBA at multipliers 1, 2, 4, 8
+          for (int i0 = 0;; i0 += SN_CHUNK) {
+            const bool has = live && i0 < steps;
+            if (__ballot(has) == 0) break;
+            if (has) bound_chunk(i0);
+            const bool skip = has && strict && T0 * obound <= LC->skip_bound;
+            if (has && !skip) {
+              uint32_t h = (uint32_t)(px * 73856093) ^ (uint32_t)(py * 19349663) ^ (fb->toc * 83492791u) ^
+                           (view_id * 2654435761u) ^ ((uint32_t)i0 * 2246822519u) ^ SO->seed;
+              h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+              if (h < SO->keep) {
+                const unsigned int slot = atomicAdd(SO->count, 1u);
+                if (slot < SO->cap) {
+                  SegSample* o = SO->out + slot;
+                  seg_features([&](int q, float v) { o->f[q] = v; });
+                  chunk_features([&](int q, float v) { o->f[q] = v; });
+                  o->t_before = T0;
+                  o->steps = (float)nf;
 #pragma unroll 1
-                for (int mi = 0; mi < 4; ++mi) {
-                  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-                  V3 pp = pos, cc = cur;
-                  run(mi, (nf + (1 << mi) - 1) >> mi, pp, cc, a0, a1, a2, a3, 1.0f);
-                  o->c[mi][0] = a0; o->c[mi][1] = a1; o->c[mi][2] = a2; o->c[mi][3] = a3;
+                  for (int mi = 0; mi < 4; ++mi) {
+                    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+                    V3 pp = pos, cc = cur;
+                    run(mi, (nf + (1 << mi) - 1) >> mi, i0, pp, cc, a0, a1, a2, a3, 1.0f);
+                    o->c[mi][0] = a0; o->c[mi][1] = a1; o->c[mi][2] = a2; o->c[mi][3] = a3;
+                  }
                 }
               }
             }
-          }
-          if (has) {
-            if (skip) {
-              for (int i = 0; i < nf; ++i) {   // the fixed march's positions, bit for bit
-                if (LIT) cur = cur + rayInc;
-                pos = pos + incTex;
-              }
-            } else {
-              live = run(k, (nf + (1 << k) - 1) >> k, pos, cur, c0, c1, c2, c3, tray);
+            if (has) {
+              if (skip) skip_steps();
+              else live = run(0, nf, i0, pos, cur, c0, c1, c2, c3, tray);
             }
           }
         }
@@ -986,11 +1011,17 @@ __device__ __forceinline__ void march_group(const MarchArgs& A) {
   __shared__ uint64_t s_cols[MODE == 1 ? 4 * 64 : 1];                          // 2D TF column masks, one slice per wave
 
   // XCD-aware remap: consecutive workgroups go round-robin over the 8 XCDs;
-  // give each XCD a contiguous range of 16x16 groups.
-  const uint32_t lin = blockIdx.x;
-  const uint32_t per_xcd = A.groups_padded >> 3;
-  const uint32_t g = (lin & 7u) * per_xcd + (lin >> 3);
-  if (g >= A.groups_total) return;  // uniform for the whole workgroup
+  // each XCD runs a contiguous range of 16x16 groups, in the host's order
+  // (heaviest first) when there is one.
+  const uint32_t lin = blockIdx.x, xcd = lin & 7u, j = lin >> 3;
+  uint32_t g;
+  if (A.order) {
+    if (j >= A.xcd_count[xcd]) return;  // uniform for the whole workgroup
+    g = A.order[A.xcd_start[xcd] + j];
+  } else {
+    g = xcd * (A.groups_padded >> 3) + j;
+    if (g >= A.groups_total) return;
+  }
 
   if (MODE == 0) {
     for (int i = threadIdx.x; i < A.S.tf_n; i += 256) s_tf[i] = A.S.tf1[i];

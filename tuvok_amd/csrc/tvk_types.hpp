@@ -164,7 +164,12 @@ struct MarchArgs {
   int nviews;
   int lighting, mode;
   uint32_t groups_total;        // sum of groups over the views
-  uint32_t groups_padded;       // rounded up to a multiple of 8 (XCD count)
+  uint32_t groups_padded;       // the grid: a multiple of 8 (XCD count)
+  // Dispatch order (or nullptr: XCD x runs groups x * groups_padded / 8 + j in
+  // turn).  With an order, XCD x runs order[xcd_start[x] + j] for j <
+  // xcd_count[x] and groups_padded is 8 x the largest count.
+  const uint32_t* order;
+  uint32_t xcd_start[8], xcd_count[8];
   int accumulate;               // continue the pixels' values (and depths) of an earlier launch
   float* depth;                 // isosurface: per-pixel depth of the best hit so far (or nullptr)
   float4* hitpos;               // isosurface: per-pixel eye-space hit and its segment parameter (or nullptr)

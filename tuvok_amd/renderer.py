@@ -95,6 +95,13 @@ class HIPRaycaster:
         self._chk(self._L.tvk_set_params(self._h, C.byref(self._p)))
         return info
 
+    def SetDispatchOrder(self, order):
+        """The order multi-view renders hand their 16x16 groups to the GPU:
+        DISPATCH_RASTER (default), DISPATCH_GROUPS or DISPATCH_ROWS (most costly
+        first within each XCD's eighth).  Frames are identical in every order."""
+        self._chk(self._L.tvk_set_dispatch_order(self._h, int(order)))
+        self._dispatch = int(order)
+
     def SetLODFilter(self, median):
         """bComputeMedian of ExtendedOctreeConverter::Convert for the next bricking."""
         self._chk(self._L.tvk_set_lod_filter(self._h, int(bool(median))))
