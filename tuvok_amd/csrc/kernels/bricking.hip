@@ -12,6 +12,7 @@
 //                     volume (clamped to the edge with bClampToEdge)
 // Bit-exactness against the procedural restatement is checked by the tests.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 #include <algorithm>
 #include "../tvk_types.hpp"
@@ -60,7 +61,8 @@ __global__ __launch_bounds__(256) void extend_rows(T* __restrict__ pool, const G
 // (every stored voxel is the own voxel of exactly one row; padding repeats real
 // ones) is folded into mm[2 * brick] / mm[2 * brick + 1] (init_minmax first).
 // A workgroup is one (brick, slot) pair: slot s takes the items s, s + gx, ...
-// of its brick.  Workgroups are dealt to the 8 XCDs in turn, so the linear
+// of its brick, each item's row loads issued together (all in flight), the
+// pool rows written with streaming stores.  Workgroups are dealt to the 8 XCDs in turn, so the linear
 // workgroup id is split into (XCD, sequence) and each XCD gets a contiguous run
 // of the order (brick row, slot, brick within the row of row_len bricks along
 // x): the bricks of one row run side by side on one XCD, and the 128-byte lines
@@ -93,33 +95,54 @@ __global__ __launch_bounds__(256) void gather_rows(const T* __restrict__ lin, ui
     const uint32_t bx0 = ch * GATHER_CB, nb = min((uint32_t)GATHER_CB, nbx - bx0);
     const uint32_t x0 = bx0 << lx, nl = (nb << lx) + W;   // voxels loaded per row
     __syncthreads();   // the previous item's rows are consumed
-    // wave wv loads rows 4wv .. 4wv+3 (y = r & 3, z = r >> 2 of the block row)
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t r = (uint32_t)(wv * 4 + k);
-      const uint32_t y = min((yb << 2) | (r & 3u), b.n[1] - 1), z = min((zb << 2) | (r >> 2), b.n[2] - 1);
-      int64_t gy = oy0 + y, gz = oz0 + z;
-      bool rin = true;
-      if (clamp) {
-        gy = gy < 0 ? 0 : (gy >= (int64_t)Y ? (int64_t)Y - 1 : gy);
-        gz = gz < 0 ? 0 : (gz >= (int64_t)Z ? (int64_t)Z - 1 : gz);
-      } else {
-        rin = gy >= 0 && gz >= 0 && gy < (int64_t)Y && gz < (int64_t)Z;
-      }
-      // a row of the linear volume is addressed from one (wave-uniform) base
-      // with 32-bit voxel offsets (X is at most 2^31)
-      const T* __restrict__ src = lin + (rin ? X * ((uint64_t)gy + Y * (uint64_t)gz) : 0);
-      const int32_t ox = (int32_t)ox0, xmax = (int32_t)X - 1;
-      for (uint32_t xi = (uint32_t)ln; xi < nl; xi += 64) {
-        int32_t gx = ox + (int32_t)min(x0 + xi, b.n[0] - 1);
-        T v = 0;
-        if (clamp) {
-          gx = min(max(gx, 0), xmax);
-          v = src[gx];
-        } else if (rin && gx >= 0 && gx <= xmax) {
-          v = src[gx];
+    // wave wv loads rows 4wv .. 4wv+3 (y = r & 3, z = r >> 2 of the block row),
+    // NJ = ceil(nl / 64) voxels per lane and row.  Every load is issued before
+    // the first LDS write, from addresses clamped into the volume (the border
+    // value is selected afterwards), so that a wave keeps its loads in flight
+    // together instead of waiting for each in turn.
+    const int32_t ox = (int32_t)ox0, xmax = (int32_t)X - 1;
+    auto stage = [&](auto njc) {
+      constexpr int NJ = decltype(njc)::value;
+      T vals[4][NJ];
+      uint32_t ok = 0u;   // bit k * NJ + j: the voxel lies in the volume
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t r = (uint32_t)(wv * 4 + k);
+        const uint32_t y = min((yb << 2) | (r & 3u), b.n[1] - 1), z = min((zb << 2) | (r >> 2), b.n[2] - 1);
+        const int64_t gy = oy0 + y, gz = oz0 + z;
+        const bool rin = gy >= 0 && gz >= 0 && gy < (int64_t)Y && gz < (int64_t)Z;
+        const int64_t cy = gy < 0 ? 0 : (gy >= (int64_t)Y ? (int64_t)Y - 1 : gy);
+        const int64_t cz = gz < 0 ? 0 : (gz >= (int64_t)Z ? (int64_t)Z - 1 : gz);
+        // a row of the linear volume is addressed from one (wave-uniform) base
+        // with 32-bit voxel offsets (X is at most 2^31)
+        const T* __restrict__ src = lin + X * ((uint64_t)cy + Y * (uint64_t)cz);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const uint32_t xi = (uint32_t)ln + 64u * (uint32_t)j;
+          const int32_t gxv = ox + (int32_t)min(x0 + xi, b.n[0] - 1);
+          vals[k][j] = src[min(max(gxv, 0), xmax)];
+          if (rin && gxv >= 0 && gxv <= xmax) ok |= 1u << (k * NJ + j);
         }
-        row[r][xi] = v;
       }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const uint32_t xi = (uint32_t)ln + 64u * (uint32_t)j;
+          if (xi < nl) row[wv * 4 + k][xi] = (clamp || ((ok >> (k * NJ + j)) & 1u)) ? vals[k][j] : (T)0;
+        }
+    };
+    static_assert((L + 63) / 64 <= 9, "rows of at most 9 x 64 voxels");
+    switch ((nl + 63u) / 64u) {
+      case 1: stage(std::integral_constant<int, 1>{}); break;
+      case 2: stage(std::integral_constant<int, 2>{}); break;
+      case 3: stage(std::integral_constant<int, 3>{}); break;
+      case 4: stage(std::integral_constant<int, 4>{}); break;
+      case 5: stage(std::integral_constant<int, 5>{}); break;
+      case 6: stage(std::integral_constant<int, 6>{}); break;
+      case 7: stage(std::integral_constant<int, 7>{}); break;
+      case 8: stage(std::integral_constant<int, 8>{}); break;
+      default: stage(std::integral_constant<int, 9>{}); break;
     }
     __syncthreads();
     // consecutive lanes on consecutive 16-byte rows: own W voxels + the W repeated
@@ -136,8 +159,10 @@ __global__ __launch_bounds__(256) void gather_rows(const T* __restrict__ lin, ui
         mn = min(mn, v);
         mx = max(mx, v);
       }
-      *reinterpret_cast<uint4*>(pool + b.offset + ((blk0 + bl) * 16u + r) * (2 * W)) =
-          make_uint4(own.x, own.y, rep.x, rep.y);
+      // streaming store: the pool is not read again by this launch
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 q = {own.x, own.y, rep.x, rep.y};
+      __builtin_nontemporal_store(q, reinterpret_cast<u32x4*>(pool + b.offset + ((blk0 + bl) * 16u + r) * (2 * W)));
     }
   }
   if (mm) {
@@ -639,8 +664,14 @@ hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const
                          uint32_t row_len, uint32_t max_vox, void* pool, const uint32_t core[3], uint32_t ov, int clamp,
                          uint32_t* mm, hipStream_t st) {
   // max_vox: the largest padded element count; one work item per 16 x 16-byte
-  // rows of GATHER_CB blocks
-  uint32_t gx = (uint32_t)(max_vox / (16u * 2u * GATHER_CB * (bpv == 1 ? 8u : 4u))) + 1u;
+  // rows of GATHER_CB blocks.  Slots per brick: about 64 items per workgroup
+  // (long-lived workgroups: 1.50 ms for the 1024^3 uint16 volume's 512 bricks
+  // against 3.8 ms at 4 items), but at least 4096 workgroups in all while a
+  // slot still has an item.
+  const uint32_t item_vox = 16u * 2u * GATHER_CB * (bpv == 1 ? 8u : 4u);
+  const uint32_t gx_items = max_vox / item_vox + 1u;
+  uint32_t gx = max_vox / (16u * item_vox) + 1u;
+  gx = std::max(gx, std::min(gx_items, (4096u + std::max(nb, 1u) - 1u) / std::max(nb, 1u)));
   if (gx > 4096) gx = 4096;
   if (row_len == 0) row_len = 1;
   // bricks per launch: at most 2^23 workgroups (the grid's work-item count
