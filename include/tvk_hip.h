@@ -513,6 +513,11 @@ typedef struct {
   int fixed_log2;        /* >= 0: no network (net may be NULL), every chunk that is not skipped
                             takes multiplier 2^fixed_log2 (the chunked march without the
                             network, for comparison); < 0: the network decides */
+  float fallback_tv;     /* fixed step where the chunk's variation risk exceeds this (<= 0: never):
+                            T_before * (bound * TVc + min(1, n * fStepScale * TVa)), TVc and TVa
+                            the total variation of the TF's colour and alpha over the chunk's
+                            value range -- a homogeneous chunk keeps its proposal however
+                            opaque it is; a forced chunk holds its wavefront to multiplier 1 */
 } tvk_proposal_params;
 int tvk_set_step_proposal(tvk_ctx* ctx, tvk_stepnet* net, const tvk_proposal_params* p);
 /* The dataset's cell grid (built on first use): per TOC brick, cells of 8^3

@@ -169,6 +169,45 @@ def test_exact_skip_leaves_frame_unchanged_with_fewer_samples():
     r.close()
 
 
+def test_variation_fallback_orders_samples_and_error():
+    """The variation-risk fallback on the chunked march at multiplier 4
+    (phantom, lit 1D TF): a threshold no chunk reaches changes nothing; lower
+    thresholds force more chunks (and their wavefronts) back to the fixed step:
+    the smallest one ends near the multiplier-1 sample count, well above the
+    multiplier-4 one, and closer to the fixed-step frame."""
+    from tuvok_amd import GEN_PHANTOM, RM_1DTRANS
+    from tuvok_amd.transfer import tf1d_phantom
+    r = HIPRaycaster(0)
+    r.GenerateVolume(GEN_PHANTOM, 4, np.uint16, (128, 128, 128), (34, 34, 34), 1, max_lods=1)
+    r.Set1DTrans(tf1d_phantom().GetByteArray())
+    r.SetRendermode(RM_1DTRANS)
+    r.SetUseLighting(True)
+    r.SetSampleRateModifier(2.0)
+    r.SetViewParameters(45.0, 0.01, 1000.0)
+    r.SetViewPosition((0.0, 0.0, 2.0))
+    r.Resize(160, 120)
+    r.SetRotation(rotation_y(0.4))
+    r.SetCountSamples(True)
+    try:
+        r.SetStepProposal(None)
+        fixed = r.Paint()
+        r.SetStepProposal(None, fixed_log2=0, skip_bound=0.0)
+        r.Paint()
+        n1 = r.last_stats.samples
+        got = []
+        for tv in (0.0, 1e9, 0.05, 0.005, 0.0005, 1e-7):
+            r.SetStepProposal(None, fixed_log2=2, skip_bound=0.0, fallback_tv=tv)
+            img = r.Paint()
+            got.append((tv, img, r.last_stats.samples, float(np.abs(img - fixed).max())))
+            print(f"fallback_tv {tv}: samples {got[-1][2]} (x1: {n1}) max err {got[-1][3] * 255:.3f}/255", flush=True)
+    finally:
+        r.SetStepProposal(None)
+        r.close()
+    assert np.array_equal(got[0][1], got[1][1]) and got[0][2] == got[1][2]
+    assert got[0][2] < 0.5 * n1 and got[0][2] < got[-1][2] <= 1.05 * n1
+    assert got[-1][3] < got[0][3]
+
+
 def test_learned_step_gain_at_error_bound(net):
     """End to end on a 128^3 uint16 fBm+shells volume, lit 2D TF, 0.25-voxel
     reference step: chunk samples from 48 views, labels, training on the

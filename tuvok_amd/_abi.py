@@ -63,7 +63,8 @@ class Tuning(C.Structure):
 
 class ProposalParams(C.Structure):
     _fields_ = [("margin", C.c_float), ("skip_threshold", C.c_float), ("max_log2", C.c_int),
-                ("fallback_bound", C.c_float), ("skip_bound", C.c_float), ("fixed_log2", C.c_int)]
+                ("fallback_bound", C.c_float), ("skip_bound", C.c_float), ("fixed_log2", C.c_int),
+                ("fallback_tv", C.c_float)]
 
 
 class LeapStats(C.Structure):

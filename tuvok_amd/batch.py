@@ -74,14 +74,15 @@ class RenderBatch:
         self._chk(self._L.tvk_batch_get_tuning(self._h, C.byref(t)))
         return dict(tile_w=t.tile_w, tile_h=t.tile_h, groups_per_cu=t.groups_per_cu, use_graph=bool(t.use_graph))
 
-    def SetStepProposal(self, net, margin=0.0, skip_threshold=1.0, max_log2=3, fallback_bound=0.0, skip_bound=0.0):
+    def SetStepProposal(self, net, margin=0.0, skip_threshold=1.0, max_log2=3, fallback_bound=0.0, skip_bound=0.0,
+                        fallback_tv=0.0):
         """Learned step for the 1D/2D-TF requests (tuvok_amd.stepnet.StepNet); None: fixed step."""
         if net is None:
             self._chk(self._L.tvk_batch_set_step_proposal(self._h, None, None))
             self._net = None
             return
         p = _abi.ProposalParams(float(margin), float(skip_threshold), int(max_log2), float(fallback_bound),
-                                float(skip_bound), -1)
+                                float(skip_bound), -1, float(fallback_tv))
         self._chk(self._L.tvk_batch_set_step_proposal(self._h, net._h, C.byref(p)))
         self._net = net
 

@@ -1507,6 +1507,7 @@ int tvk_set_step_proposal(tvk_ctx* c, tvk_stepnet* net, const tvk_proposal_param
     c->learn.skip_logit = q >= 1.0 ? INFINITY : (q <= 0.0 ? -INFINITY : (float)std::log(q / (1.0 - q)));
     c->learn.max_log2 = p->max_log2;
     c->learn.fallback = p->fallback_bound > 0.0f ? p->fallback_bound : INFINITY;
+    c->learn.fallback_tv = p->fallback_tv > 0.0f ? p->fallback_tv : INFINITY;
     c->learn.skip_bound = p->skip_bound;
   }
   return TVK_OK;

@@ -390,6 +390,7 @@ int tvk_batch_set_step_proposal(tvk_batch* b, tvk_stepnet* net, const tvk_propos
     b->learn.skip_logit = q >= 1.0 ? INFINITY : (q <= 0.0 ? -INFINITY : (float)std::log(q / (1.0 - q)));
     b->learn.max_log2 = p->max_log2;
     b->learn.fallback = p->fallback_bound > 0.0f ? p->fallback_bound : INFINITY;
+    b->learn.fallback_tv = p->fallback_tv > 0.0f ? p->fallback_tv : INFINITY;
     b->learn.skip_bound = p->skip_bound;
   }
   return TVK_OK;

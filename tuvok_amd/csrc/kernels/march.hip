@@ -890,12 +890,18 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             if (__ballot(has) == 0) break;
             bool skip = false;
             int k = 0;
-            // a lane whose bound exceeds the fallback marches the fixed step and
-            // so sets the wave's multiplier to 1: then the network is not asked
-            const bool forced = has && T0 * obound > LC->fallback;
+            // a lane whose bound (or variation risk) exceeds its fallback marches
+            // the fixed step and so sets the wave's multiplier to 1: then the
+            // network is not asked
+            bool forced = has && T0 * obound > LC->fallback;
+            if (LC->fallback_tv < INFINITY && has && !forced) {
+              const float tva = ts[tfst_ptv(tn) + hi] - ts[tfst_ptv(tn) + lo];
+              const float tvc = ts[tfst_prgb(tn) + hi] - ts[tfst_prgb(tn) + lo];
+              forced = T0 * (obound * tvc + fminf(1.0f, (float)nf * sscale * tva)) > LC->fallback_tv;
+            }
             const bool ask = has && !forced;
             if (LC->fixed_k >= 0) {
-              k = LC->fixed_k;
+              k = forced ? 0 : LC->fixed_k;
             } else if (__ballot(forced) == 0 && __ballot(ask) != 0) {
               chunk_features([&](int q, float v) { xrow[q] = bf16_bits(v); });
               float y0, y1;

@@ -121,6 +121,10 @@ struct LearnConst {
   float skip_logit;    // skip a segment when y1 exceeds this logit
   int max_log2;        // largest log2 multiplier (multiplier 1..8)
   float fallback;      // fixed step when the chunk's contribution bound exceeds this
+  // fixed step when the chunk's variation risk exceeds this: T_before * (bound *
+  // colour total variation + min(1, steps * step scale * alpha total variation))
+  // over the TF entries of the chunk's value range
+  float fallback_tv;
   float skip_bound;    // skip a chunk whose contribution bound is at most this (< 0: never)
   int fixed_k = -1;    // >= 0: no network, every marched chunk takes multiplier 2^fixed_k
 };

@@ -476,16 +476,18 @@ class HIPRaycaster:
 
     # -- step proposal ------------------------------------------------------------
     def SetStepProposal(self, net, margin=0.0, skip_threshold=1.0, max_log2=3, fallback_bound=0.0, skip_bound=0.0,
-                        fixed_log2=-1):
+                        fixed_log2=-1, fallback_tv=0.0):
         """Attach a StepNet (tuvok_amd.stepnet) to march in chunks with proposed
         step multipliers; net=None restores the fixed step, unless fixed_log2 >= 0
-        (the chunked march without a network: multiplier 2^fixed_log2)."""
+        (the chunked march without a network: multiplier 2^fixed_log2).
+        fallback_bound / fallback_tv: a chunk whose contribution bound / variation
+        risk exceeds them marches the fixed step (0: never)."""
         if net is None and fixed_log2 < 0:
             self._chk(self._L.tvk_set_step_proposal(self._h, None, None))
             self._net = None
             return
         p = _abi.ProposalParams(float(margin), float(skip_threshold), int(max_log2), float(fallback_bound),
-                                float(skip_bound), int(fixed_log2))
+                                float(skip_bound), int(fixed_log2), float(fallback_tv))
         self._chk(self._L.tvk_set_step_proposal(self._h, net._h if net is not None else None, C.byref(p)))
         self._net = net   # keep it alive while attached
 

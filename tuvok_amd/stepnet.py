@@ -142,29 +142,30 @@ def train_proposal(ren, rotations, cap=2_000_000, keep=1.0, tau=0.25 / 255, tau_
 
 def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.5, 1.0, 1.5, 2.0, 3.0),
               skips=(0.999, 1.0), fallbacks=(0.0, 0.05, 0.02, 0.01, 0.005, 0.002, 0.001), max_log2=3,
-              skip_bounds=(0.0,)):
+              skip_bounds=(0.0,), fallback_tvs=(0.005, 0.002, 0.001), tv_margins=(0.0, 1.0)):
     """The most aggressive proposal setting (least device time) whose frames stay
     within `bound` of the fixed-step frames on the given views (fallback 0: no
     fallback).  skip_bounds: contribution bounds below which a chunk is skipped
-    without the network (0: only chunks that cannot contribute).  The fixed
-    march itself is a candidate: when no setting is faster within the bound the
-    result is None (march the fixed step).  Returns (setting or None,
+    without the network (0: only chunks that cannot contribute).  fallback_tvs
+    (with tv_margins, no skip, no plain fallback): the variation-risk fallback.
+    The fixed march itself is a candidate: when no setting is faster within the
+    bound the result is None (march the fixed step).  Returns (setting or None,
     [(setting, max error, ms)])."""
     ren.SetStepProposal(None)
     refs = ren.PaintViews(rotations)
     ren.PaintViews(rotations, download=False)
     tried, best = [(None, 0.0, float(ren.last_stats.ms))], (None, float(ren.last_stats.ms))
-    for sb in skip_bounds:
-        for fb in fallbacks:
-            for m in margins:
-                for s in skips:
-                    prop = dict(margin=m, skip_threshold=s, max_log2=max_log2, skip_bound=sb, fallback_bound=fb)
-                    ren.SetStepProposal(net, **prop)
-                    imgs = ren.PaintViews(rotations)
-                    err = float(np.abs(imgs - refs).max())
-                    ms = float(ren.last_stats.ms)
-                    tried.append((prop, err, ms))
-                    if err <= bound and (best is None or ms < best[1]):
-                        best = (prop, ms)
+    props = [dict(margin=m, skip_threshold=s, max_log2=max_log2, skip_bound=sb, fallback_bound=fb)
+             for sb in skip_bounds for fb in fallbacks for m in margins for s in skips]
+    props += [dict(margin=m, skip_threshold=1.0, max_log2=max_log2, skip_bound=skip_bounds[0], fallback_bound=0.0,
+                   fallback_tv=tv) for tv in fallback_tvs for m in tv_margins]
+    for prop in props:
+        ren.SetStepProposal(net, **prop)
+        imgs = ren.PaintViews(rotations)
+        err = float(np.abs(imgs - refs).max())
+        ms = float(ren.last_stats.ms)
+        tried.append((prop, err, ms))
+        if err <= bound and (best is None or ms < best[1]):
+            best = (prop, ms)
     ren.SetStepProposal(None)
     return best[0], tried
