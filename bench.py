@@ -25,7 +25,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ERT_EPS = 2.0 ** -11
-DISPATCH_NAMES = {0: "raster", 1: "groups", 2: "rows"}
 
 
 def cpu_baseline(ren, vol, views, threads, budget_s=15.0):
@@ -174,7 +173,7 @@ def other_workloads(learned=True):
     print(f"[bench] bricking: {res.get('bricking_1024')} leaper: {res.get('leaper_1024')}", file=sys.stderr,
           flush=True)
     try:
-        from tuvok_amd.autotune import tune_dispatch_order
+        from tuvok_amd.autotune import dispatch_name, tune_dispatch_order
         c = workloads.SDF_ISO_2048
         r = workloads.setup(c)
         iso_order, _ = tune_dispatch_order(r, [np.eye(4, dtype=np.float32)])
@@ -189,7 +188,7 @@ def other_workloads(learned=True):
         r.close()
         m = float(np.median(ms))
         res["iso_sdf2048"] = dict(workload=c["name"], ms_per_frame=round(m, 3), frames_per_s=round(1e3 / m, 1),
-                                  dispatch_order=DISPATCH_NAMES[iso_order], roofline=hbm_roofline(alg, m))
+                                  dispatch=dispatch_name(iso_order), roofline=hbm_roofline(alg, m))
     except Exception as e:
         res["iso_sdf2048"] = dict(error=f"{type(e).__name__}: {e}")
     print(f"[bench] iso: {res['iso_sdf2048']}", file=sys.stderr, flush=True)
@@ -253,7 +252,8 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     views at 480x270, training on the device, calibration on 10 other views
     (bound 0.75/255, the fixed march a candidate), then `views` timed with and
     without the proposal and compared pixel by pixel."""
-    from tuvok_amd.autotune import tune_dispatch_order
+    from tuvok_amd import DISPATCH_QUEUES, DISPATCH_RASTER
+    from tuvok_amd.autotune import dispatch_name, tune_dispatch_order
     from tuvok_amd.stepnet import calibrate, train_proposal
     from tuvok_amd.renderer import rotation_y
     rng = np.random.default_rng(seed)
@@ -292,7 +292,7 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     learned, lms = timed(prop, label="learned")   # prop None: the calibration kept the fixed march
     ren.SetStepProposal(None)
     ren.SetSampleRateModifier(1.0)
-    ren.SetDispatchOrder(0)
+    ren.SetDispatchOrder(DISPATCH_RASTER | DISPATCH_QUEUES)   # the default again
     d = np.abs(learned - fixed)
     fms = float(np.median(t_fixed))
     return dict(reference_step_voxels=0.25, views=len(views), fixed_fps=round(len(views) / (fms / 1e3), 2),
@@ -302,7 +302,7 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
                 max_err=float(d.max()), mean_err=float(d.mean()), max_err_x255=round(float(d.max()) * 255, 3),
                 proposal=prop if prop is not None else "fixed march (no proposal setting faster within the bound)",
                 train_s=round(train_s, 1), train=info,
-                dispatch_order={k: DISPATCH_NAMES[v] for k, v in orders.items()})
+                dispatch={k: dispatch_name(v) for k, v in orders.items()})
 
 
 PROFILE = os.path.join(ROOT, "profiles", "march_fbm1024_lit2dtf_1080p_x16.md")
@@ -390,10 +390,10 @@ def main():
           f"views, {u_bricks} bricks / {u_bytes / 1e9:.2f} GB distinct)", file=sys.stderr, flush=True)
 
     # the group order of the launch (frames are the same in every order)
-    from tuvok_amd.autotune import tune_dispatch_order
+    from tuvok_amd.autotune import dispatch_name, tune_dispatch_order
     order, order_ms = tune_dispatch_order(ren, views)
-    print(f"[bench] dispatch order: {DISPATCH_NAMES[order]} "
-          f"({', '.join(f'{DISPATCH_NAMES[k]} {v:.2f} ms' for k, v in order_ms.items())})", file=sys.stderr, flush=True)
+    print(f"[bench] dispatch: {dispatch_name(order)} "
+          f"({', '.join(f'{dispatch_name(k)} {v:.2f} ms' for k, v in order_ms.items())})", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ren.PaintViews(views, download=False)
     ren.Synchronize()
@@ -464,7 +464,7 @@ def main():
         "config": {"workload": f"fbm_shells{n}_u16_2dtf_lit_1920x1080_x{nviews}views",
                    "volume": list(dims), "brick_stored": list(cfg["brick"]), "overlap": cfg["overlap"],
                    "mode": "2D TF + lighting", "step_voxels": "0.5 (fRayStepsize)", "views": nviews,
-                   "ert_eps": ERT_EPS, "dispatch_order": DISPATCH_NAMES[order]},
+                   "ert_eps": ERT_EPS, "dispatch": dispatch_name(order)},
         "frames_per_s": round(nviews / (ms_step / 1e3), 2),
         "samples_ref": n_ref,
         "samples_executed": n_exec,

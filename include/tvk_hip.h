@@ -141,18 +141,22 @@ int tvk_generate_volume(tvk_ctx* ctx, int kind, uint64_t seed, int dtype, const 
  * does (ExtendedOctreeConverter.inc:250-356) and need an overlap of at least 1
  * and bClampToEdge off when coarser levels are kept (TVK_ERR_UNSUPPORTED). */
 int tvk_set_lod_filter(tvk_ctx* ctx, int median);
-/* The order in which a multi-view render (tvk_render, tvk_render_views) hands
- * its 16x16 pixel groups to the GPU; no counterpart in the reference, whose
- * GLRaycaster draws one brick at a time (GLRaycaster.cpp:408-470).
- * TVK_DISPATCH_RASTER (default): each XCD runs a contiguous eighth of the groups
- * in raster order.  TVK_DISPATCH_GROUPS / TVK_DISPATCH_ROWS: the same eighths,
+/* How a multi-view render (tvk_render, tvk_render_views) hands its 16x16 pixel
+ * groups to the GPU; no counterpart in the reference, whose GLRaycaster draws
+ * one brick at a time (GLRaycaster.cpp:408-470).  The order:
+ * TVK_DISPATCH_RASTER: each XCD runs a contiguous eighth of the groups in
+ * raster order.  TVK_DISPATCH_GROUPS / TVK_DISPATCH_ROWS: the same eighths,
  * each run most costly group (or row of groups) first, the cost being the
  * samples a ray takes across the non-empty bricks covering the group -- long
- * rays then start early instead of trailing the launch.  Frames are identical
- * in every order. */
+ * rays then start early instead of trailing the launch.  Or'ed with
+ * TVK_DISPATCH_QUEUES: a persistent launch whose waves pull 8x8 tiles of those
+ * eighths, in that order, from one work queue per XCD (then from the other
+ * XCDs'), instead of one workgroup per group.  Default: RASTER | QUEUES.
+ * Frames are identical in every setting. */
 #define TVK_DISPATCH_RASTER 0
 #define TVK_DISPATCH_GROUPS 1
 #define TVK_DISPATCH_ROWS 2
+#define TVK_DISPATCH_QUEUES 4
 int tvk_set_dispatch_order(tvk_ctx* ctx, int order);
 /* Copy the retained linear LOD0 volume back to the host. */
 /* Makes dst render src's dataset: the brick pool (and kept linear volume) in HBM

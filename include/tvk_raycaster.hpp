@@ -56,6 +56,9 @@ class Raycaster {
   void SetPoolBudget(uint64_t bytes) { chk(tvk_set_pool_budget(ctx_, bytes)); }
   // ExtendedOctreeConverter's bComputeMedian for the next bricking
   void SetLODFilter(bool median) { chk(tvk_set_lod_filter(ctx_, median ? 1 : 0)); }
+  // TVK_DISPATCH_RASTER / _GROUPS / _ROWS: the order multi-view renders hand
+  // their pixel groups to the GPU (frames are identical in every order)
+  void SetDispatchOrder(int order) { chk(tvk_set_dispatch_order(ctx_, order)); }
   // SetClipPlane + EnableClipPlane / DisableClipPlane (AbstrRenderer.h:215-219)
   void SetClipPlane(const std::array<float, 4>& plane, bool enabled = true) {
     chk(tvk_set_clip_plane(ctx_, plane.data(), enabled ? 1 : 0));

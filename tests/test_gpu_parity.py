@@ -10,7 +10,8 @@ import pytest
 
 import oracle_ffi as O
 from helpers import blobs_volume, err_stats, oracle_camera_from, oracle_params_from
-from tuvok_amd import (DISPATCH_GROUPS, DISPATCH_RASTER, DISPATCH_ROWS, GEN_BLOBS, GEN_FBM_SHELLS, RM_1DTRANS,
+from tuvok_amd import (DISPATCH_GROUPS, DISPATCH_QUEUES, DISPATCH_RASTER, DISPATCH_ROWS, GEN_BLOBS, GEN_FBM_SHELLS,
+                       RM_1DTRANS,
                        RM_2DTRANS, RM_ISOSURFACE, HIPRaycaster, TvkError, rotation_y, tf1d_blue_white_red,
                        tf2d_gaussian_lobes)
 
@@ -348,9 +349,11 @@ def test_views_batch_equals_single(ren):
 @pytest.mark.parametrize("mode,lit,learned", [(RM_1DTRANS, True, False), (RM_2DTRANS, False, False),
                                               (RM_ISOSURFACE, True, False), (RM_1DTRANS, True, True)])
 def test_dispatch_orders_give_the_same_frames(ren, mode, lit, learned):
-    """The cost-ordered dispatches render the raster order's frames bit for bit
-    with the same sample count: views of a size with partial 16x16 groups, the
-    fixed march and the chunked march with its exact skip."""
+    """Every dispatch setting (raster, cost-ordered groups or rows, each with
+    one workgroup per group or with per-XCD work queues) renders the same
+    frames bit for bit with the same sample count: views of a size with
+    partial 16x16 groups, the fixed march and the chunked march with its exact
+    skip."""
     print(f"dispatch orders mode={mode} lit={lit} learned={learned}", flush=True)
     setup_blobs(ren, w=200, h=136)
     ren.Set2DTrans(tf2d_gaussian_lobes().GetByteArray())
@@ -360,22 +363,24 @@ def test_dispatch_orders_give_the_same_frames(ren, mode, lit, learned):
     if learned:
         ren.SetStepProposal(None, fixed_log2=0, skip_bound=0.0)
     rots = [rotation_y(a) for a in (0.0, 0.9, 2.1, 4.0, 5.5)]
+    settings = [o | q for q in (0, DISPATCH_QUEUES) for o in (DISPATCH_RASTER, DISPATCH_GROUPS, DISPATCH_ROWS)]
     ren.SetCountSamples(True)
     got = {}
     try:
-        for o in (DISPATCH_RASTER, DISPATCH_GROUPS, DISPATCH_ROWS):
+        for o in settings:
             ren.SetDispatchOrder(o)
             got[o] = (ren.PaintViews(rots), ren.last_stats.samples)
-        with pytest.raises(TvkError):
-            ren.SetDispatchOrder(3)
+        for bad in (3, 7, 8, -1):
+            with pytest.raises(TvkError):
+                ren.SetDispatchOrder(bad)
     finally:
-        ren.SetDispatchOrder(DISPATCH_RASTER)
+        ren.SetDispatchOrder(DISPATCH_RASTER | DISPATCH_QUEUES)
         ren.SetCountSamples(False)
         ren.SetStepProposal(None)
     ref, ns = got[DISPATCH_RASTER]
     assert ref.max() > 0.05 and ns > 0
-    for o in (DISPATCH_GROUPS, DISPATCH_ROWS):
-        assert np.array_equal(got[o][0], ref) and got[o][1] == ns
+    for o in settings[1:]:
+        assert np.array_equal(got[o][0], ref) and got[o][1] == ns, o
 
 
 def test_fbm_shells_u16_lit_2d_crop(ren):

@@ -9,7 +9,7 @@ TVK_OK = 0
 TVK_U8, TVK_U16 = 1, 2
 RM_1DTRANS, RM_2DTRANS, RM_ISOSURFACE = 0, 1, 2
 GEN_BLOBS, GEN_FBM_SHELLS, GEN_ELLIPSOID_SDF, GEN_PHANTOM = 1, 2, 3, 4
-DISPATCH_RASTER, DISPATCH_GROUPS, DISPATCH_ROWS = 0, 1, 2
+DISPATCH_RASTER, DISPATCH_GROUPS, DISPATCH_ROWS, DISPATCH_QUEUES = 0, 1, 2, 4
 
 # every symbol include/tvk_hip.h declares
 EXPORTS = [

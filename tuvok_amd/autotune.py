@@ -9,7 +9,14 @@ here, on the renderer that owns the dataset (its linear volume must be kept).
 """
 import numpy as np
 
-from ._abi import DISPATCH_GROUPS, DISPATCH_RASTER, DISPATCH_ROWS
+from ._abi import DISPATCH_GROUPS, DISPATCH_QUEUES, DISPATCH_RASTER, DISPATCH_ROWS
+
+DISPATCH_SETTINGS = tuple(o | q for q in (DISPATCH_QUEUES, 0) for o in (DISPATCH_RASTER, DISPATCH_GROUPS, DISPATCH_ROWS))
+
+
+def dispatch_name(d):
+    """'raster', 'groups' or 'rows', with '+queues' for the persistent launch."""
+    return ("raster", "groups", "rows")[d & 3] + ("+queues" if d & DISPATCH_QUEUES else "")
 
 DEFAULT_BRICKS = (34, 66, 130, 258)   # stored sizes: cores 32, 64, 128, 256 + 1-voxel overlap
 
@@ -32,12 +39,12 @@ def tune_brick_size(ren, rotations, candidates=DEFAULT_BRICKS, overlap=1, max_lo
     return best, times
 
 
-def tune_dispatch_order(ren, rotations, orders=(DISPATCH_RASTER, DISPATCH_GROUPS, DISPATCH_ROWS), reps=3):
-    """Renders `rotations` in every dispatch order (the renderer's current
-    mode, TF and step settings) and keeps the fastest launch.  Frames are the
-    same in every order; long rays behind many non-empty bricks favour the
-    cost orders, dense early-terminating volumes the raster one.  Returns
-    (best order, {order: median ms})."""
+def tune_dispatch_order(ren, rotations, orders=DISPATCH_SETTINGS, reps=3):
+    """Renders `rotations` with every dispatch setting (order x work queues;
+    the renderer's current mode, TF and step settings) and keeps the fastest
+    launch.  Frames are the same in every setting; long rays behind many
+    non-empty bricks favour the cost orders, dense early-terminating volumes
+    the raster one.  Returns (best setting, {setting: median ms})."""
     times = {}
     for o in orders:
         ren.SetDispatchOrder(o)

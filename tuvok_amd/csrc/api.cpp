@@ -325,6 +325,7 @@ void tvk_destroy(tvk_ctx* c) {
   if (c->d_list) (void)hipFree(c->d_list);
   if (c->d_views) (void)hipFree(c->d_views);
   if (c->d_order) (void)hipFree(c->d_order);
+  if (c->d_queues) (void)hipFree(c->d_queues);
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->d_counter) (void)hipFree(c->d_counter);
   if (c->d_touched) (void)hipFree(c->d_touched);
@@ -563,8 +564,8 @@ int tvk_set_lod_filter(tvk_ctx* c, int median) {
 
 int tvk_set_dispatch_order(tvk_ctx* c, int order) {
   if (!c) return TVK_ERR_ARG;
-  if (order < TVK_DISPATCH_RASTER || order > TVK_DISPATCH_ROWS)
-    return fail(c, TVK_ERR_ARG, "dispatch order must be TVK_DISPATCH_RASTER, _GROUPS or _ROWS");
+  if ((order & ~(3 | TVK_DISPATCH_QUEUES)) != 0 || (order & 3) > TVK_DISPATCH_ROWS)
+    return fail(c, TVK_ERR_ARG, "dispatch order must be TVK_DISPATCH_RASTER, _GROUPS or _ROWS, or'ed with _QUEUES");
   c->dispatch = order;
   return TVK_OK;
 }
@@ -1335,7 +1336,8 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
   if (pixels >= (1ull << 32)) return fail(c, TVK_ERR_ARG, "too many pixels in one call");
   if (fixed) return launch_fixed(c, nviews, pixels, groups, in_list, *fixed, st);
   // the dispatch order (raster: none)
-  const bool ordered = c->dispatch != TVK_DISPATCH_RASTER && groups > 0;
+  const int dorder = c->dispatch & 3;
+  const bool ordered = dorder != TVK_DISPATCH_RASTER && groups > 0;
   uint32_t xcd_start[8] = {}, xcd_count[8] = {};
   if (ordered) {
     c->h_cost.assign((size_t)groups, 0.0f);
@@ -1343,7 +1345,7 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
       const ViewConst& vc = c->h_views[(size_t)v];
       group_costs(vc, c->h_list.data() + vc.list_offset, c->h_cost.data() + vc.group_offset);
     }
-    dispatch_order(c->h_views.data(), nviews, c->h_cost, c->dispatch, c->h_order, xcd_start, xcd_count);
+    dispatch_order(c->h_views.data(), nviews, c->h_cost, dorder, c->h_order, xcd_start, xcd_count);
   }
   int rc = ensure(c, &c->d_list, &c->list_cap, std::max<size_t>(c->h_list.size(), 1));
   if (!rc) rc = ensure(c, &c->d_views, &c->views_cap, (size_t)nviews);
@@ -1410,6 +1412,20 @@ int render_impl(tvk_ctx* c, int nviews, const float* rots, float* host_out, tvk_
       most = std::max(most, xcd_count[x]);
     }
     a.groups_padded = 8u * most;
+  }
+  if ((c->dispatch & TVK_DISPATCH_QUEUES) && groups > 0) {
+    // persistent launch: waves pull 8x8 tiles from the 8 XCD ranges
+    if (!ordered) {
+      const uint32_t per = (groups + 7u) / 8u;
+      for (int x = 0; x < 8; ++x) {
+        a.xcd_start[x] = std::min(groups, (uint32_t)x * per);
+        a.xcd_count[x] = std::min(per, groups - a.xcd_start[x]);
+      }
+    }
+    rc = ensure(c, &c->d_queues, &c->queues_cap, 8);
+    if (rc) return rc;
+    HIPCHK(c, launch_fill_u32(c->d_queues, 8u, 0u, c->stream));
+    a.queues = c->d_queues;
   }
   if (c->count) HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipEventRecord(c->ev[6], c->stream));

@@ -150,6 +150,8 @@ struct tvk_ctx {
   size_t views_cap = 0;
   uint32_t* d_order = nullptr;   // dispatch order of the 16x16 groups
   size_t order_cap = 0;
+  unsigned int* d_queues = nullptr;   // the persistent launch's 8 XCD work queues
+  size_t queues_cap = 0;
   float* d_out = nullptr;
   size_t out_cap = 0;
   unsigned long long* d_counter = nullptr;
@@ -210,7 +212,7 @@ struct tvk_ctx {
   // second pool of padded and/or quantized bricks that rendering reads, made
   // on first use after the dataset or the flags change
   int compat_pow2 = 0, compat_8bit = 0;
-  int dispatch = 0;     // TVK_DISPATCH_* of multi-view renders
+  int dispatch = TVK_DISPATCH_RASTER | TVK_DISPATCH_QUEUES;   // of multi-view renders
   int lod_median = 0;   // ExtendedOctreeConverter's bComputeMedian for the next bricking
   struct CompatPool {
     void* d = nullptr;

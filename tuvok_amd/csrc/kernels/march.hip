@@ -1010,41 +1010,77 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
 }
 
 // Multi-view launch: one 256-thread workgroup = 16x16 pixels = 4 waves of 8x8.
+// Without work queues each workgroup renders one 16x16 group: consecutive
+// workgroups go round-robin over the 8 XCDs, and each XCD runs a contiguous
+// range of groups, in the host's order (heaviest first) when there is one.
+// With work queues (A.queues, cleared to 0 before the launch) the grid is
+// persistent: every wave pulls 8x8 tiles -- item t of XCD x's queue is
+// quarter t & 3 of group order[xcd_start[x] + (t >> 2)] (or that index) --
+// first from its own XCD's queue, then from the others in turn, and exits when
+// all 8 are drained (every wave reaches that exit).  A wave that draws a light
+// tile comes back for another at once instead of waiting for its workgroup.
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
 __device__ __forceinline__ void march_group(const MarchArgs& A) {
   extern __shared__ float4 s_tf[];  // 1D TF (tf_n entries, MODE 0 only)
   __shared__ __align__(16) uint16_t s_sn[LEARN > 0 ? 4 * SN_WAVE_LDS : 8];   // step network scratch, one slice per wave
   __shared__ uint64_t s_cols[MODE == 1 ? 4 * 64 : 1];                          // 2D TF column masks, one slice per wave
 
-  // XCD-aware remap: consecutive workgroups go round-robin over the 8 XCDs;
-  // each XCD runs a contiguous range of 16x16 groups, in the host's order
-  // (heaviest first) when there is one.
   const uint32_t lin = blockIdx.x, xcd = lin & 7u, j = lin >> 3;
-  uint32_t g;
-  if (A.order) {
-    if (j >= A.xcd_count[xcd]) return;  // uniform for the whole workgroup
-    g = A.order[A.xcd_start[xcd] + j];
-  } else {
-    g = xcd * (A.groups_padded >> 3) + j;
-    if (g >= A.groups_total) return;
-  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // renders quarter q (x = q & 1, y = q >> 1) of 16x16 group g with this wave
+  auto tile = [&](uint32_t g, int q) {
+    int v = 0;
+    while (v + 1 < A.nviews && g >= A.views[v + 1].group_offset) ++v;
+    const ViewConst* VC = &A.views[v];
+    const uint32_t gl = g - VC->group_offset;
+    const uint32_t gx = gl % VC->groups_x, gy = gl / VC->groups_x;
+    const int tx = (int)gx * 2 + (q & 1), ty = (int)gy * 2 + (q >> 1);
+    march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
+                                           A.counter, (COUNT && A.touched) ? A.touched + (size_t)v * A.ntoc : nullptr,
+                                           &A.learn, &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v,
+                                           s_cols + (MODE == 1 ? wave * 64 : 0), A.accumulate != 0, A.depth, A.hitpos);
+  };
 
+  // without queues: this workgroup's group (those past the range leave
+  // before the barrier, the whole workgroup at once)
+  uint32_t g0 = 0;
+  if (A.queues == nullptr) {
+    if (A.order) {
+      if (j >= A.xcd_count[xcd]) return;
+      g0 = A.order[A.xcd_start[xcd] + j];
+    } else {
+      g0 = xcd * (A.groups_padded >> 3) + j;
+      if (g0 >= A.groups_total) return;
+    }
+  }
   if (MODE == 0) {
     for (int i = threadIdx.x; i < A.S.tf_n; i += 256) s_tf[i] = A.S.tf1[i];
-    __syncthreads();
+    __syncthreads();   // the only workgroup barrier: the waves run independently after it
   }
-
-  int v = 0;
-  while (v + 1 < A.nviews && g >= A.views[v + 1].group_offset) ++v;
-  const ViewConst* VC = &A.views[v];
-  const uint32_t gl = g - VC->group_offset;
-  const uint32_t gx = gl % VC->groups_x, gy = gl / VC->groups_x;
-  const int wave = threadIdx.x >> 6;
-  const int tx = (int)gx * 2 + (wave & 1), ty = (int)gy * 2 + (wave >> 1);
-  march_tile<MODE, LIT, T, COUNT, LEARN>(A.S, s_tf, VC, A.list + VC->list_offset, A.out, tx * 8, ty * 8, 8, 8,
-                                         A.counter, (COUNT && A.touched) ? A.touched + (size_t)v * A.ntoc : nullptr, &A.learn,
-                                         &A.seg, s_sn + (LEARN > 0 ? wave * SN_WAVE_LDS : 0), (uint32_t)v,
-                                         s_cols + (MODE == 1 ? wave * 64 : 0), A.accumulate != 0, A.depth, A.hitpos);
+  uint32_t k = 0;   // queues tried so far, starting with this XCD's own
+  for (int it = 0;; ++it) {
+    uint32_t g;
+    int q;
+    if (A.queues == nullptr) {
+      if (it > 0) return;
+      g = g0;
+      q = wave;
+    } else {
+      uint32_t t = 0, x = 0;
+      for (; k < 8; ++k) {
+        x = (xcd + k) & 7u;
+        uint32_t tt = 0;
+        if (lane == 0) tt = atomicAdd(&A.queues[x], 1u);
+        tt = (uint32_t)__builtin_amdgcn_readfirstlane((int)tt);
+        if (tt < A.xcd_count[x] * 4u) { t = tt; break; }   // else drained (the counter grows past it: harmless)
+      }
+      if (k == 8) return;
+      const uint32_t gi = A.xcd_start[x] + (t >> 2);
+      g = A.order ? (uint32_t)__builtin_amdgcn_readfirstlane((int)A.order[gi]) : gi;
+      q = (int)(t & 3u);
+    }
+    tile(g, q);
+  }
 }
 
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
@@ -1117,19 +1153,38 @@ hipError_t launch_fill_u32(unsigned int* p, unsigned int n, unsigned int v, hipS
 }
 
 // host-side launchers (extern "C++" within the library)
+
+// Persistent grid of a kernel: the workgroups the device holds at once (its
+// occupancy at this LDS size times the CU count), a multiple of 8 XCDs.
+static dim3 persistent_grid(const void* kernel, size_t lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) != hipSuccess || per_cu <= 0) per_cu = 1;
+  const uint32_t n = (uint32_t)per_cu * (uint32_t)cus;
+  return dim3((n + 7u) & ~7u);
+}
+
+template <typename K>
+static void launch_k(K kernel, const MarchArgs& a, dim3 grid, size_t lds, hipStream_t st) {
+  if (a.queues) grid = persistent_grid(reinterpret_cast<const void*>(kernel), lds);
+  hipLaunchKernelGGL(kernel, grid, dim3(256), lds, st, a);
+}
+
 template <int M, bool L, typename T>
 static void launch_one(const MarchArgs& a, dim3 grid, size_t lds, hipStream_t st, int learn) {
   if (learn == 2 && M != 2)
-    hipLaunchKernelGGL((march_kernel<M, L, T, false, (M != 2 ? 2 : 0)>), grid, dim3(256), lds, st, a);
+    launch_k(march_kernel<M, L, T, false, (M != 2 ? 2 : 0)>, a, grid, lds, st);
   else if (learn == 1 && M != 2) {
     if (a.counter)
-      hipLaunchKernelGGL((march_kernel_learn<M, L, T, true>), grid, dim3(256), lds, st, a);
+      launch_k(march_kernel_learn<M, L, T, true>, a, grid, lds, st);
     else
-      hipLaunchKernelGGL((march_kernel_learn<M, L, T, false>), grid, dim3(256), lds, st, a);
+      launch_k(march_kernel_learn<M, L, T, false>, a, grid, lds, st);
   } else if (a.counter)
-    hipLaunchKernelGGL((march_kernel<M, L, T, true, 0>), grid, dim3(256), lds, st, a);
+    launch_k(march_kernel<M, L, T, true, 0>, a, grid, lds, st);
   else
-    hipLaunchKernelGGL((march_kernel<M, L, T, false, 0>), grid, dim3(256), lds, st, a);
+    launch_k(march_kernel<M, L, T, false, 0>, a, grid, lds, st);
 }
 
 // learn: 0 fixed step, 1 step proposal (a.learn), 2 fixed step recording segment samples (a.seg)

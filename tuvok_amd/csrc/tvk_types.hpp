@@ -174,6 +174,11 @@ struct MarchArgs {
   // xcd_count[x] and groups_padded is 8 x the largest count.
   const uint32_t* order;
   uint32_t xcd_start[8], xcd_count[8];
+  // Work queues (or nullptr): 8 counters cleared before the launch; the grid
+  // is then persistent and every wave pulls 8x8 tiles, item t of XCD x being
+  // quarter t & 3 of the range's group t >> 2 (xcd_start / xcd_count as above,
+  // in the order when there is one).
+  unsigned int* queues;
   int accumulate;               // continue the pixels' values (and depths) of an earlier launch
   float* depth;                 // isosurface: per-pixel depth of the best hit so far (or nullptr)
   float4* hitpos;               // isosurface: per-pixel eye-space hit and its segment parameter (or nullptr)

@@ -96,9 +96,11 @@ class HIPRaycaster:
         return info
 
     def SetDispatchOrder(self, order):
-        """The order multi-view renders hand their 16x16 groups to the GPU:
-        DISPATCH_RASTER (default), DISPATCH_GROUPS or DISPATCH_ROWS (most costly
-        first within each XCD's eighth).  Frames are identical in every order."""
+        """How multi-view renders hand their 16x16 groups to the GPU:
+        DISPATCH_RASTER, DISPATCH_GROUPS or DISPATCH_ROWS (most costly first
+        within each XCD's eighth), or'ed with DISPATCH_QUEUES (a persistent
+        launch whose waves pull 8x8 tiles from per-XCD queues; default
+        RASTER | QUEUES).  Frames are identical in every setting."""
         self._chk(self._L.tvk_set_dispatch_order(self._h, int(order)))
         self._dispatch = int(order)
 
