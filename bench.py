@@ -355,6 +355,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-step legs")
     ap.add_argument("--no-phantom", action="store_true", help="skip the learned-step leg on the 2048^3 phantom")
     ap.add_argument("--size", type=int, default=1024, help="volume edge (default: the 1024^3 workload)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default dispatch (raster order, work queues) instead of timing the settings")
     ap.add_argument("--no-other", action="store_true",
                     help="skip the isosurface, 48-request batch and 64 GiB LOD batch measurements")
     args = ap.parse_args()
@@ -391,7 +393,11 @@ def main():
 
     # the group order of the launch (frames are the same in every order)
     from tuvok_amd.autotune import dispatch_name, tune_dispatch_order
-    order, order_ms = tune_dispatch_order(ren, views)
+    if args.no_tune:
+        from tuvok_amd import DISPATCH_QUEUES, DISPATCH_RASTER
+        order, order_ms = DISPATCH_RASTER | DISPATCH_QUEUES, {}
+    else:
+        order, order_ms = tune_dispatch_order(ren, views)
     print(f"[bench] dispatch: {dispatch_name(order)} "
           f"({', '.join(f'{dispatch_name(k)} {v:.2f} ms' for k, v in order_ms.items())})", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
