@@ -12,6 +12,7 @@
 // reference samples the same point through a 3D texture atlas).  The float
 // operations are those of the oracle in the same order (-ffp-contract=off).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "../tvk_types.hpp"
 #include "common.hpp"
 
@@ -259,9 +260,7 @@ __device__ __forceinline__ uint32_t compute_lod(const LeapArgs& A, float depth) 
 }
 
 template <int MODE, bool LIT, typename T>
-__global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint32_t tile = blockIdx.x * 4 + wave;
+__device__ __forceinline__ void leap_tile(const LeapArgs& A, uint32_t tile, int lane) {
   const int tx = (int)(tile % A.tiles_x), ty = (int)(tile / A.tiles_x);
   const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
   if (px >= A.W || py >= A.H) return;   // no wave-level operations below
@@ -528,9 +527,43 @@ __global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
   if (A.counter && ns) atomicAdd(A.counter, (unsigned long long)ns);
 }
 
+// one 8x8 tile per wave of the grid
+template <int MODE, bool LIT, typename T>
+__global__ __launch_bounds__(256) void leap_kernel(LeapArgs A) {
+  leap_tile<MODE, LIT, T>(A, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+// persistent grid: every wave pulls 8x8 tiles from A.queue until tiles
+// 0 .. ntiles-1 are drained (every wave reaches that exit; the counter keeps
+// growing past ntiles: harmless)
+template <int MODE, bool LIT, typename T>
+__global__ __launch_bounds__(256) void leap_kernel_queue(LeapArgs A) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(A.queue, 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    if (t >= A.ntiles) break;
+    leap_tile<MODE, LIT, T>(A, t, lane);
+  }
+}
+
 template <int M, bool L, typename T>
 void launch_leap_one(const LeapArgs& a, uint32_t blocks, hipStream_t st) {
-  hipLaunchKernelGGL((leap_kernel<M, L, T>), dim3(blocks), dim3(256), 0, st, a);
+  if (a.queue) {
+    // persistent grid: the workgroups the device holds at once
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(leap_kernel_queue<M, L, T>), 256, 0) != hipSuccess || per_cu <= 0)
+      per_cu = 1;
+    blocks = std::min(blocks, (uint32_t)per_cu * (uint32_t)cus);
+    hipLaunchKernelGGL((leap_kernel_queue<M, L, T>), dim3(blocks), dim3(256), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((leap_kernel<M, L, T>), dim3(blocks), dim3(256), 0, st, a);
+  }
 }
 
 }  // namespace

@@ -234,5 +234,10 @@ struct LeapArgs {
   float lod_factor, l0err, srm;
   int nlod, W, H, first, lighting, mode, overlap;
   uint32_t tiles_x;
+  // work queue (or nullptr: one 8x8 tile per wave of the grid): a counter
+  // cleared before the launch; the grid is then persistent and every wave
+  // pulls tiles 0 .. ntiles-1 in order until they are drained
+  unsigned int* queue;
+  uint32_t ntiles;
 };
 }  // namespace tvk
