@@ -298,31 +298,45 @@ template <typename T> __device__ __forceinline__ T filter_taps(const T* t, int n
 // 2x filter of a level volume (mean or median); an axis of size 1 is not halved
 // (ExtendedOctree.cpp:201-213).  With even brick cores the reference's
 // brick-to-brick filtering (ExtendedOctreeConverter.inc) equals this
-// whole-level form: the 2x2x2 groups never straddle a brick.
+// whole-level form: the 2x2x2 groups never straddle a brick.  Grid: x along
+// blockIdx.x * 256 + threadIdx.x, destination rows (y, z) along blockIdx.y in
+// steps of gridDim.y, so no index is divided per voxel.  A voxel whose eight
+// taps all exist loads them together (no per-tap branch); the taps keep the
+// x-major order (dx, then dy, then dz) the filter sees in either form.
 template <typename T>
 __global__ __launch_bounds__(256) void downsample(const T* __restrict__ src, uint64_t SX, uint64_t SY, uint64_t SZ,
                                                   T* __restrict__ dst, uint64_t DX, uint64_t DY, uint64_t DZ,
                                                   int median) {
-  const uint64_t n = DX * DY * DZ;
   const uint64_t fx = SX > 1 ? 2 : 1, fy = SY > 1 ? 2 : 1, fz = SZ > 1 ? 2 : 1;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-    const uint64_t x = i % DX, r = i / DX, y = r % DY, z = r / DY;
+  const uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (x >= DX) return;
+  const uint64_t rows = DY * DZ;
+  for (uint64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+    const uint64_t y = r % DY, z = r / DY;   // once per row and thread, not per voxel
     T taps[8];
     int cnt = 0;
-    for (uint64_t dx = 0; dx < fx; ++dx) {
-      const uint64_t xx = x * fx + dx;
-      if (xx >= SX) continue;
-      for (uint64_t dy = 0; dy < fy; ++dy) {
-        const uint64_t yy = y * fy + dy;
-        if (yy >= SY) continue;
-        for (uint64_t dz = 0; dz < fz; ++dz) {
-          const uint64_t zz = z * fz + dz;
-          if (zz >= SZ) continue;
-          taps[cnt++] = src[xx + SX * (yy + SY * zz)];
+    if (fx == 2 && fy == 2 && fz == 2 && 2 * x + 1 < SX && 2 * y + 1 < SY && 2 * z + 1 < SZ) {
+      const T* __restrict__ s0 = src + 2 * x + SX * (2 * y + SY * (2 * z));
+      const uint64_t oy = SX, oz = SX * SY;
+      taps[0] = s0[0]; taps[1] = s0[oz]; taps[2] = s0[oy]; taps[3] = s0[oy + oz];
+      taps[4] = s0[1]; taps[5] = s0[1 + oz]; taps[6] = s0[1 + oy]; taps[7] = s0[1 + oy + oz];
+      cnt = 8;
+    } else {
+      for (uint64_t dx = 0; dx < fx; ++dx) {
+        const uint64_t xx = x * fx + dx;
+        if (xx >= SX) continue;
+        for (uint64_t dy = 0; dy < fy; ++dy) {
+          const uint64_t yy = y * fy + dy;
+          if (yy >= SY) continue;
+          for (uint64_t dz = 0; dz < fz; ++dz) {
+            const uint64_t zz = z * fz + dz;
+            if (zz >= SZ) continue;
+            taps[cnt++] = src[xx + SX * (yy + SY * zz)];
+          }
         }
       }
     }
-    dst[i] = filter_taps(taps, cnt, median != 0);
+    dst[x + DX * r] = filter_taps(taps, cnt, median != 0);
   }
 }
 
@@ -393,11 +407,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void brick_minmax(const T* __restrict__ pool, const GatherBrick* __restrict__ bricks,
                                                     uint32_t* __restrict__ out) {
   constexpr int lx = sizeof(T) == 1 ? 3 : 2, W = 1 << lx;
+  __shared__ uint32_t s_mm[2][4];
   const GatherBrick b = bricks[blockIdx.y];
   const uint64_t rows = padded_count(b.n, lx) >> (lx + 1);
   uint32_t mn = 0xffffffffu, mx = 0u;
-  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < rows; j += (uint64_t)gridDim.x * 256) {
-    const uint2 w = *reinterpret_cast<const uint2*>(pool + b.offset + j * (2 * W));   // the own half
+  auto fold = [&](uint2 w) {
     const uint32_t ww[2] = {w.x, w.y};
 #pragma unroll
     for (int e = 0; e < W; ++e) {
@@ -405,12 +419,29 @@ __global__ __launch_bounds__(256) void brick_minmax(const T* __restrict__ pool, 
       mn = min(mn, v);
       mx = max(mx, v);
     }
+  };
+  // four rows per lane and pass, their loads issued together
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  for (; j + 3 * stride < rows; j += 4 * stride) {
+    uint2 w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = *reinterpret_cast<const uint2*>(pool + b.offset + (j + q * stride) * (2 * W));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) fold(w[q]);
   }
+  for (; j < rows; j += stride) fold(*reinterpret_cast<const uint2*>(pool + b.offset + j * (2 * W)));   // the own half
   for (int o = 32; o > 0; o >>= 1) {
     mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, 64));
     mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
   }
-  if ((threadIdx.x & 63) == 0) {
+  // one atomic pair per workgroup: the brick's counters are shared by all of its workgroups
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { s_mm[0][wv] = mn; s_mm[1][wv] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mn = min(min(s_mm[0][0], s_mm[0][1]), min(s_mm[0][2], s_mm[0][3]));
+    mx = max(max(s_mm[1][0], s_mm[1][1]), max(s_mm[1][2], s_mm[1][3]));
     atomicMin(out + 2 * blockIdx.y, mn);
     atomicMax(out + 2 * blockIdx.y + 1, mx);
   }
@@ -698,15 +729,18 @@ hipError_t launch_gather(const void* lin, int bpv, const uint64_t dims[3], const
 
 hipError_t launch_downsample(const void* src, const uint64_t sd[3], void* dst, const uint64_t dd[3], int bpv,
                              hipStream_t st, int median) {
-  const uint64_t n = dd[0] * dd[1] * dd[2];
-  uint64_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
+  const uint64_t rows = dd[1] * dd[2];
+  if (dd[0] == 0 || rows == 0) return hipSuccess;
+  const uint64_t bx = (dd[0] + 255) / 256;
+  // about 2^18 workgroups in all; each takes rows blockIdx.y, + gridDim.y, ...
+  const uint64_t by = std::max<uint64_t>(1, std::min<uint64_t>({rows, 65535, ((uint64_t)1 << 18) / bx}));
+  const dim3 grid((uint32_t)bx, (uint32_t)by);
   if (bpv == 1)
-    hipLaunchKernelGGL(downsample<uint8_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint8_t*)src, sd[0], sd[1],
-                       sd[2], (uint8_t*)dst, dd[0], dd[1], dd[2], median);
+    hipLaunchKernelGGL(downsample<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)src, sd[0], sd[1], sd[2],
+                       (uint8_t*)dst, dd[0], dd[1], dd[2], median);
   else
-    hipLaunchKernelGGL(downsample<uint16_t>, dim3((uint32_t)blocks), dim3(256), 0, st, (const uint16_t*)src, sd[0],
-                       sd[1], sd[2], (uint16_t*)dst, dd[0], dd[1], dd[2], median);
+    hipLaunchKernelGGL(downsample<uint16_t>, grid, dim3(256), 0, st, (const uint16_t*)src, sd[0], sd[1], sd[2],
+                       (uint16_t*)dst, dd[0], dd[1], dd[2], median);
   return hipGetLastError();
 }
 
@@ -836,8 +870,9 @@ hipError_t launch_crop(void* vol, int bpv, const uint64_t dims[3], const float p
 
 hipError_t launch_minmax(const void* pool, int bpv, const GatherBrick* d_bricks, uint32_t nb, uint32_t max_vox,
                          uint32_t* out, hipStream_t st) {
-  uint32_t gx = (max_vox / (bpv == 1 ? 16 : 8) + 255) / 256;
-  if (gx > 512) gx = 512;
+  // about 16 rows per lane (4 passes of 4), at most 64 workgroups per brick
+  uint32_t gx = (max_vox / (bpv == 1 ? 16 : 8) + 4095) / 4096;
+  if (gx > 64) gx = 64;
   if (gx == 0) gx = 1;
   hipLaunchKernelGGL(init_minmax, dim3((nb + 255) / 256), dim3(256), 0, st, out, nb);
   for (uint32_t first = 0; first < nb; first += 65535u) {
