@@ -175,7 +175,9 @@ def other_workloads(learned=True):
     try:
         from tuvok_amd.autotune import dispatch_name, tune_dispatch_order
         c = workloads.SDF_ISO_2048
-        r = workloads.setup(c)
+        # early termination on: for the isosurface it only skips the bricks behind
+        # a ray's hit (the same frame as the exact mode, fewer samples)
+        r = workloads.setup(c, ert_eps=ERT_EPS)
         iso_order, _ = tune_dispatch_order(r, [np.eye(4, dtype=np.float32)])
         r.SetCountSamples(True)
         r.Paint(download=False)

@@ -496,6 +496,11 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
   bool have_hit = false;
   V3 hit_pos = mk(0, 0, 0), hit_tex = mk(0, 0, 0);
   int hit_brick = 0;
+  // ray parameter of this launch's best hit: with early termination on, a brick
+  // whose segment starts beyond it (with a margin far above the fp16 rounding
+  // of the entry point) cannot give a nearer hit -- every hit of a segment lies
+  // on it, from its entry on -- and is skipped: the same frame, fewer samples
+  float best_t = INFINITY;
 
   const int nlist = (int)VC->nlist;
   for (int base = 0; base < nlist; base += 64) {
@@ -542,7 +547,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
       }
       ok = ok && tf > tn && tf > 0.0f;
       const float te = fmaxf(tn, 1.0f);  // near-plane entry when the front face is clipped
-      const bool active = !done && ok && tf > te;
+      const bool active = !done && ok && tf > te && !(MODE == 2 && eps > 0.0f && te > best_t * (1.0f + 0x1p-8f));
       if (COUNT && touched) {
         // every ray that reaches a brick takes at least one sample there (the
         // record is optional: the allocation-free render counts samples only)
@@ -625,6 +630,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         if (depth < best_depth) {
           best_depth = depth;
           have_hit = true;
+          best_t = te + param * (tf - te);
           hit_pos = he;
           hit_param = param;
           hit_tex = pos;   // the normal is taken here, after the brick loop
