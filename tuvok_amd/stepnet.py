@@ -151,10 +151,18 @@ def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.5, 1.0, 1.5,
     The fixed march itself is a candidate: when no setting is faster within the
     bound the result is None (march the fixed step).  Returns (setting or None,
     [(setting, max error, ms)])."""
+    def timed():
+        # the faster of two launches: one noisy launch can hide the best setting
+        ms = []
+        for _ in range(2):
+            ren.PaintViews(rotations, download=False)
+            ms.append(float(ren.last_stats.ms))
+        return min(ms)
+
     ren.SetStepProposal(None)
     refs = ren.PaintViews(rotations)
-    ren.PaintViews(rotations, download=False)
-    tried, best = [(None, 0.0, float(ren.last_stats.ms))], (None, float(ren.last_stats.ms))
+    t_fixed = timed()
+    tried, best = [(None, 0.0, t_fixed)], (None, t_fixed)
     props = [dict(margin=m, skip_threshold=s, max_log2=max_log2, skip_bound=sb, fallback_bound=fb)
              for sb in skip_bounds for fb in fallbacks for m in margins for s in skips]
     props += [dict(margin=m, skip_threshold=1.0, max_log2=max_log2, skip_bound=skip_bounds[0], fallback_bound=0.0,
@@ -163,7 +171,7 @@ def calibrate(ren, net, rotations, bound=0.9 / 255, margins=(0.0, 0.5, 1.0, 1.5,
         ren.SetStepProposal(net, **prop)
         imgs = ren.PaintViews(rotations)
         err = float(np.abs(imgs - refs).max())
-        ms = float(ren.last_stats.ms)
+        ms = timed() if err <= bound else float(ren.last_stats.ms)
         tried.append((prop, err, ms))
         if err <= bound and (best is None or ms < best[1]):
             best = (prop, ms)
