@@ -152,7 +152,8 @@ int tvk_set_lod_filter(tvk_ctx* ctx, int median);
  * TVK_DISPATCH_QUEUES: a persistent launch whose waves pull 8x8 tiles of those
  * eighths, in that order, from one work queue per XCD (then from the other
  * XCDs'), instead of one workgroup per group.  Default: RASTER | QUEUES.
- * Frames are identical in every setting. */
+ * Frames are identical in every setting.  tvk_render_views_noalloc always
+ * runs raster order with one workgroup per group (it allocates nothing). */
 #define TVK_DISPATCH_RASTER 0
 #define TVK_DISPATCH_GROUPS 1
 #define TVK_DISPATCH_ROWS 2
