@@ -776,6 +776,9 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         // the chunk when SN_PROBE steps span at most 6 voxels.  Without overlap
         // (and GL_CLAMP) the border colour 0 can enter the interpolation.
         const bool strict = (float)SN_PROBE * dl <= 6.0f;
+        // the probe spacing: as many steps as span just under 6 voxels (at most
+        // a chunk), SN_PROBE where that bound does not hold
+        const int pstep = strict ? max(SN_PROBE, (int)fminf((float)SN_CHUNK, 5.99f / dl)) : SN_PROBE;
         const bool border0 = !edge && S.overlap == 0;
         // the network inputs that stay fixed over the segment, put(index, value)
         auto seg_features = [&](auto&& put) {
@@ -814,9 +817,10 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           nf = min(SN_CHUNK, steps - i0);
           T0 = tray * (1.0f - c3);
           uint32_t vlo = 0xffffu, vhi = 0u;
-#pragma unroll
-          for (int q = 0; q <= SN_CHUNK; q += SN_PROBE) {
-            const V3 p = pos + incTex * (float)min(q, nf - 1);
+          // probes at steps 0, pstep, 2 pstep, ... and the chunk's last step
+          for (int q = 0;; q += pstep) {
+            const int qs = min(q, nf - 1);
+            const V3 p = pos + incTex * (float)qs;
             const float pc[3] = {p.x, p.y, p.z};
             const int nn3[3] = {nx, ny, nz};
             int ci[3];
@@ -828,6 +832,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             const uint32_t w = cg[((uint32_t)ci[2] * (uint32_t)ncy + (uint32_t)ci[1]) * (uint32_t)ncx + (uint32_t)ci[0]];
             vlo = min(vlo, w & 0xffffu);
             vhi = max(vhi, w >> 16);
+            if (qs == nf - 1) break;
           }
           if (border0) vlo = 0u;
           // TF entries of the range (one stored value of slack on either side)
