@@ -281,37 +281,47 @@ __device__ __forceinline__ float sample_brick_iso(const T* __restrict__ d, int n
                   f[0], f[1], f[2]) * Unorm<T>::inv();
 }
 
-// Two isosurface search samples at once (p and q): when both 2x2x2
-// neighbourhoods of every lane lie inside the brick, their eight x-pair loads
-// are issued together before either value is formed; the values and float
-// operations are sample_brick_iso's.
+// ISO_BATCH isosurface search samples at once (p, p + inc, ...): when every
+// lane's 2x2x2 neighbourhoods of all of them lie inside the brick, their
+// x-pair loads are issued together before any value is formed; the positions
+// (accumulated one increment at a time), values and float operations are
+// sample_brick_iso's.
+#ifndef TVK_ISO_BATCH
+#define TVK_ISO_BATCH 4
+#endif
+constexpr int ISO_BATCH = TVK_ISO_BATCH;
 template <typename T>
-__device__ __forceinline__ void sample_brick_iso2(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p,
-                                                  V3 q, float& vp, float& vq) {
-  int i0[2][3];
-  float f[2][3];
+__device__ __forceinline__ void sample_brick_iso_n(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p,
+                                                   V3 inc, float (&val)[ISO_BATCH]) {
+  V3 ps[ISO_BATCH];
+  ps[0] = p;
+#pragma unroll
+  for (int s = 1; s < ISO_BATCH; ++s) ps[s] = ps[s - 1] + inc;
+  int i0[ISO_BATCH][3];
+  float f[ISO_BATCH][3];
   const int n[3] = {nx, ny, nz};
-  const float pc[2][3] = {{p.x, p.y, p.z}, {q.x, q.y, q.z}};
   bool in = true;
 #pragma unroll
-  for (int s = 0; s < 2; ++s)
+  for (int s = 0; s < ISO_BATCH; ++s) {
+    const float pc[3] = {ps[s].x, ps[s].y, ps[s].z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const float c = fminf(fmaxf(pc[s][a], 0.0f), 1.0f), u = c * (float)n[a] - 0.5f, fl = floorf(u);
+      const float c = fminf(fmaxf(pc[a], 0.0f), 1.0f), u = c * (float)n[a] - 0.5f, fl = floorf(u);
       i0[s][a] = (int)fl;
       f[s][a] = u - fl;
       in = in && i0[s][a] >= 0 && i0[s][a] + 1 < n[a];
     }
+  }
   if (__ballot(!in) != 0) {
-    vp = sample_brick_iso<T>(d, nx, ny, nz, edge, p);
-    vq = sample_brick_iso<T>(d, nx, ny, nz, edge, q);
+#pragma unroll
+    for (int s = 0; s < ISO_BATCH; ++s) val[s] = sample_brick_iso<T>(d, nx, ny, nz, edge, ps[s]);
     return;
   }
   const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
   const uint8_t* __restrict__ b = reinterpret_cast<const uint8_t*>(d);
-  uint32_t v[2][2][2][2];
+  uint32_t v[ISO_BATCH][2][2][2];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < ISO_BATCH; ++s) {
     const uint32_t x0 = ox<T>(i0[s][0]);
 #pragma unroll
     for (int j = 0; j < 2; ++j)
@@ -319,9 +329,8 @@ __device__ __forceinline__ void sample_brick_iso2(const T* __restrict__ d, int n
       for (int k = 0; k < 2; ++k)
         run2<T>(b, (x0 + oy<T>(i0[s][1] + j, nbx) + oz<T>(i0[s][2] + k, nbxy)) * (uint32_t)sizeof(T), v[s][j][k]);
   }
-  float r[2];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < ISO_BATCH; ++s) {
     float c[2][2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j)
@@ -330,11 +339,9 @@ __device__ __forceinline__ void sample_brick_iso2(const T* __restrict__ d, int n
         c[0][j][k] = (float)v[s][j][k][0];
         c[1][j][k] = (float)v[s][j][k][1];
       }
-    r[s] = tri_lerp(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1],
-                    f[s][0], f[s][1], f[s][2]) * Unorm<T>::inv();
+    val[s] = tri_lerp(c[0][0][0], c[1][0][0], c[0][1][0], c[1][1][0], c[0][0][1], c[1][0][1], c[0][1][1], c[1][1][1],
+                      f[s][0], f[s][1], f[s][2]) * Unorm<T>::inv();
   }
-  vp = r[0];
-  vq = r[1];
 }
 
 // Interior form (every lane's 4x4x4 neighbourhood inside the brick): the four
@@ -663,22 +670,21 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
 
       if (MODE == 2) {
         // GLRaycaster-ISO-FS.glsl:56-105 + RefineIsosurface.glsl:37-52
-        // two samples per iteration, their loads in flight together; the
-        // samples, their order and the count are the one-by-one search's (the
-        // second value of the last odd step is formed and not used)
+        // ISO_BATCH samples per iteration, their loads in flight together; the
+        // samples, their order and the count are the one-by-one search's (values
+        // past the segment's last step are formed and not used)
         V3 pos = entryTex;
         bool found = false;
-        for (int i = 0; i < steps; i += 2) {
-          const V3 p1 = pos + incTex;
-          float v0, v1;
-          sample_brick_iso2<T>(bd, nx, ny, nz, edge, pos, p1, v0, v1);
-          ++ns;
-          if (v0 >= S.iso_n) { found = true; break; }
-          if (i + 1 >= steps) break;
-          ++ns;
-          pos = p1;
-          if (v1 >= S.iso_n) { found = true; break; }
-          pos = pos + incTex;
+        for (int i = 0; i < steps && !found; i += ISO_BATCH) {
+          float v[ISO_BATCH];
+          sample_brick_iso_n<T>(bd, nx, ny, nz, edge, pos, incTex, v);
+#pragma unroll
+          for (int s = 0; s < ISO_BATCH; ++s) {
+            if (i + s >= steps) break;
+            ++ns;
+            if (v[s] >= S.iso_n) { found = true; break; }
+            pos = pos + incTex;
+          }
         }
         if (!found) continue;
         V3 rd = incTex / 2.0f;
