@@ -768,12 +768,13 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         // one sample at pos (the interior form when fast_c is true): shade,
         // UnderCompositing (Compositing.glsl:33-38), advance; false once the
         // segment ends ('a >= 0.99' or early ray termination)
-        auto step = [&](auto fast_c) -> bool {
+        // pre: the unlit 1D sample's value, formed by the caller (other modes sample here)
+        auto step = [&](auto fast_c, float pre = 0.0f) -> bool {
           ++ns;
           float s0, s1, s2, s3;
           if constexpr (MODE == 0 && !LIT) {
-            // no gradient: the 2x2x2 taps alone decide the fast form
-            const float val = sample_brick_iso<T>(bd, nx, ny, nz, edge, pos);
+            // no gradient: the 2x2x2 taps alone (sample_brick_iso_n) give the value
+            const float val = pre;
             const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
             const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
             const float4 tv = tf1[ti];
@@ -816,8 +817,19 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           for (; i < nsamp; ++i)
             if (!step(std::false_type{})) return false;
         } else {
-          for (int i = 0; i < nsamp; ++i)
-            if (!step(std::false_type{})) return false;
+          // ISO_BATCH samples' loads in flight together (positions accumulated
+          // one increment at a time, as step() advances them: x * 2^k is exact),
+          // then composited in order; values past a termination are not used
+          const V3 inc = mk(incTex.x * mlt, incTex.y * mlt, incTex.z * mlt);
+          for (int i = 0; i < nsamp; i += ISO_BATCH) {
+            float v[ISO_BATCH];
+            sample_brick_iso_n<T>(bd, nx, ny, nz, edge, pos, inc, v);
+#pragma unroll
+            for (int s = 0; s < ISO_BATCH; ++s) {
+              if (i + s >= nsamp) break;
+              if (!step(std::false_type{}, v[s])) return false;
+            }
+          }
         }
         return true;
       };
