@@ -275,7 +275,7 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
     orders = {"fixed": tune_dispatch_order(ren, views)[0]}
     fixed = ren.PaintViews(views)
     t_fixed = []
-    for _ in range(3):
+    for _ in range(5):
         ren.PaintViews(views, download=False)
         t_fixed.append(ren.last_stats.ms)
     def timed(prop, use_net=True, label=""):
@@ -283,7 +283,7 @@ def learned_leg(ren, views, cfg, train_views=48, seed=123):
         orders[label] = tune_dispatch_order(ren, views)[0]
         img = ren.PaintViews(views)
         ts = []
-        for _ in range(3):
+        for _ in range(5):
             ren.PaintViews(views, download=False)
             ts.append(ren.last_stats.ms)
         return img, float(np.median(ts))
