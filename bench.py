@@ -364,7 +364,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a single "
-              f"MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+              f"MI355X: several GPUs would each render their own views or bricks)", file=sys.stderr)
         sys.exit(2)
 
     from tuvok_amd import workloads

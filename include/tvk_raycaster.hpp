@@ -2,7 +2,8 @@
 //
 // tvk::Raycaster mirrors the part of tuvok::AbstrRenderer (Renderer/AbstrRenderer.h:112-881)
 // that drives a frame of GLRaycaster, with the same method names, so a Tuvok
-// renderer subclass forwards its virtuals one to one.
+// renderer subclass (an AbstrRenderer holding a tvk::Raycaster) forwards its
+// virtuals one to one.
 // Errors surface as tvk::Error exceptions on this (C++) side of the boundary,
 // like the std::runtime_error / std::domain_error the reference throws.
 #pragma once
