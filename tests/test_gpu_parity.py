@@ -381,6 +381,15 @@ def test_dispatch_orders_give_the_same_frames(ren, mode, lit, learned):
     assert ref.max() > 0.05 and ns > 0
     for o in settings[1:]:
         assert np.array_equal(got[o][0], ref) and got[o][1] == ns, o
+    if not learned:
+        # the tuner keeps one of the settings, and the frames stay the same
+        from tuvok_amd.autotune import DISPATCH_SETTINGS, tune_dispatch_order
+        best, times = tune_dispatch_order(ren, rots, reps=1)
+        try:
+            assert best in DISPATCH_SETTINGS and set(times) == set(DISPATCH_SETTINGS)
+            assert np.array_equal(ren.PaintViews(rots), ref)
+        finally:
+            ren.SetDispatchOrder(DISPATCH_RASTER | DISPATCH_QUEUES)
 
 
 def test_isosurface_early_termination_skips_bricks_behind_the_hit(ren):

@@ -96,7 +96,8 @@ def resident_and_budgeted(setup, budgets, frames):
 
 
 @pytest.mark.parametrize("mode,lit,eps", [(RM_1DTRANS, False, 0.0), (RM_1DTRANS, True, 1 / 512),
-                                          (RM_2DTRANS, True, 0.0), (RM_ISOSURFACE, False, 0.0)])
+                                          (RM_2DTRANS, True, 0.0), (RM_ISOSURFACE, False, 0.0),
+                                          (RM_ISOSURFACE, True, 1 / 512)])
 def test_budgeted_frames_equal_resident(mode, lit, eps):
     print(f"budget mode={mode} lit={lit} eps={eps}", flush=True)
 
@@ -125,7 +126,10 @@ def test_budgeted_frames_equal_resident(mode, lit, eps):
     for b, g, st in got:
         assert np.array_equal(g[0], want[0]), (b, float(np.abs(g[0] - want[0]).max()))
         assert np.array_equal(g[2], want[2]), b
-        assert g[1] == want[1] and g[3] == want[3], (b, g[1], want[1])
+        if not (mode == RM_ISOSURFACE and eps > 0):
+            # the isosurface's brick skip behind the hit knows only the hits of
+            # its own part, so a budgeted frame may take more samples
+            assert g[1] == want[1] and g[3] == want[3], (b, g[1], want[1])
         if b == 3 * slot:
             assert st.slots == 3 and st.parts > 1 and st.evictions > 0, (st.slots, st.parts, st.evictions)
 
