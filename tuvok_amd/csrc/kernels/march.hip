@@ -1199,7 +1199,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TVK_LEARN_W
 // all are drained.  Requests in one launch share MODE, LIT and T.  A cap of 3
 // waves per SIMD lets the lit variants keep their registers (at 4 the lit 2D
 // one spills 2 VGPRs and the learned one 41; measured 44.1 against 44.4 ms on
-// the 64 GiB LOD batch); the unlit ones need far fewer and run at more waves.
+// the 64 GiB LOD batch).  Capping only the learned variants at 3 lets the
+// fixed-step ones fit 4 waves with no spill, but that measured slower too
+// (64.5 against 61.6 ms on the 48-request batch, 41.5 against 40.4 ms on the
+// 64 GiB LOD batch).
 template <int MODE, bool LIT, typename T, bool COUNT, int LEARN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(
     BatchArgs B, const BatchRequest* __restrict__ req) {
