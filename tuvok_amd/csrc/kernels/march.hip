@@ -722,63 +722,60 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         const float mlt = (float)(1 << k);
         const float ssc = k == 0 ? sscale : sscale * mlt;
         V3 pos_s = pos;   // the position shade() samples
-        // TF, gradient and shading of the sample at pos_s (the cross sampler's two forms)
+        // 2D TF, gradient and shading of the sample at pos_s (the cross sampler's two forms)
         auto shade = [&](auto fast_c, float& s0, float& s1, float& s2, float& s3) {
           CrossSampler<T, decltype(fast_c)::value> cs;
           const float val = cs.value(bd, nx, ny, nz, edge, pos_s);
-          if (MODE == 0) {
-            const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
-            const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
-            const float4 tv = tf1[ti];
+          const float sx = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
+          const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
+          if (((cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
+            // every row of this value column is transparent: alpha 0 whatever the gradient
+            s0 = s1 = s2 = s3 = 0.0f;
+          } else {
+            const V3 gg = cs.gradient(vstep);
+            const float gm = sqrt_rn(dot(gg, gg));   // length(gg), bit for bit
+            const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
+            const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
+            const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];   // byte/255, precomputed
             s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
-            if (LIT && s3 != 0.0f) {   // a transparent sample composites to nothing: skip its shading
-              const V3 gg = cs.gradient(vstep);
+            if (LIT) {
               const V3 nn = normal_eye(gg, nm, dsc);
               const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
-              s0 = lc.x; s1 = lc.y; s2 = lc.z;
-            }
-            const float tt = 1.0f - s3;
-            s3 = 1.0f - opacity_pow(tt, ssc);
-          } else {
-            const float sx = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
-            const int xi = min((int)floorf(sx * (float)S.tf_w), S.tf_w - 1);
-            if (((cols[xi >> 6] >> (xi & 63)) & 1ull) == 0) {
-              // every row of this value column is transparent: alpha 0 whatever the gradient
-              s0 = s1 = s2 = s3 = 0.0f;
+              const float tt = 1.0f - s3;
+              s3 = 1.0f - opacity_pow(tt, ssc);
+              s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
             } else {
-              const V3 gg = cs.gradient(vstep);
-              const float gm = sqrt_rn(dot(gg, gg));   // length(gg), bit for bit
-              const float sy = fminf(fmaxf(1.0f - gm * S.grad_scale, 0.0f), 1.0f);
-              const int yi = min((int)floorf(sy * (float)S.tf_h), S.tf_h - 1);
-              const float4 tv = S.tf2f[(size_t)yi * (size_t)S.tf_w + (size_t)xi];   // byte/255, precomputed
-              s0 = tv.x; s1 = tv.y; s2 = tv.z; s3 = tv.w;
-              if (LIT) {
-                const V3 nn = normal_eye(gg, nm, dsc);
-                const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
-                const float tt = 1.0f - s3;
-                s3 = 1.0f - opacity_pow(tt, ssc);
-                s0 = clamp01(lc.x); s1 = clamp01(lc.y); s2 = clamp01(lc.z); s3 = clamp01(s3);
-              } else {
-                const float tt = 1.0f - s3;
-                s3 = 1.0f - opacity_pow(tt, ssc);
-              }
+              const float tt = 1.0f - s3;
+              s3 = 1.0f - opacity_pow(tt, ssc);
             }
           }
         };
         // one sample at pos (the interior form when fast_c is true): shade,
         // UnderCompositing (Compositing.glsl:33-38), advance; false once the
         // segment ends ('a >= 0.99' or early ray termination)
-        // pre: the unlit 1D sample's value, formed by the caller (other modes sample here)
+        // pre: the 1D sample's value, formed by the caller from the 2x2x2 taps
+        // (sample_brick_iso_n), except in the lit chunked march (PRE1D false),
+        // which samples one value at a time: batches there cost it spills
+        constexpr bool PRE1D = MODE == 0 && (!LIT || LEARN == 0);
         auto step = [&](auto fast_c, float pre = 0.0f) -> bool {
           ++ns;
           float s0, s1, s2, s3;
-          if constexpr (MODE == 0 && !LIT) {
-            // no gradient: the 2x2x2 taps alone (sample_brick_iso_n) give the value
-            const float val = pre;
+          if constexpr (MODE == 0) {
+            CrossSampler<T, decltype(fast_c)::value> cs;
+            const float val = PRE1D ? pre : cs.value(bd, nx, ny, nz, edge, pos);
             const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
             const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
             const float4 tv = tf1[ti];
             s0 = tv.x; s1 = tv.y; s2 = tv.z;
+            if (LIT && tv.w != 0.0f) {
+              // a transparent sample composites to nothing: only the others
+              // take the gradient (the batched march loads their cross here)
+              if (PRE1D) (void)cs.value(bd, nx, ny, nz, edge, pos);
+              const V3 gg = cs.gradient(vstep);
+              const V3 nn = normal_eye(gg, nm, dsc);
+              const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
+              s0 = lc.x; s1 = lc.y; s2 = lc.z;
+            }
             s3 = 1.0f - opacity_pow(1.0f - tv.w, ssc);
           } else {
             pos_s = pos;
@@ -797,7 +794,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           pos = mk(fmaf(incTex.x, mlt, pos.x), fmaf(incTex.y, mlt, pos.y), fmaf(incTex.z, mlt, pos.z));
           return true;
         };
-        if constexpr (!(MODE == 0 && !LIT)) {
+        if constexpr (MODE != 0 || LIT) {
           // Each lane marches its samples before the segment's interior range
           // in the general form, the interior range in the interior form, and
           // the rest in the general form again, so that only the few samples
@@ -810,6 +807,29 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           const int j_hi = min(in_hi - i0 >= 0 ? (in_hi - i0) >> k : -1, nsamp - 1);
           const int h_end = j_hi >= j_lo ? j_lo : nsamp;
           int i = 0;
+          if constexpr (PRE1D) {
+            // lit 1D: each phase takes ISO_BATCH samples' values at once (as
+            // below); a batch that runs past the phase's end discards the rest.
+            // One step body (not unrolled) takes the values in turn, shifted
+            // down, so that the shading's registers are not held four times.
+            const V3 inc = mk(incTex.x * mlt, incTex.y * mlt, incTex.z * mlt);
+            auto phase = [&](auto fast_c, int end) -> bool {
+              while (i < end) {
+                float v[ISO_BATCH];
+                sample_brick_iso_n<T>(bd, nx, ny, nz, edge, pos, inc, v);
+                const int e = min(end, i + ISO_BATCH);
+#pragma nounroll
+                for (; i < e; ++i) {
+                  if (!step(fast_c, v[0])) return false;
+#pragma unroll
+                  for (int s = 0; s + 1 < ISO_BATCH; ++s) v[s] = v[s + 1];
+                }
+              }
+              return true;
+            };
+            return phase(std::false_type{}, h_end) && phase(std::true_type{}, j_hi + 1) &&
+                   phase(std::false_type{}, nsamp);
+          }
           for (; i < h_end; ++i)
             if (!step(std::false_type{})) return false;
           for (; i <= j_hi; ++i)
