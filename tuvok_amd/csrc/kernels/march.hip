@@ -290,19 +290,19 @@ __device__ __forceinline__ float sample_brick_iso(const T* __restrict__ d, int n
 #define TVK_ISO_BATCH 4
 #endif
 constexpr int ISO_BATCH = TVK_ISO_BATCH;
-template <typename T>
+template <typename T, int N = ISO_BATCH>
 __device__ __forceinline__ void sample_brick_iso_n(const T* __restrict__ d, int nx, int ny, int nz, bool edge, V3 p,
-                                                   V3 inc, float (&val)[ISO_BATCH]) {
-  V3 ps[ISO_BATCH];
+                                                   V3 inc, float (&val)[N]) {
+  V3 ps[N];
   ps[0] = p;
 #pragma unroll
-  for (int s = 1; s < ISO_BATCH; ++s) ps[s] = ps[s - 1] + inc;
-  int i0[ISO_BATCH][3];
-  float f[ISO_BATCH][3];
+  for (int s = 1; s < N; ++s) ps[s] = ps[s - 1] + inc;
+  int i0[N][3];
+  float f[N][3];
   const int n[3] = {nx, ny, nz};
   bool in = true;
 #pragma unroll
-  for (int s = 0; s < ISO_BATCH; ++s) {
+  for (int s = 0; s < N; ++s) {
     const float pc[3] = {ps[s].x, ps[s].y, ps[s].z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -314,14 +314,14 @@ __device__ __forceinline__ void sample_brick_iso_n(const T* __restrict__ d, int 
   }
   if (__ballot(!in) != 0) {
 #pragma unroll
-    for (int s = 0; s < ISO_BATCH; ++s) val[s] = sample_brick_iso<T>(d, nx, ny, nz, edge, ps[s]);
+    for (int s = 0; s < N; ++s) val[s] = sample_brick_iso<T>(d, nx, ny, nz, edge, ps[s]);
     return;
   }
   const uint32_t nbx = nblocks((uint32_t)nx, Blk<T>::LX), nbxy = nbx * nblocks((uint32_t)ny, 2);
   const uint8_t* __restrict__ b = reinterpret_cast<const uint8_t*>(d);
-  uint32_t v[ISO_BATCH][2][2][2];
+  uint32_t v[N][2][2][2];
 #pragma unroll
-  for (int s = 0; s < ISO_BATCH; ++s) {
+  for (int s = 0; s < N; ++s) {
     const uint32_t x0 = ox<T>(i0[s][0]);
 #pragma unroll
     for (int j = 0; j < 2; ++j)
@@ -330,7 +330,7 @@ __device__ __forceinline__ void sample_brick_iso_n(const T* __restrict__ d, int 
         run2<T>(b, (x0 + oy<T>(i0[s][1] + j, nbx) + oz<T>(i0[s][2] + k, nbxy)) * (uint32_t)sizeof(T), v[s][j][k]);
   }
 #pragma unroll
-  for (int s = 0; s < ISO_BATCH; ++s) {
+  for (int s = 0; s < N; ++s) {
     float c[2][2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j)
@@ -754,15 +754,13 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         // UnderCompositing (Compositing.glsl:33-38), advance; false once the
         // segment ends ('a >= 0.99' or early ray termination)
         // pre: the 1D sample's value, formed by the caller from the 2x2x2 taps
-        // (sample_brick_iso_n), except in the lit chunked march (PRE1D false),
-        // which samples one value at a time: batches there cost it spills
-        constexpr bool PRE1D = MODE == 0 && (!LIT || LEARN == 0);
+        // (sample_brick_iso_n); the 2D modes sample here
         auto step = [&](auto fast_c, float pre = 0.0f) -> bool {
           ++ns;
           float s0, s1, s2, s3;
           if constexpr (MODE == 0) {
             CrossSampler<T, decltype(fast_c)::value> cs;
-            const float val = PRE1D ? pre : cs.value(bd, nx, ny, nz, edge, pos);
+            const float val = pre;
             const float cc = fminf(fmaxf(val * S.tf_scale, 0.0f), 1.0f);
             const int ti = min((int)floorf(cc * (float)S.tf_n), S.tf_n - 1);
             const float4 tv = tf1[ti];
@@ -770,7 +768,7 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
             if (LIT && tv.w != 0.0f) {
               // a transparent sample composites to nothing: only the others
               // take the gradient (the batched march loads their cross here)
-              if (PRE1D) (void)cs.value(bd, nx, ny, nz, edge, pos);
+              (void)cs.value(bd, nx, ny, nz, edge, pos);
               const V3 gg = cs.gradient(vstep);
               const V3 nn = normal_eye(gg, nm, dsc);
               const V3 lc = lighting(cur, nn, amb, dif * mk(s0, s1, s2), spc, ldir);
@@ -807,22 +805,25 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
           const int j_hi = min(in_hi - i0 >= 0 ? (in_hi - i0) >> k : -1, nsamp - 1);
           const int h_end = j_hi >= j_lo ? j_lo : nsamp;
           int i = 0;
-          if constexpr (PRE1D) {
+          if constexpr (MODE == 0) {
             // lit 1D: each phase takes ISO_BATCH samples' values at once (as
             // below); a batch that runs past the phase's end discards the rest.
             // One step body (not unrolled) takes the values in turn, shifted
             // down, so that the shading's registers are not held four times.
+            // The chunked march takes two at a time: four cost it 8-12 spilled
+            // VGPRs (two: 6-10) against 1-4 sampling one at a time.
             const V3 inc = mk(incTex.x * mlt, incTex.y * mlt, incTex.z * mlt);
+            constexpr int NB = LEARN == 0 ? ISO_BATCH : 2;
             auto phase = [&](auto fast_c, int end) -> bool {
               while (i < end) {
-                float v[ISO_BATCH];
-                sample_brick_iso_n<T>(bd, nx, ny, nz, edge, pos, inc, v);
-                const int e = min(end, i + ISO_BATCH);
+                float v[NB];
+                sample_brick_iso_n<T, NB>(bd, nx, ny, nz, edge, pos, inc, v);
+                const int e = min(end, i + NB);
 #pragma nounroll
                 for (; i < e; ++i) {
                   if (!step(fast_c, v[0])) return false;
 #pragma unroll
-                  for (int s = 0; s + 1 < ISO_BATCH; ++s) v[s] = v[s + 1];
+                  for (int s = 0; s + 1 < NB; ++s) v[s] = v[s + 1];
                 }
               }
               return true;
