@@ -840,7 +840,9 @@ __device__ __forceinline__ void march_tile(const ShadeConst& S, const float4* __
         } else {
           // ISO_BATCH samples' loads in flight together (positions accumulated
           // one increment at a time, as step() advances them: x * 2^k is exact),
-          // then composited in order; values past a termination are not used
+          // then composited in order; values past a termination are not used.
+          // Eight at a time measured slower on the 48-request batch (69.7
+          // against 61.7 ms: 167 against 110 VGPRs, the same 3 waves).
           const V3 inc = mk(incTex.x * mlt, incTex.y * mlt, incTex.z * mlt);
           for (int i = 0; i < nsamp; i += ISO_BATCH) {
             float v[ISO_BATCH];
