@@ -345,7 +345,9 @@ int tvk_plan(tvk_ctx* ctx, tvk_brick_rec* out, int cap, int* n);
 int tvk_render(tvk_ctx* ctx, float* host_out, tvk_stats* stats);
 /* Render n views (rotation matrices rots[n*16]) of the current dataset, TF and
  * camera in one kernel launch on the context stream; host_out (may be NULL)
- * receives n frames. */
+ * receives n frames.  With host_out and stats both NULL (also in tvk_render) the
+ * call returns once the frame is queued on the context stream, so the host plans
+ * the next frame while this one runs; tvk_synchronize waits for it. */
 int tvk_render_views(tvk_ctx* ctx, int n, const float* rots, float* host_out, tvk_stats* stats);
 /* Enable/disable the per-frame sample counter and touched-brick record (off by
  * default: costs an atomic per wave). */

@@ -694,3 +694,22 @@ def test_odd_cores_with_clamp_are_refused(ren):
     vol = np.zeros((20, 20, 20), np.uint8)
     with pytest.raises(Exception):
         ren.LoadVolume(vol, (11, 11, 11), 1, clamp_to_edge=True, max_lods=0)
+
+
+@pytest.mark.gpu
+def test_queued_renders_give_the_same_frames(ren):
+    """Renders that return once queued (no download, no stats) leave the frames
+    a waiting render gives: the next call's host planning and staging copies do
+    not disturb the frame still on the GPU."""
+    setup_blobs(ren)
+    views = [rotation_y(a) for a in (0.0, 0.7, 1.9, 3.1)]
+    other = [rotation_y(a) for a in (0.3, 1.1, 2.5, 4.0)]
+    ren.PaintViews(views)
+    ref = ren.Display()
+    for _ in range(3):
+        ren.PaintViews(other, download=False, stats=False)
+        ren.PaintViews(views, download=False, stats=False)
+    assert np.array_equal(ren.Display(), ref)
+    ren.PaintViews(other, download=False, stats=False)
+    ren.Synchronize()
+    assert np.array_equal(ren.PaintViews(other), ren.PaintViews(other))

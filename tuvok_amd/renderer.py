@@ -422,12 +422,16 @@ class HIPRaycaster:
         self._last_views = 1
         return out
 
-    def PaintViews(self, rotations, download=True):
+    def PaintViews(self, rotations, download=True, stats=True):
+        """One launch over every rotation; returns the frames (download) or None.
+        With download and stats both False the call returns once the frame is
+        queued (last_stats is left as it was): the next call plans while this
+        frame runs, and Synchronize() waits for it."""
         rots = np.ascontiguousarray(np.stack([np.asarray(r, np.float32).reshape(4, 4) for r in rotations]))
         n = rots.shape[0]
         out = np.empty((n, self._h_px, self._w, 4), np.float32) if download else None
         self._chk(self._L.tvk_render_views(self._h, n, _fp(rots), _fp(out) if download else None,
-                                           C.byref(self.last_stats)))
+                                           C.byref(self.last_stats) if (stats or download) else None))
         self._last_views = n
         return out
 
